@@ -1,0 +1,151 @@
+/*
+ * ac_mi355x.h — C ABI of the MI355X ApplicationClassification matcher.
+ *
+ * The matcher scores a data graph against a pattern graph and produces, for every pattern vertex p and data
+ * vertex v, the log-probability MU[v][p] that v plays the role of p.  Every entry point here replaces one piece of
+ * the reference's own interface; each declaration cites the reference file:line it replaces.  There is no other
+ * FFI in the reference: its only caller of the ac:: operators is its driver (src/main.cu:66-257).
+ *
+ * Conventions
+ *  - Every function returns an int status: AC_OK (0) or one of the AC_ERR_* codes.  No exception crosses the ABI.
+ *    ac_last_error(ctx) (or ac_global_error() for calls without a context) returns a NUL-terminated message.
+ *  - Values are FP32, vertex and edge ids are int32, sizes are int64.  The reference used FP64/uint64
+ *    (src/helpers.hxx:3-4); the FP32 path is checked against an FP64 restatement at rtol 1e-5 / atol 1e-8.
+ *  - "stream" arguments are hipStream_t passed as void* (NULL = default stream).  Device pointers are plain
+ *    device addresses (hipMalloc, or a torch.cuda tensor's data_ptr()).
+ *  - One host thread per context; contexts are independent.
+ */
+#ifndef AC_MI355X_H
+#define AC_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AC_OK 0
+#define AC_ERR_ARG 1   /* bad argument: null pointer, size mismatch, id out of range, ...          */
+#define AC_ERR_OOM 2   /* device or host allocation failed                                        */
+#define AC_ERR_HIP 3   /* a HIP runtime call failed                                                */
+#define AC_ERR_STATE 4 /* call out of order, e.g. ac_run before both graphs are set                 */
+#define AC_ERR_IO 5    /* file could not be opened, is truncated or malformed                      */
+#define AC_ERR_NODEV 6 /* no HIP device / the requested device does not exist                      */
+
+#define AC_DATA 0
+#define AC_PATTERN 1
+
+/* -------------------------------------------------------------------------------------------------------------
+ * Host-side graph (the reference's Graph, src/main.cu:8-19, with FP32 features and int32 ids).
+ * Arrays are row-major: node_feats[num_nodes * node_feat_dim], edge_feats[num_edges * edge_feat_dim].
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct ac_host_graph {
+  int64_t num_nodes;
+  int64_t node_feat_dim;
+  int64_t num_edges;
+  int64_t edge_feat_dim;
+  float* node_feats;
+  float* edge_feats;
+  int32_t* srcs;
+  int32_t* dsts;
+} ac_host_graph;
+
+/* Replaces loadGraph (src/main.cu:21-63): reads the binary layout written by prob2bin.py:46-64
+ * (uint64 {nV, dV, nE, dE}, f64 vertex features, f64 edge features, uint64 srcs, uint64 dsts).
+ * Unlike the reference it validates sizes and ids.  Arrays are malloc'd; release with ac_graph_free. */
+int ac_graph_load_bin(const char* path, ac_host_graph* out);
+/* Writes the same binary layout (the output of prob2bin.py:46-64). */
+int ac_graph_save_bin(const char* path, const ac_host_graph* g);
+/* Reads the CSV pair prob2bin.py:11-39 consumes: header line "N C" skipped; vertex rows "id f1..fd" (id must
+ * equal the row index, prob2bin.py:28-29); edge rows "src dst f1..fd".  Whitespace separated, CRLF tolerated. */
+int ac_graph_load_csv(const char* vertex_csv, const char* edge_csv, ac_host_graph* out);
+/* Writes that CSV pair (values printed with 17 significant digits, so a float round-trips exactly). */
+int ac_graph_save_csv(const char* vertex_csv, const char* edge_csv, const ac_host_graph* g);
+void ac_graph_free(ac_host_graph* g);
+
+/* -------------------------------------------------------------------------------------------------------------
+ * Synthetic inputs (no counterpart in the reference, whose datasets are absent; these make its configurations).
+ * All generators are deterministic functions of their seed and use every host core.
+ * ------------------------------------------------------------------------------------------------------------- */
+/* Directed Erdős–Rényi G(n, m): m distinct (src, dst) pairs, no self-loops, sorted by (src, dst). */
+int ac_gen_er(int64_t n, int64_t m, uint64_t seed, int32_t* srcs, int32_t* dsts);
+/* R-MAT with 2^scale vertices and edge_factor * 2^scale draws, quadrant probabilities (a, b, c, 1-a-b-c);
+ * self-loops and duplicates removed; sorted by (src, dst).  *srcs / *dsts are malloc'd (free with ac_free). */
+int ac_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int64_t* num_edges,
+                int32_t** srcs, int32_t** dsts);
+/* out[i] = sigma * z_i, z_i ~ N(0,1) from a counter-based generator keyed by (seed, stream, i). */
+int ac_gen_normal(float* out, int64_t count, uint64_t seed, uint64_t stream, float sigma);
+/* Planted pattern: a connected subgraph of the data graph with n_pv vertices and n_pe of the edges they induce.
+ * Outputs pattern edges (p_srcs/p_dsts in pattern ids), vmap[p] = data vertex of pattern vertex p and
+ * emap[q] = data edge index of pattern edge q.  Returns AC_ERR_ARG if no such subgraph was found. */
+int ac_gen_pattern(int64_t n, int64_t m, const int32_t* srcs, const int32_t* dsts, int n_pv, int n_pe,
+                   uint64_t seed, int32_t* p_srcs, int32_t* p_dsts, int32_t* vmap, int64_t* emap);
+void ac_free(void* p);
+
+/* -------------------------------------------------------------------------------------------------------------
+ * Matcher context: the driver's buffers and its prep + loop (src/main.cu:80-247) on one GPU.
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct ac_ctx ac_ctx;
+
+int ac_create(ac_ctx** out, int device);
+int ac_destroy(ac_ctx* ctx);
+const char* ac_last_error(const ac_ctx* ctx);
+const char* ac_global_error(void);
+
+/* Replaces the H2D half of loadGraph (src/main.cu:49-62) for which = AC_DATA or AC_PATTERN.  Host arrays are copied;
+ * the context keeps the device copies plus the CSR/CSC views it builds from them.  dV/dE of the two graphs must
+ * agree (cdist compares them, src/ac.hxx:33-48). */
+int ac_set_graph(ac_ctx* ctx, int which, int64_t num_nodes, int64_t node_feat_dim, const float* node_feats,
+                 int64_t num_edges, int64_t edge_feat_dim, const float* edge_feats, const int32_t* srcs,
+                 const int32_t* dsts);
+
+/* Runs prep (src/main.cu:107-148) and n_iter iterations of the loop (src/main.cu:160-245); n_iter < 0 means the
+ * number of pattern vertices, as src/main.cu:160.  Synchronous. */
+int ac_run(ac_ctx* ctx, int n_iter);
+/* Enqueues the same work on the context's stream and returns without waiting (for timing loops). */
+int ac_run_async(ac_ctx* ctx, int n_iter);
+int ac_sync(ac_ctx* ctx);
+
+/* Replaces ac::transpose + the D2H copy (src/main.cu:247,255-256): out[v * pn + p] = MU[p][v]. */
+int ac_get_mu(ac_ctx* ctx, float* host_out);
+/* Thresholded match matrix (no counterpart in the reference, which prints MU only): out[v * pn + p] = 1 if v is
+ * the first data vertex attaining the column maximum of MU[.][p], or 2 if additionally MU[v][p] >= log_tau;
+ * 0 otherwise. */
+int ac_get_match(ac_ctx* ctx, float log_tau, uint8_t* host_out);
+/* Sizes {dn, de, pn, pe}. */
+int ac_get_sizes(const ac_ctx* ctx, int64_t* out4);
+/* Device time of the last run, from HIP events: prep (src/main.cu:107-148) and loop (src/main.cu:160-245). */
+int ac_get_timing(const ac_ctx* ctx, double* prep_ms, double* loop_ms);
+/* Device stream the context runs on (a hipStream_t). */
+void* ac_stream(ac_ctx* ctx);
+
+/* -------------------------------------------------------------------------------------------------------------
+ * Stage operators on caller-owned device buffers: the ac:: operator library (src/ac.hxx) with FP32 values and
+ * int32 ids, same argument meaning and pattern-major layouts.  Asynchronous on `stream`.
+ * ------------------------------------------------------------------------------------------------------------- */
+/* ac::cdist (src/ac.hxx:33-56): out[j * n_b + i] = || a_j - b_i ||_2 for j < n_a, i < n_b. */
+int ac_cdist(int64_t n_a, int64_t n_b, int64_t dim, const float* feats_a, const float* feats_b, float* out,
+             void* stream);
+/* ac::RowSoftmax2 (src/ac.hxx:77-113): in-place log-softmax of each row of an n_row x n_col matrix, computed
+ * max-stabilised: x - max - log1p(sum_{k != argmax} exp(x_k - max)). */
+int ac_row_log_softmax(int64_t n_row, int64_t n_col, float* x, void* stream);
+/* ac::RowMax2 (src/ac.hxx:58-75): out[r] = max_c in[r * n_col + c]. */
+int ac_row_max(int64_t n_row, int64_t n_col, const float* in, float* out, void* stream);
+/* ac::EdgeMaxReduce2_t (src/ac.hxx:115-149): xmax[r * n_col_out + v] = max(vymax[r],
+ * max_{c : nodes[c] = v} xe[r * n_col_in + c]).  Exact (bit-identical for any execution order). */
+int ac_edge_max_reduce(int64_t n_col_in, int64_t n_col_out, int64_t n_row, const float* vymax, const float* xe,
+                       float* xmax, const int32_t* nodes, void* stream);
+/* ac::ComputeMU2_t (src/ac.hxx:151-186): mu[p * n_col_out + v] = -cv[p * n_col_out + v]
+ *   + sum_{q : dsts[q] = p} fmax[q * n_col_in + v] + sum_{q : srcs[q] = p} rmax[q * n_col_in + v],
+ * summed in ascending q (fmax term before rmax term), so the result is deterministic. */
+int ac_compute_mu(int64_t n_col_in, int64_t n_row_in, int64_t n_col_out, int64_t n_row_out, const float* cv,
+                  const float* fmax, const float* rmax, const int32_t* srcs, const int32_t* dsts, float* mu,
+                  void* stream);
+/* ac::transpose (src/ac.hxx:17-31): out[c * n_rows + r] = in[r * n_cols + c]. */
+int ac_transpose(const float* in, float* out, int64_t n_rows, int64_t n_cols, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AC_MI355X_H */

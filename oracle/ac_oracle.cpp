@@ -1,0 +1,277 @@
+// ORACLE — test infrastructure only.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+// this library, and only as the checker or the timed CPU baseline; the product path never calls it.
+//
+// A CPU restatement of the reference matcher, op by op, with the reference's pattern-major buffers materialised
+// exactly as its driver allocates them (src/main.cu:80-99,153-158).  Real = double is the parity oracle (the
+// reference computes in double, src/helpers.hxx:4); Real = float is kept for precision studies only.
+//
+// Decisions where the reference is ill-defined (each one is a flag, default = the decision):
+//  * prep negates the FULL pe x de edge-cost matrix into RE/FE.  The reference negates only its first pn x dn
+//    entries (src/main.cu:113-114) and leaves the rest of the cudaMalloc'd buffer undefined; OR_COMPAT_ZERO_TAIL
+//    emulates a zero-filled tail instead.
+//  * log-softmax is max-stabilised, x - (max + log sum exp(x - max)).  The reference evaluates
+//    log(exp(x)) - log(sum exp(x)) (src/ac.hxx:94-105); OR_LITERAL_LSM evaluates exactly that.
+//  * MU accumulates in ascending pattern-edge order, the FMax term before the RMax term of each edge; the
+//    reference uses atomicAdd in block-schedule order (src/ac.hxx:164-170).
+//  * every reduction is blocked in a fixed order, so results do not depend on the thread count.
+//
+// Parity status of each function (what pins it) is stated beside it and checked by tests/test_oracle_pins.py.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+extern "C" {
+
+#define OR_COMPAT_ZERO_TAIL 1u
+#define OR_LITERAL_LSM 2u
+
+typedef struct or_graph {
+  int64_t nV, dV, nE, dE;
+  const double* vfeat;  // nV x dV
+  const double* efeat;  // nE x dE
+  const int64_t* src;   // nE
+  const int64_t* dst;   // nE
+} or_graph;
+
+}  // extern "C"
+
+namespace {
+
+constexpr int64_t kBlock = 4096;  // fixed reduction block, independent of the thread count
+
+// ac::cdist (src/ac.hxx:33-56): out[j * n_b + i] = sqrt(sum_k (a_j[k] - b_i[k])^2), a = pattern, b = data.
+// Pinned by: scipy.spatial.distance.cdist on random inputs (tests/test_oracle_pins.py::test_cdist_*).
+template <typename R>
+void cdist(int64_t n_a, int64_t n_b, int64_t dim, const double* a, const double* b, R* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n_b; ++i)
+    for (int64_t j = 0; j < n_a; ++j) {
+      R acc = 0;
+      for (int64_t k = 0; k < dim; ++k) {
+        R d = (R)a[j * dim + k] - (R)b[i * dim + k];
+        acc += d * d;
+      }
+      out[j * n_b + i] = std::sqrt(acc);
+    }
+}
+
+// Row maximum (ac::RowMax2, src/ac.hxx:58-75).  Pinned by: numpy max (exact).
+template <typename R>
+R row_max(const R* x, int64_t n) {
+  int64_t nb = (n + kBlock - 1) / kBlock;
+  std::vector<R> part(std::max<int64_t>(nb, 1), -std::numeric_limits<R>::infinity());
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < nb; ++b) {
+    R m = -std::numeric_limits<R>::infinity();
+    for (int64_t i = b * kBlock; i < std::min(n, (b + 1) * kBlock); ++i) m = std::max(m, x[i]);
+    part[b] = m;
+  }
+  R m = -std::numeric_limits<R>::infinity();
+  for (R v : part) m = std::max(m, v);
+  return m;
+}
+
+template <typename R, typename F>
+R row_sum(int64_t n, F term) {
+  int64_t nb = (n + kBlock - 1) / kBlock;
+  std::vector<R> part(std::max<int64_t>(nb, 1), 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < nb; ++b) {
+    R s = 0;
+    for (int64_t i = b * kBlock; i < std::min(n, (b + 1) * kBlock); ++i) s += term(i);
+    part[b] = s;
+  }
+  R s = 0;
+  for (R v : part) s += v;
+  return s;
+}
+
+// ac::RowSoftmax2 / RowSoftmax2_prealloc (src/ac.hxx:77-113): in-place log-softmax of each row.
+// Pinned by: scipy.special.log_softmax; sum(exp(row)) == 1; shift invariance; n_col == 1 gives 0.
+template <typename R>
+void row_log_softmax(int64_t n_row, int64_t n_col, R* x, bool literal) {
+  for (int64_t r = 0; r < n_row; ++r) {
+    R* row = x + r * n_col;
+    if (n_col == 0) continue;
+    if (literal) {
+      // exp, reduce_by_key(+), log of the sum, log of each exp, subtract (src/ac.hxx:94-105).
+      R s = row_sum<R>(n_col, [&](int64_t i) { return std::exp(row[i]); });
+      R ls = std::log(s);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n_col; ++i) row[i] = std::log(std::exp(row[i])) - ls;
+    } else {
+      // Max-stabilised, and the sum excludes one maximal element so that log-probabilities near 0 keep their
+      // relative accuracy: x - max - log1p(sum_{k != argmax} exp(x_k - max)).
+      R m = row_max(row, n_col);
+      int64_t am = 0;
+      while (row[am] != m) ++am;
+      R t = row_sum<R>(n_col, [&](int64_t i) { return i == am ? R(0) : std::exp(row[i] - m); });
+      R l = std::log1p(t);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n_col; ++i) row[i] = (row[i] - m) - l;
+    }
+  }
+}
+
+// ac::EdgeMaxReduce2_t (src/ac.hxx:115-149): fill xmax[r][v] = vymax[r], then xmax[r][nodes[c]] =
+// max(xmax[r][nodes[c]], xe[r][c]).  Pinned by: numpy.maximum.at (exact).
+template <typename R>
+void edge_max_reduce(int64_t n_col_in, int64_t n_col_out, int64_t n_row, const R* vymax, const R* xe, R* xmax,
+                     const int64_t* nodes) {
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n_row; ++r) {
+    R* out = xmax + r * n_col_out;
+    for (int64_t v = 0; v < n_col_out; ++v) out[v] = vymax[r];
+    const R* in = xe + r * n_col_in;
+    for (int64_t c = 0; c < n_col_in; ++c) out[nodes[c]] = std::max(out[nodes[c]], in[c]);
+  }
+}
+
+// ac::ComputeMU2_t (src/ac.hxx:151-186): mu = -cv, then mu[dsts[q]][v] += fmax[q][v], mu[srcs[q]][v] += rmax[q][v],
+// in ascending q.  Pinned by: numpy loop in the same order (the sum is a fixed sequence of adds).
+template <typename R>
+void compute_mu(int64_t n_col_in, int64_t n_row_in, int64_t n_col_out, int64_t n_row_out, const R* cv,
+                const R* fmax, const R* rmax, const int64_t* srcs, const int64_t* dsts, R* mu) {
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < n_col_out; ++v) {
+    for (int64_t p = 0; p < n_row_out; ++p) mu[p * n_col_out + v] = -cv[p * n_col_out + v];
+    for (int64_t q = 0; q < n_row_in; ++q) {
+      mu[dsts[q] * n_col_out + v] += fmax[q * n_col_in + v];
+      mu[srcs[q] * n_col_out + v] += rmax[q * n_col_in + v];
+    }
+  }
+}
+
+template <typename R>
+void row_max_all(int64_t n_row, int64_t n_col, const R* x, R* out) {
+  for (int64_t r = 0; r < n_row; ++r) out[r] = n_col ? row_max(x + r * n_col, n_col) : -std::numeric_limits<R>::infinity();
+}
+
+// The driver's prep + loop + transpose (src/main.cu:80-247) on pattern-major buffers.
+template <typename R>
+int run(const or_graph* D, const or_graph* P, int n_iter, unsigned flags, double* mu_out, double* fmax_out,
+        double* rmax_out, double* elapsed_s) {
+  const int64_t dn = D->nV, de = D->nE, pn = P->nV, pe = P->nE;
+  if (D->dV != P->dV || D->dE != P->dE || dn < 1 || pn < 1) return 1;
+  const bool literal = flags & OR_LITERAL_LSM;
+  if (n_iter < 0) n_iter = (int)pn;  // src/main.cu:160
+
+  // Buffers of src/main.cu:85-99.
+  std::vector<R> CV(pn * dn), MU(pn * dn), CE(pe * de), RE(pe * de), FE(pe * de);
+  std::vector<R> VR(pe * dn), VF(pe * dn), RMax(pe * dn), FMax(pe * dn), VFmax(pe), VRmax(pe);
+  auto t0 = std::chrono::steady_clock::now();
+
+  // prep (src/main.cu:107-148)
+  cdist<R>(pn, dn, D->dV, P->vfeat, D->vfeat, CV.data());
+  cdist<R>(pe, de, D->dE, P->efeat, D->efeat, CE.data());
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < pn * dn; ++i) MU[i] = -CV[i];
+  const int64_t neg_len = (flags & OR_COMPAT_ZERO_TAIL) ? std::min(pn * dn, pe * de) : pe * de;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < pe * de; ++i) {
+    RE[i] = i < neg_len ? -CE[i] : R(0);
+    FE[i] = RE[i];
+  }
+  row_log_softmax<R>(pn, dn, CV.data(), literal);
+  row_log_softmax<R>(pn, dn, MU.data(), literal);
+  row_log_softmax<R>(pe, de, CE.data(), literal);
+  row_log_softmax<R>(pe, de, RE.data(), literal);
+  row_log_softmax<R>(pe, de, FE.data(), literal);
+  // init_VX (src/main.cu:122-133)
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < pe * dn; ++i) {
+    int64_t v = i % dn, q = i / dn;
+    VR[i] = MU[dn * P->src[q] + v];
+    VF[i] = MU[dn * P->dst[q] + v];
+  }
+  row_max_all<R>(pe, dn, VF.data(), VFmax.data());
+  row_max_all<R>(pe, dn, VR.data(), VRmax.data());
+  edge_max_reduce<R>(de, dn, pe, VFmax.data(), RE.data(), RMax.data(), D->src);
+  edge_max_reduce<R>(de, dn, pe, VRmax.data(), FE.data(), FMax.data(), D->dst);
+
+  // loop (src/main.cu:160-245)
+  for (int it = 0; it < n_iter; ++it) {
+    // step1: update_VX (src/main.cu:165-176)
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < pe * dn; ++i) {
+      int64_t q = i / dn, v = i % dn;
+      VF[i] = MU[dn * P->dst[q] + v] - FMax[i];
+      VR[i] = MU[dn * P->src[q] + v] - RMax[i];
+    }
+    // step2: update_XE, both gathers at the edge SOURCE (src/main.cu:181-192)
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < pe * de; ++i) {
+      int64_t q = i / de, e = i % de;
+      FE[i] = VR[dn * q + D->src[e]] - CE[i];
+      RE[i] = VF[dn * q + D->src[e]] - CE[i];
+    }
+    // step3: RowMax2(VF), RowSoftmax2(RE), EdgeMaxReduce2_t by source (src/main.cu:197-211)
+    row_max_all<R>(pe, dn, VF.data(), VFmax.data());
+    row_log_softmax<R>(pe, de, RE.data(), literal);
+    edge_max_reduce<R>(de, dn, pe, VFmax.data(), RE.data(), RMax.data(), D->src);
+    // step4: the same for VR / FE by destination (src/main.cu:216-224)
+    row_max_all<R>(pe, dn, VR.data(), VRmax.data());
+    row_log_softmax<R>(pe, de, FE.data(), literal);
+    edge_max_reduce<R>(de, dn, pe, VRmax.data(), FE.data(), FMax.data(), D->dst);
+    // step5: ComputeMU2_t + RowSoftmax2(MU) (src/main.cu:229-241)
+    compute_mu<R>(dn, pe, dn, pn, CV.data(), FMax.data(), RMax.data(), P->src, P->dst, MU.data());
+    row_log_softmax<R>(pn, dn, MU.data(), literal);
+  }
+  // ac::transpose (src/main.cu:247): mu_out[v * pn + p] = MU[p][v]
+  if (mu_out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < dn; ++v)
+      for (int64_t p = 0; p < pn; ++p) mu_out[v * pn + p] = (double)MU[p * dn + v];
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (elapsed_s) *elapsed_s = std::chrono::duration<double>(t1 - t0).count();
+  if (fmax_out)
+    for (int64_t i = 0; i < pe * dn; ++i) fmax_out[i] = (double)FMax[i];
+  if (rmax_out)
+    for (int64_t i = 0; i < pe * dn; ++i) rmax_out[i] = (double)RMax[i];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Whole matcher.  mu_out: dn x pn (src/main.cu:247 order); fmax_out / rmax_out (optional): pe x dn final state.
+int or_run_f64(const or_graph* D, const or_graph* P, int n_iter, unsigned flags, int threads, double* mu_out,
+               double* fmax_out, double* rmax_out, double* elapsed_s) {
+  if (threads > 0) omp_set_num_threads(threads);
+  return run<double>(D, P, n_iter, flags, mu_out, fmax_out, rmax_out, elapsed_s);
+}
+int or_run_f32(const or_graph* D, const or_graph* P, int n_iter, unsigned flags, int threads, double* mu_out,
+               double* fmax_out, double* rmax_out, double* elapsed_s) {
+  if (threads > 0) omp_set_num_threads(threads);
+  return run<float>(D, P, n_iter, flags, mu_out, fmax_out, rmax_out, elapsed_s);
+}
+
+// Stage functions, FP64, for the pinning tests.
+void or_cdist(int64_t n_a, int64_t n_b, int64_t dim, const double* a, const double* b, double* out) {
+  cdist<double>(n_a, n_b, dim, a, b, out);
+}
+void or_row_log_softmax(int64_t n_row, int64_t n_col, double* x, int literal) {
+  row_log_softmax<double>(n_row, n_col, x, literal != 0);
+}
+void or_row_max(int64_t n_row, int64_t n_col, const double* x, double* out) {
+  row_max_all<double>(n_row, n_col, x, out);
+}
+void or_edge_max_reduce(int64_t n_col_in, int64_t n_col_out, int64_t n_row, const double* vymax, const double* xe,
+                        double* xmax, const int64_t* nodes) {
+  edge_max_reduce<double>(n_col_in, n_col_out, n_row, vymax, xe, xmax, nodes);
+}
+void or_compute_mu(int64_t n_col_in, int64_t n_row_in, int64_t n_col_out, int64_t n_row_out, const double* cv,
+                   const double* fmax, const double* rmax, const int64_t* srcs, const int64_t* dsts, double* mu) {
+  compute_mu<double>(n_col_in, n_row_in, n_col_out, n_row_out, cv, fmax, rmax, srcs, dsts, mu);
+}
+int or_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"
