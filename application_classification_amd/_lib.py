@@ -20,6 +20,7 @@ AC_ERR_IO = 5
 AC_ERR_NODEV = 6
 AC_DATA = 0
 AC_PATTERN = 1
+STAGES = ["prep", "vertex_a", "reduce_a", "edge_walk", "walk_fixup", "vertex_c", "reduce_c", "output"]
 
 ERROR_NAMES = {
     AC_ERR_ARG: "AC_ERR_ARG",
@@ -82,6 +83,7 @@ SIGNATURES = {
     "ac_get_sizes": (I, [P, P]),
     "ac_get_timing": (I, [P, ctypes.POINTER(D), ctypes.POINTER(D)]),
     "ac_stream": (P, [P]),
+    "ac_run_profiled": (I, [P, I, P]),
     "ac_cdist": (I, [I64, I64, I64, P, P, P, P]),
     "ac_row_log_softmax": (I, [I64, I64, P, P]),
     "ac_row_max": (I, [I64, I64, P, P, P]),

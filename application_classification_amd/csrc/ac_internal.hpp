@@ -1,0 +1,98 @@
+// Shared definitions of the matcher's device buffers, scalar records and kernel launchers.
+//
+// Layout in HBM (one data graph, one pattern; n data vertices, m data edges, P pattern vertices, Q pattern edges):
+//   - per-vertex x pattern-edge arrays are row-major [n][Qs], Qs = Q rounded up to a multiple of 4, so a row is a
+//     whole number of float4s (the edge walk gathers whole rows);
+//   - per-vertex x pattern-vertex arrays are row-major [n][P];
+//   - the edge-cost matrix CE is [m][Qs] in CSC (destination-sorted) edge order;
+//   - edge ids travel as int2 pairs: CSC order {src, dst}, CSR order {CSC position, src}.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ac {
+
+// log-sum-exp accumulator: value = m + log1p(t).  m is the largest element seen, t the sum of exp(x - m) over all
+// other elements, so a log-probability near 0 is formed as (x - m) - log1p(t) without cancellation.
+struct Lse {
+  double m, t;
+};
+// Finished log-sum-exp: value = m + l.
+struct Lse2 {
+  double m, l;
+};
+
+// Per pattern edge q: the fill values of the edge max-reductions and the two edge log-sum-exps of one iteration
+// (src/main.cu:197-224).  fillF = max_v VR[q][v] (fills FMax), fillR = max_v VF[q][v] (fills RMax).
+struct QStat {
+  double fillF, fillR;
+  double mRE, lRE;  // log sum_e exp(RE[q][e])
+  double mFE, lFE;  // log sum_e exp(FE[q][e])
+};
+// Per-block partial of QStat.
+struct QPart {
+  double vfmax, vrmax;
+  Lse re, fe;
+};
+
+// A contiguous run of edges in CSC or CSR order covering whole vertices, or one piece of a vertex whose edge list
+// is longer than a run (slot >= 0 then names its carry slot).
+struct Run {
+  int kb, ke, slot, pad;
+};
+// A vertex split into `count` pieces with carry slots [first, first + count).
+struct Split {
+  int v, first, count, pad;
+};
+
+constexpr int kVertexBlock = 256;
+constexpr int kWalkBlock = 256;
+
+struct VertexArgs {
+  int64_t n;
+  int P, Q, Qs, VT;
+  const float* CV;      // [n][P]  vertex costs
+  const float* FMaxN;   // [n][Qs] normalised destination maxima (-inf: no in-edge)
+  float* VFm;           // [n][Qs] VF - min out-edge cost (-inf: no out-edge)
+  const float* minCEo;  // [n][Qs] min out-edge cost (+inf: none)
+  const float* lsO;     // [n][Qs] log1p(sum over other out-edges of exp(min - cost))
+  float* VR;            // [n][Qs]
+  float* out;           // [n][P]
+  const int* ps;
+  const int* pd;
+  const int* inc_off;  // [P + 1]
+  const int* inc;      // incidence of each pattern vertex: (q << 1) | (1 if RMax term else 0), ascending q
+  const QStat* qs;     // scalars that finish FMax / RMax of the state being read
+  const Lse2* pst;     // log-sum-exp of each MU column
+  QPart* qpart;        // [gridDim][Q]
+  Lse* ppart;          // [gridDim][P]
+  int first;           // MU = LSM(-CV): the prep state, no FMax / RMax terms
+};
+
+enum VertexMode { kModeA = 0, kModeC = 1, kModeOut = 2 };
+
+// Launchers (ac_kernels.hip).  All asynchronous on `st`.
+void launch_cdist_vertex(int64_t n, int P, int dV, const float* fd, const double* pf, float* CV, hipStream_t st);
+void launch_cdist_edge(int64_t m, int Q, int Qs, int dE, const float* fe, const int* eid, const double* pf,
+                       float* CE, hipStream_t st);
+void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks, hipStream_t st);
+void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st);
+void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st);
+void launch_prep_scalars(int P, int Q, const int* ps, const int* pd, const Lse2* cv, const Lse2* ce, QStat* qs,
+                         hipStream_t st);
+void launch_fill(float* p, int64_t count, float value, hipStream_t st);
+void launch_walk_src(int64_t nruns, const Run* runs, const int2* ks, const float* CE, int Q, int Qs, float* minCEo,
+                     float* lsO, float* VFm, double* carry, Lse* gpart, int nblocks, hipStream_t st);
+void launch_walk_src_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, float* minCEo,
+                           float* lsO, float* VFm, hipStream_t st);
+void launch_walk_max(int64_t nruns, const Run* runs, const int2* sd, const float* CE, const float* VR, int Q, int Qs,
+                     const QStat* qs, float* FMaxN, double* carry, hipStream_t st);
+void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
+                           float* FMaxN, hipStream_t st);
+void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st);
+int walk_blocks(int64_t nruns, int Qs);
+int vertex_tile(int P, int Q);
+
+}  // namespace ac

@@ -1,0 +1,244 @@
+"""Host-side mirror of the reference's driver and operator library, on top of the C ABI.
+
+``Matcher`` is the driver (src/main.cu:66-257): load or hand over the two graphs, run prep + the loop, read MU.
+``ac`` mirrors the ``ac::`` operator namespace (src/ac.hxx) over device buffers (torch.cuda tensors), with the
+reference's names and argument meaning, for per-stage use and checks.
+
+There is no CPU fallback: every call goes to libac_mi355x.so and raises ``ACError`` on failure.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import AC_DATA, AC_PATTERN, ACError, HostGraph  # noqa: F401
+
+
+@dataclass
+class GraphArrays:
+    """A graph as host arrays: features float32 row-major, ids int32 (the reference's Graph, src/main.cu:8-19)."""
+
+    vfeat: np.ndarray
+    efeat: np.ndarray
+    src: np.ndarray
+    dst: np.ndarray
+
+    @property
+    def num_nodes(self) -> int:
+        return int(self.vfeat.shape[0])
+
+    @property
+    def num_edges(self) -> int:
+        return int(self.src.shape[0])
+
+
+def _as_graph(g) -> GraphArrays:
+    vf = np.ascontiguousarray(g.vfeat, dtype=np.float32)
+    ef = np.ascontiguousarray(g.efeat, dtype=np.float32)
+    if ef.ndim == 1:
+        ef = ef.reshape(len(g.src), -1)
+    return GraphArrays(vf, ef, np.ascontiguousarray(g.src, dtype=np.int32), np.ascontiguousarray(g.dst, dtype=np.int32))
+
+
+def load_bin(path: str) -> GraphArrays:
+    """Read a graph in the binary layout of prob2bin.py:46-64 (what src/main.cu:21-63 reads)."""
+    L = _lib.lib()
+    hg = HostGraph()
+    _lib.check(L.ac_graph_load_bin(path.encode(), ctypes.byref(hg)))
+    try:
+        return _copy_host_graph(hg)
+    finally:
+        L.ac_graph_free(ctypes.byref(hg))
+
+
+def load_csv(vertex_csv: str, edge_csv: str) -> GraphArrays:
+    """Read the CSV pair that prob2bin.py:11-39 converts."""
+    L = _lib.lib()
+    hg = HostGraph()
+    _lib.check(L.ac_graph_load_csv(vertex_csv.encode(), edge_csv.encode(), ctypes.byref(hg)))
+    try:
+        return _copy_host_graph(hg)
+    finally:
+        L.ac_graph_free(ctypes.byref(hg))
+
+
+def _copy_host_graph(hg: HostGraph) -> GraphArrays:
+    nV, dV, nE, dE = hg.num_nodes, hg.node_feat_dim, hg.num_edges, hg.edge_feat_dim
+
+    def arr(ptr, count, dtype):
+        if count == 0:
+            return np.zeros(0, dtype)
+        return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+    return GraphArrays(
+        arr(hg.node_feats, nV * dV, np.float32).reshape(nV, dV),
+        arr(hg.edge_feats, nE * dE, np.float32).reshape(nE, dE),
+        arr(hg.srcs, nE, np.int32),
+        arr(hg.dsts, nE, np.int32),
+    )
+
+
+def _host_graph(g: GraphArrays) -> HostGraph:
+    return HostGraph(
+        g.num_nodes,
+        g.vfeat.shape[1],
+        g.num_edges,
+        g.efeat.shape[1] if g.efeat.ndim == 2 else 0,
+        g.vfeat.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+        g.efeat.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+        g.src.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+        g.dst.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+    )
+
+
+def save_bin(path: str, g) -> None:
+    g = _as_graph(g)
+    _lib.check(_lib.lib().ac_graph_save_bin(path.encode(), ctypes.byref(_host_graph(g))))
+
+
+def save_csv(vertex_csv: str, edge_csv: str, g) -> None:
+    g = _as_graph(g)
+    _lib.check(_lib.lib().ac_graph_save_csv(vertex_csv.encode(), edge_csv.encode(), ctypes.byref(_host_graph(g))))
+
+
+class Matcher:
+    """One matcher context on one GPU (ac_create / ac_set_graph / ac_run / ac_get_mu)."""
+
+    def __init__(self, device: int = 0):
+        self._L = _lib.lib()
+        h = ctypes.c_void_p()
+        _lib.check(self._L.ac_create(ctypes.byref(h), device))
+        self._h = h
+        self._dn = self._pn = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ac_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        _lib.check(rc, self._h)
+
+    def _set(self, which: int, g) -> GraphArrays:
+        g = _as_graph(g)
+        dE = g.efeat.shape[1] if g.efeat.ndim == 2 else 0
+        self._check(self._L.ac_set_graph(self._h, which, g.num_nodes, g.vfeat.shape[1], g.vfeat.ctypes.data,
+                                         g.num_edges, dE, g.efeat.ctypes.data, g.src.ctypes.data, g.dst.ctypes.data))
+        return g
+
+    def set_data(self, g) -> None:
+        self._dn = self._set(AC_DATA, g).num_nodes
+
+    def set_pattern(self, g) -> None:
+        self._pn = self._set(AC_PATTERN, g).num_nodes
+
+    def run(self, n_iter: int = -1) -> np.ndarray:
+        """Prep + n_iter iterations (default: number of pattern vertices, src/main.cu:160); returns MU, dn x pn."""
+        self._check(self._L.ac_run(self._h, n_iter))
+        return self.mu()
+
+    def run_async(self, n_iter: int = -1) -> None:
+        self._check(self._L.ac_run_async(self._h, n_iter))
+
+    def sync(self) -> None:
+        self._check(self._L.ac_sync(self._h))
+
+    def mu(self) -> np.ndarray:
+        out = np.empty((self._dn, self._pn), np.float32)
+        self._check(self._L.ac_get_mu(self._h, out.ctypes.data))
+        return out
+
+    def match(self, log_tau: float = float(np.log(0.5))) -> np.ndarray:
+        out = np.empty((self._dn, self._pn), np.uint8)
+        self._check(self._L.ac_get_match(self._h, log_tau, out.ctypes.data))
+        return out
+
+    def timing(self):
+        """(prep_ms, loop_ms) of the last run, from HIP events on the context's stream."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self._L.ac_get_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def run_profiled(self, n_iter: int = -1) -> dict:
+        """Runs like run() with a HIP event after each launch; returns device ms per stage (summed over iterations)."""
+        out = (ctypes.c_double * len(_lib.STAGES))()
+        self._check(self._L.ac_run_profiled(self._h, n_iter, out))
+        return dict(zip(_lib.STAGES, list(out)))
+
+    def sizes(self):
+        out = (ctypes.c_int64 * 4)()
+        self._check(self._L.ac_get_sizes(self._h, out))
+        return tuple(out)
+
+    @property
+    def stream(self) -> int:
+        return self._L.ac_stream(self._h) or 0
+
+
+class ac:
+    """The reference's operator library (src/ac.hxx) on device tensors (float32 values, int32 ids).
+
+    Each method takes torch.cuda tensors (or raw device pointers as ints) with the reference's argument order and
+    runs on ``stream`` (a hipStream_t as int; 0 = default stream).
+    """
+
+    @staticmethod
+    def _p(x):
+        return x if isinstance(x, int) else x.data_ptr()
+
+    @staticmethod
+    def cdist(n_a, n_b, dim, feats_a, feats_b, out, stream=0):
+        """src/ac.hxx:33-56: out[j * n_b + i] = ||a_j - b_i||."""
+        _lib.check(_lib.lib().ac_cdist(n_a, n_b, dim, ac._p(feats_a), ac._p(feats_b), ac._p(out), stream))
+
+    @staticmethod
+    def RowSoftmax2(n_row, n_col, x, stream=0):
+        """src/ac.hxx:108-113: in-place row log-softmax."""
+        _lib.check(_lib.lib().ac_row_log_softmax(n_row, n_col, ac._p(x), stream))
+
+    RowSoftmax2_prealloc = RowSoftmax2
+
+    @staticmethod
+    def RowMax2(n_row, n_col, d_in, d_out, stream=0):
+        """src/ac.hxx:58-75: per-row maximum."""
+        _lib.check(_lib.lib().ac_row_max(n_row, n_col, ac._p(d_in), ac._p(d_out), stream))
+
+    @staticmethod
+    def EdgeMaxReduce2_t(n_col_in, n_col_out, n_row, VYMax, XE_t, XMax_t, nodes, stream=0):
+        """src/ac.hxx:115-149: fill with VYMax then scatter-max XE_t by nodes."""
+        _lib.check(_lib.lib().ac_edge_max_reduce(n_col_in, n_col_out, n_row, ac._p(VYMax), ac._p(XE_t),
+                                                 ac._p(XMax_t), ac._p(nodes), stream))
+
+    @staticmethod
+    def ComputeMU2_t(n_col_in, n_row_in, n_col_out, n_row_out, CV_t, FMax_t, RMax_t, srcs, dsts, MU_t, stream=0):
+        """src/ac.hxx:151-186: MU = -CV + scatter-add of FMax by dsts and RMax by srcs (ascending q)."""
+        _lib.check(_lib.lib().ac_compute_mu(n_col_in, n_row_in, n_col_out, n_row_out, ac._p(CV_t), ac._p(FMax_t),
+                                            ac._p(RMax_t), ac._p(srcs), ac._p(dsts), ac._p(MU_t), stream))
+
+    @staticmethod
+    def transpose(d_in, d_out, num_rows, num_cols, stream=0):
+        """src/ac.hxx:17-31."""
+        _lib.check(_lib.lib().ac_transpose(ac._p(d_in), ac._p(d_out), num_rows, num_cols, stream))
+
+
+def run_matcher(data, pattern, n_iter: int = -1, device: int = 0) -> np.ndarray:
+    """One-shot helper: MU (dn x pn) for a data graph and a pattern graph."""
+    with Matcher(device) as m:
+        m.set_data(data)
+        m.set_pattern(pattern)
+        return m.run(n_iter)

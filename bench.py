@@ -1,0 +1,187 @@
+"""Benchmark of the matcher on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config s22|s18|small] [--no-cpu-baseline]
+
+A step is one full match of the workload: prep (cdist and edge-cost statistics, src/main.cu:107-148) plus pn
+iterations of the loop (src/main.cu:160-245) plus the output write, with the graphs already resident in HBM.
+Throughput is the reference's unit of work, one (data edge, pattern edge) pair per iteration:
+    value = de * pe * pn / t_loop   (edge-pair updates per second, t_loop = device time of the iterations).
+The default workload is R-MAT scale 22 (64-dim features, planted 16-vertex / 24-edge pattern).
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(dn, de, pn, pe):
+    """Per iteration.  iteration: the fixed figure 4*pe*de + 8*de + (40*pe + 12*pn)*dn (FP32 values, int32 ids);
+    edge_walk: the edge stream (costs + ids) plus one read of VR and one write of FMax per vertex."""
+    return {
+        "iteration": 4 * pe * de + 8 * de + (40 * pe + 12 * pn) * dn,
+        "edge_walk": 4 * pe * de + 8 * de + 8 * pe * dn,
+        "vertex_a": (24 * pe + 4 * pn) * dn,
+        "vertex_c": (8 * pe + 4 * pn) * dn,
+    }
+
+
+def cpu_baseline(threads):
+    """The FP64 C++ restatement of the reference (oracle/), timed on this host's cores over the reference's timed
+    region (prep + loop + transpose), on a scale-18 instance of the benchmark workload."""
+    from application_classification_amd import workloads as W
+    from oracle import oracle as O
+
+    w = W.rmat(18, 64, 16, 24, seed=0)
+    d, p = w.data, w.pattern
+    D = O.Graph64.of(d.vfeat, d.efeat, d.src, d.dst)
+    P = O.Graph64.of(p.vfeat, p.efeat, p.src, p.dst)
+    prep_s, loop_s, _ = O.timed_run(D, P, threads=threads)
+    de, pe, pn = d.num_edges, p.num_edges, p.num_nodes
+    return {
+        "value": de * pe * pn / loop_s,
+        "unit": "edge-pair updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"R-MAT scale 18 ({d.num_nodes} vertices, {de} edges), 64-dim features, planted 16/24 pattern, "
+                  f"{pn} iterations, FP64; loop {loop_s:.2f} s, prep {prep_s:.2f} s",
+        "loop_s": loop_s,
+        "prep_s": prep_s,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="s22", choices=["s22", "s18", "small"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the matcher is a single-GPU path here (multi-GPU sharding is a design "
+              "note, not built); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from application_classification_amd import workloads as W
+    from application_classification_amd.matcher import Matcher
+
+    t = time.time()
+    w = W.CONFIGS[args.config](0)
+    d, p = w.data, w.pattern
+    dn, de, pn, pe = d.num_nodes, d.num_edges, p.num_nodes, p.num_edges
+    log(f"workload {w.name}: dn={dn} de={de} pn={pn} pe={pe} generated in {time.time() - t:.1f} s")
+
+    m = Matcher(0)
+    t = time.time()
+    m.set_data(d)
+    m.set_pattern(p)
+    log(f"graphs resident on the GPU in {time.time() - t:.1f} s")
+    for i in range(args.warmup):
+        m.run_async()
+        m.sync()
+        log(f"warmup {i + 1}/{args.warmup}: prep {m.timing()[0]:.3f} ms, loop {m.timing()[1]:.3f} ms")
+
+    try:
+        import torch
+
+        sync_all = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing here; the matcher does not need it
+        sync_all = lambda: None  # noqa: E731
+    loop_ms = prep_ms = 0.0
+    m.sync()
+    sync_all()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        m.run_async()
+        m.sync()
+        a, b = m.timing()
+        prep_ms += a
+        loop_ms += b
+    m.sync()
+    sync_all()
+    t1 = time.perf_counter()
+    wall_ms = (t1 - t0) * 1e3 / args.steps
+    loop_ms /= args.steps
+    prep_ms /= args.steps
+    log(f"timed {args.steps} steps: {wall_ms:.3f} ms/step (device: prep {prep_ms:.3f} ms, loop {loop_ms:.3f} ms)")
+
+    mu = m.mu()
+    checks = {
+        "planted_recovered": bool((mu.argmax(axis=0) == p.vmap).all()),
+        "max_abs_column_logsumexp": float(np.abs(np.log(np.exp(mu.astype(np.float64)).sum(axis=0))).max()),
+        "all_log_probs_nonpositive": bool((mu <= 0).all()),
+    }
+    log(f"checks: {checks}")
+
+    updates = de * pe * pn
+    B = algorithmic_bytes(dn, de, pn, pe)
+    iter_frac = B["iteration"] * pn / (loop_ms * 1e-3) / HBM_PEAK
+    roofline = None
+    stages = None
+    if not args.no_profile:
+        stages = m.run_profiled()
+        loop_stages = {k: v for k, v in stages.items() if k not in ("prep",)}
+        dom = max(("edge_walk", "vertex_a", "vertex_c"), key=lambda k: loop_stages[k])
+        avg_ms = stages[dom] / pn
+        achieved = B[dom] / (avg_ms * 1e-3)
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                    "alg_bytes_per_launch": B[dom], "avg_launch_ms": avg_ms}
+        log(f"stages (ms, summed over {pn} iterations): {json.dumps({k: round(v, 4) for k, v in stages.items()})}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"cpu baseline: FP64 restatement on {threads} threads ...")
+        cpu = cpu_baseline(threads)
+        log(f"cpu baseline: {cpu['value']:.3e} updates/s ({cpu['sample']})")
+        cpu = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
+
+    out = {
+        "metric": "edge-pair updates/s per iteration + % HBM roofline",
+        "value": updates / (loop_ms * 1e-3),
+        "unit": "edge-pair updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_ms,
+        "loop_ms_per_step": loop_ms,
+        "prep_ms_per_step": prep_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "storage_dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "data_vertices": dn, "data_edges": de, "pattern_vertices": pn,
+                   "pattern_edges": pe, "feature_dim": int(d.vfeat.shape[1]), "iterations": pn,
+                   "parallelism": "single-gpu"},
+        "hbm_roofline_frac_iteration": iter_frac,
+        "alg_bytes_per_iteration": B["iteration"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stage_ms": stages,
+        "checks": checks,
+    }
+    m.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -119,6 +119,21 @@ int ac_get_timing(const ac_ctx* ctx, double* prep_ms, double* loop_ms);
 /* Device stream the context runs on (a hipStream_t). */
 void* ac_stream(ac_ctx* ctx);
 
+/* Stages of one run, timed separately by ac_run_profiled (they replace the reference's NVTX ranges "prep", "loop",
+ * "step1".."step5", src/main.cu:107,161-244). */
+#define AC_STAGE_PREP 0        /* cdist, edge-cost statistics, prep maxima (src/main.cu:107-148)              */
+#define AC_STAGE_VERTEX_A 1    /* VF / VR, column maxima, edge log-sum-exps (steps 1, 3a, 4a)                 */
+#define AC_STAGE_REDUCE_A 2    /* combine the per-block partials of stage 1                                   */
+#define AC_STAGE_EDGE_WALK 3   /* destination maxima over the data edges (steps 2, 4b, 4c)                    */
+#define AC_STAGE_WALK_FIXUP 4  /* combine the pieces of vertices with very long in-edge lists                 */
+#define AC_STAGE_VERTEX_C 5    /* MU column log-sum-exp (step 5)                                               */
+#define AC_STAGE_REDUCE_C 6    /* combine the per-block partials of stage 5                                   */
+#define AC_STAGE_OUTPUT 7      /* normalised MU written in dn x pn order (src/main.cu:247)                     */
+#define AC_NSTAGES 8
+/* Like ac_run, with a HIP event after every launch; stage_ms[s] (AC_NSTAGES entries) receives the device time
+ * of stage s summed over the iterations.  Extra events make it slightly slower than ac_run. */
+int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms);
+
 /* -------------------------------------------------------------------------------------------------------------
  * Stage operators on caller-owned device buffers: the ac:: operator library (src/ac.hxx) with FP32 values and
  * int32 ids, same argument meaning and pattern-major layouts.  Asynchronous on `stream`.

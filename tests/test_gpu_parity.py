@@ -1,0 +1,249 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle and the golden fixtures.
+
+Bar (the reference's own comparison, validate.py:7): every log-probability within |a - b| <= 1e-8 + 1e-5 |b| of the
+FP64 oracle; max stages bit-exact on identical inputs; the match matrix identical.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from application_classification_amd import workloads as W
+from application_classification_amd.matcher import Matcher, ac
+from application_classification_amd._lib import ACError, AC_ERR_ARG, AC_ERR_STATE
+from oracle import oracle as O
+from tests import cases
+from tests.conftest import GOLDEN, allclose_report
+from tests.golden.make_golden import digest
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL = 1e-5, 1e-8
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def g64(g):
+    return O.Graph64.of(g.vfeat, g.efeat, g.src, g.dst)
+
+
+def gpu_mu(data, patt, n_iter=-1):
+    with Matcher(0) as m:
+        m.set_data(data)
+        m.set_pattern(patt)
+        return m.run(n_iter)
+
+
+def assert_parity(mu, ref, what):
+    bad, ratio = allclose_report(mu, ref, RTOL, ATOL)
+    say(f"{what}: max err/tol = {ratio:.4f}, violations = {bad}")
+    assert bad == 0, f"{what}: {bad} log-probabilities outside rtol={RTOL}, atol={ATOL} (max err/tol {ratio:.3f})"
+
+
+@pytest.fixture(scope="module")
+def small():
+    return W.small(0)
+
+
+@pytest.mark.parametrize("n_iter", [0, 1, 2, 5, -1])
+def test_small_against_oracle(small, n_iter):
+    say(f"small-er1000 n_iter={n_iter}")
+    mu = gpu_mu(small.data, small.pattern, n_iter)
+    ref = O.run(g64(small.data), g64(small.pattern), n_iter=n_iter)
+    assert_parity(mu, ref, f"small n_iter={n_iter}")
+
+
+@pytest.mark.parametrize("name,make", [("small_er1000", lambda: W.small(0)), ("georgiy", cases.georgiy),
+                                       ("tiny_hand", cases.tiny_hand)])
+def test_golden_fixtures(name, make):
+    say(f"golden {name}")
+    w = make()
+    data, patt = w.data, w.pattern
+    fx = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
+    key = digest(data.vfeat, data.efeat, data.src, data.dst, patt.vfeat, patt.efeat, patt.src, patt.dst)
+    assert str(fx["inputs_sha256"]) == key, "the input generator changed; regenerate tests/golden/make_golden.py"
+    mu = gpu_mu(data, patt, int(fx["n_iter"]))
+    assert_parity(mu, fx["mu"], f"golden {name}")
+
+
+def test_planted_match_recovered_and_match_matrix(small):
+    with Matcher(0) as m:
+        m.set_data(small.data)
+        m.set_pattern(small.pattern)
+        mu = m.run()
+        match = m.match(np.log(0.5))
+    np.testing.assert_array_equal(mu.argmax(axis=0), small.pattern.vmap)
+    ref = O.run(g64(small.data), g64(small.pattern))
+    expect = np.zeros_like(match)
+    am = ref.argmax(axis=0)
+    for p, v in enumerate(am):
+        expect[v, p] = 2 if ref[v, p] >= np.log(0.5) else 1
+    np.testing.assert_array_equal(match, expect)
+
+
+@pytest.mark.parametrize("scale,dim,pv,pe", [(10, 8, 6, 9), (14, 16, 12, 16)])
+def test_rmat_against_oracle(scale, dim, pv, pe):
+    """R-MAT hubs exceed one edge run, so the split / carry path is exercised; many vertices have no in- or
+    out-edges."""
+    c = cases.rmat_case(scale, dim, pv, pe)
+    say(f"{c.name}: {c.data.num_edges} edges")
+    indeg = np.bincount(c.data.dst, minlength=c.data.num_nodes)
+    assert indeg.max() > 64 and (indeg == 0).any()
+    mu = gpu_mu(c.data, c.pattern)
+    ref = O.run(g64(c.data), g64(c.pattern))
+    assert_parity(mu, ref, c.name)
+    np.testing.assert_array_equal(mu.argmax(axis=0), c.pattern.vmap)
+
+
+def test_s18_against_oracle():
+    w = W.s18(0)
+    say(f"{w.name}: {w.data.num_edges} edges")
+    mu = gpu_mu(w.data, w.pattern)
+    ref = O.run(g64(w.data), g64(w.pattern))
+    assert_parity(mu, ref, w.name)
+
+
+def test_deterministic(small):
+    a = gpu_mu(small.data, small.pattern)
+    b = gpu_mu(small.data, small.pattern)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_edge_order_does_not_matter(small):
+    d, p = small.data, small.pattern
+    perm = np.random.default_rng(0).permutation(d.num_edges)
+    d2 = W.Graph(d.vfeat, d.efeat[perm], d.src[perm], d.dst[perm])
+    assert_parity(gpu_mu(d2, p), O.run(g64(d), g64(p)), "shuffled edges")
+
+
+def test_pattern_without_edges(small):
+    d, p = small.data, small.pattern
+    patt = W.Graph(p.vfeat, np.zeros((0, 13), np.float32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    mu = gpu_mu(d, patt, 3)
+    ref = O.run(g64(d), g64(patt), n_iter=3)
+    assert_parity(mu, ref, "pattern without edges")
+
+
+def test_data_without_edges(small):
+    d, p = small.data, small.pattern
+    data = W.Graph(d.vfeat, np.zeros((0, 13), np.float32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    mu = gpu_mu(data, p, 2)
+    ref = O.run(g64(data), g64(p), n_iter=2)
+    assert_parity(mu, ref, "data without edges")
+
+
+def test_single_data_vertex():
+    r = np.random.default_rng(2)
+    data = W.Graph(r.normal(size=(1, 3)).astype(np.float32), r.normal(size=(1, 2)).astype(np.float32),
+                   np.zeros(1, np.int32), np.zeros(1, np.int32))
+    patt = W.Graph(r.normal(size=(2, 3)).astype(np.float32), r.normal(size=(1, 2)).astype(np.float32),
+                   np.array([0], np.int32), np.array([1], np.int32))
+    mu = gpu_mu(data, patt)
+    ref = O.run(g64(data), g64(patt))
+    assert_parity(mu, ref, "single data vertex")
+    np.testing.assert_array_equal(mu, 0.0)
+
+
+def test_large_pattern(small):
+    """A pattern wider than one float4 row chunk per lane group and than the default vertex tile."""
+    d = small.data
+    p = W.planted_pattern(d, 40, 70, seed=3)
+    mu = gpu_mu(d, p, 4)
+    ref = O.run(g64(d), g64(p), n_iter=4)
+    assert_parity(mu, ref, "40-vertex / 70-edge pattern")
+
+
+def test_errors_are_reported():
+    r = np.random.default_rng(0)
+    with Matcher(0) as m:
+        with pytest.raises(ACError) as e:
+            m.run()
+        assert e.value.code == AC_ERR_STATE
+        bad = W.Graph(r.normal(size=(3, 2)).astype(np.float32), r.normal(size=(1, 2)).astype(np.float32),
+                      np.array([0], np.int32), np.array([3], np.int32))
+        with pytest.raises(ACError) as e:
+            m.set_data(bad)
+        assert e.value.code == AC_ERR_ARG
+        m.set_data(W.Graph(bad.vfeat, bad.efeat, bad.src, np.array([2], np.int32)))
+        m.set_pattern(W.Graph(r.normal(size=(2, 5)).astype(np.float32), r.normal(size=(1, 2)).astype(np.float32),
+                              np.array([0], np.int32), np.array([1], np.int32)))
+        with pytest.raises(ACError) as e:
+            m.run()
+        assert e.value.code == AC_ERR_ARG
+
+
+# --- stage operators (src/ac.hxx) on device buffers ---------------------------------------------------------------
+
+torch = pytest.importorskip("torch")
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def test_stage_cdist():
+    r = np.random.default_rng(0)
+    a = r.normal(size=(7, 13)).astype(np.float32)
+    b = r.normal(size=(300, 13)).astype(np.float32)
+    out = torch.empty(7 * 300, dtype=torch.float32, device="cuda")
+    ac.cdist(7, 300, 13, dev(a), dev(b), out)
+    torch.cuda.synchronize()
+    ref = O.cdist(a, b).astype(np.float32)
+    got = out.cpu().numpy().reshape(7, 300)
+    ulp = np.abs(got.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1
+
+
+def test_stage_row_max_and_edge_max_reduce_bit_exact():
+    r = np.random.default_rng(1)
+    n_row, n_in, n_out = 5, 4000, 300
+    xe = r.normal(size=(n_row, n_in)).astype(np.float32)
+    xe[0, :7] = -0.0
+    nodes = r.integers(0, n_out - 3, n_in).astype(np.int32)
+    vy = (r.normal(size=n_row) - 2.0).astype(np.float32)
+    vy[1] = -0.0
+    out = torch.empty(n_row * n_out, dtype=torch.float32, device="cuda")
+    ac.EdgeMaxReduce2_t(n_in, n_out, n_row, dev(vy), dev(xe), out, dev(nodes))
+    rm = torch.empty(n_row, dtype=torch.float32, device="cuda")
+    ac.RowMax2(n_row, n_in, dev(xe), rm)
+    torch.cuda.synchronize()
+    expect = np.repeat(vy[:, None], n_out, axis=1)
+    for i in range(n_row):
+        np.maximum.at(expect[i], nodes, xe[i])
+    np.testing.assert_array_equal(out.cpu().numpy().reshape(n_row, n_out), expect)
+    np.testing.assert_array_equal(rm.cpu().numpy(), xe.max(axis=1))
+
+
+def test_stage_row_log_softmax():
+    r = np.random.default_rng(2)
+    x = (r.normal(size=(6, 5000)) * 20).astype(np.float32)
+    x[3, 17] = 60.0  # one dominant entry: its log-probability is tiny and must keep relative accuracy
+    d = dev(x)
+    ac.RowSoftmax2(6, 5000, d)
+    torch.cuda.synchronize()
+    ref = O.row_log_softmax(x.astype(np.float64))
+    bad, ratio = allclose_report(d.cpu().numpy(), ref)
+    assert bad == 0, ratio
+
+
+def test_stage_compute_mu_and_transpose():
+    r = np.random.default_rng(3)
+    pn, pe, dn = 5, 9, 1000
+    cv = r.normal(size=(pn, dn)).astype(np.float32)
+    fm = r.normal(size=(pe, dn)).astype(np.float32)
+    rm = r.normal(size=(pe, dn)).astype(np.float32)
+    ps = r.integers(0, pn, pe).astype(np.int32)
+    pd = r.integers(0, pn, pe).astype(np.int32)
+    mu = torch.empty(pn * dn, dtype=torch.float32, device="cuda")
+    ac.ComputeMU2_t(dn, pe, dn, pn, dev(cv), dev(fm), dev(rm), dev(ps), dev(pd), mu)
+    out = torch.empty(pn * dn, dtype=torch.float32, device="cuda")
+    ac.transpose(mu, out, pn, dn)
+    torch.cuda.synchronize()
+    ref = O.compute_mu(cv, fm, rm, ps, pd)
+    got = mu.cpu().numpy().reshape(pn, dn)
+    np.testing.assert_array_equal(got, ref.astype(np.float32))  # FP64 sum in the same order, rounded once
+    np.testing.assert_array_equal(out.cpu().numpy().reshape(dn, pn), got.T)
