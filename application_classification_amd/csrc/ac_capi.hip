@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -22,8 +23,26 @@ using ac::Split;
 
 namespace {
 
-// Edges per run of an edge walk.  Vertices whose edge list is longer are split into pieces of this length.
-constexpr int kRunEdges = 64;
+// Edges per run of an edge walk (a tuning knob, AC_RUN_EDGES).  Vertices whose edge list is longer are split into
+// pieces of this length.
+int run_edges() {
+  if (const char* e = std::getenv("AC_RUN_EDGES")) {
+    const int v = std::atoi(e);
+    if (v >= 4 && v <= 4096) return v;
+  }
+  return 64;
+}
+
+int round8(int x) { return (x + 7) / 8 * 8; }
+
+// Row stride of the gathered VR rows: whole 128-byte lines (or an exact fraction of one), so that a gather of one
+// row never straddles two lines.
+int gather_stride(int Qs) {
+  if (Qs > 32) return (Qs + 31) / 32 * 32;
+  int s = 4;
+  while (s < Qs) s *= 2;
+  return s;
+}
 
 template <typename T>
 struct DevBuf {
@@ -61,6 +80,7 @@ struct DevBuf {
 
 void build_runs(const std::vector<int64_t>& off, int64_t n, std::vector<Run>& runs, std::vector<Split>& splits,
                 int& nslots) {
+  const int kRunEdges = run_edges();
   runs.clear();
   splits.clear();
   nslots = 0;
@@ -128,7 +148,7 @@ struct ac_ctx {
   DevBuf<Lse> ppart, lpart;
   DevBuf<QStat> qstat;  // [3][Q]: prep state, then by iteration parity
   DevBuf<Lse2> pstat, lse_cv, lse_ce;
-  int nb_vertex = 0, nb_col = 0, nb_src = 0, VT = 0;
+  int nb_vertex = 0, nb_col = 0, nb_src = 0, VT = 0, VRs = 0;
   int64_t work_key[4] = {-1, -1, -1, -1};
 
   bool have_result = false;
@@ -203,7 +223,7 @@ extern "C" void* ac_stream(ac_ctx* ctx) { return ctx ? (void*)ctx->stream : null
 
 static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_t nE, int64_t dE, const float* ef,
                     const int32_t* src, const int32_t* dst) {
-  if (nE >= (int64_t)INT32_MAX - kRunEdges)
+  if (nE >= (int64_t)INT32_MAX - 4096)
     return ctx->fail(AC_ERR_ARG, "ac_set_graph: more than 2^31 edges is not supported");
   // CSC (by destination, stable) and CSR (by source, stable) views.
   std::vector<int64_t> in_off(nV + 1, 0), out_off(nV + 1, 0);
@@ -277,11 +297,14 @@ static int set_pattern(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int
   ctx->pefeat.assign(ef, ef + nE * dE);
   ctx->ps.assign(src, src + nE);
   ctx->pd.assign(dst, dst + nE);
-  // Pattern features in FP64: vertex features [P][dV]; edge features transposed to [dE][Qs] with zero padding.
-  std::vector<double> pv(nV * dV), pe((size_t)dE * ctx->Qs, 0.0);
-  for (int64_t i = 0; i < nV * dV; ++i) pv[i] = vf[i];
+  // Pattern features in FP64, transposed to [dim][count rounded up to 8] with zero padding (cdist reads eight
+  // patterns per step).
+  const int Pp = round8((int)nV), Qp = round8((int)nE);
+  std::vector<double> pv((size_t)dV * Pp, 0.0), pe((size_t)dE * Qp, 0.0);
+  for (int64_t p = 0; p < nV; ++p)
+    for (int64_t k = 0; k < dV; ++k) pv[k * Pp + p] = vf[p * dV + k];
   for (int64_t q = 0; q < nE; ++q)
-    for (int64_t k = 0; k < dE; ++k) pe[k * ctx->Qs + q] = ef[q * dE + k];
+    for (int64_t k = 0; k < dE; ++k) pe[k * Qp + q] = ef[q * dE + k];
   // Incidence of each pattern vertex in ascending q: the FMax term (q ends at p) before the RMax term (q starts at p).
   std::vector<int> inc_off(nV + 1, 0), inc;
   for (int64_t p = 0; p < nV; ++p) {
@@ -335,17 +358,18 @@ static int ensure_work(ac_ctx* ctx) {
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
   ctx->VT = ac::vertex_tile(P, Q);
   const int64_t ntiles = (n + ctx->VT - 1) / ctx->VT;
-  ctx->nb_vertex = (int)std::min<int64_t>(ntiles, 1024);
+  ctx->nb_vertex = (int)std::min<int64_t>(ntiles, 8192);
   ctx->nb_col = (int)std::min<int64_t>((n + 255) / 256, 1024);
   ctx->nb_src = std::max(1, ac::walk_blocks(ctx->nruns_out, Qs));
   const int nb_l = std::max(ctx->nb_col, ctx->nb_src);
-  HIPCHK(ctx, ctx->CE.alloc((size_t)std::max<int64_t>(1, m * Qs)));
+  HIPCHK(ctx, ctx->CE.alloc((size_t)std::max<int64_t>(1, (m + 3) / 4 * 4 * Qs)));
   HIPCHK(ctx, ctx->CV.alloc((size_t)(n * P)));
   HIPCHK(ctx, ctx->FMaxN.alloc((size_t)(n * Qs)));
   HIPCHK(ctx, ctx->VFm.alloc((size_t)(n * Qs)));
   HIPCHK(ctx, ctx->minCEo.alloc((size_t)(n * Qs)));
   HIPCHK(ctx, ctx->lsO.alloc((size_t)(n * Qs)));
-  HIPCHK(ctx, ctx->VR.alloc((size_t)(n * Qs)));
+  ctx->VRs = gather_stride(Qs);
+  HIPCHK(ctx, ctx->VR.alloc((size_t)(n * ctx->VRs)));
   HIPCHK(ctx, ctx->OUT.alloc((size_t)(n * P)));
   HIPCHK(ctx, ctx->carry_in.alloc((size_t)std::max(1, ctx->nslots_in) * Qs));
   HIPCHK(ctx, ctx->carry_out.alloc((size_t)std::max(1, ctx->nslots_out) * Qs * 2));
@@ -405,12 +429,13 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
   ctx->pev_used = 0;
   MARK(-1);
   // ---- prep (src/main.cu:107-148)
-  ac::launch_cdist_vertex(n, P, (int)ctx->dV, ctx->vfeat.p, ctx->pf_v.p, ctx->CV.p, st);
+  ac::launch_cdist(n, nullptr, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P, 0, st);
   ac::launch_col_lse_neg(n, P, ctx->CV.p, ctx->lpart.p, ctx->nb_col, st);
   ac::launch_reduce_lse(ctx->nb_col, P, ctx->lpart.p, ctx->lse_cv.p, st);
   HIPCHK(ctx, hipMemcpyAsync(ctx->pstat.p, ctx->lse_cv.p, sizeof(Lse2) * P, hipMemcpyDeviceToDevice, st));
   if (Q > 0) {
-    ac::launch_cdist_edge(m, Q, Qs, (int)ctx->dE, ctx->efeat.p, ctx->csc_eid.p, ctx->pf_e.p, ctx->CE.p, st);
+    ac::launch_cdist(m, ctx->csc_eid.p, ctx->efeat.p, (int)ctx->dE, ctx->pf_e.p, Q, round8(Q), Qs, ctx->CE.p, Qs, 1,
+                     st);
     ac::launch_fill(ctx->minCEo.p, n * Qs, INFINITY, st);
     ac::launch_fill(ctx->lsO.p, n * Qs, 0.f, st);
     ac::launch_fill(ctx->VFm.p, n * Qs, -INFINITY, st);
@@ -427,7 +452,7 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
       HIPCHK(ctx, hipStreamSynchronize(st));
     }
     ac::launch_prep_scalars(P, Q, ctx->d_ps.p, ctx->d_pd.p, ctx->lse_cv.p, ctx->lse_ce.p, qs_prep, st);
-    ac::launch_walk_max(ctx->nruns_in, ctx->runs_in.p, ctx->csc_sd.p, ctx->CE.p, nullptr, Q, Qs, qs_prep,
+    ac::launch_walk_max(ctx->nruns_in, ctx->runs_in.p, ctx->csc_sd.p, ctx->CE.p, nullptr, ctx->VRs, Q, Qs, qs_prep,
                         ctx->FMaxN.p, ctx->carry_in.p, st);
     ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_prep, ctx->FMaxN.p, st);
   }
@@ -447,6 +472,7 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
   va.minCEo = ctx->minCEo.p;
   va.lsO = ctx->lsO.p;
   va.VR = ctx->VR.p;
+  va.VRs = ctx->VRs;
   va.out = ctx->OUT.p;
   va.ps = ctx->d_ps.p;
   va.pd = ctx->d_pd.p;
@@ -465,8 +491,8 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
       MARK(AC_STAGE_VERTEX_A);
       ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_new, st);
       MARK(AC_STAGE_REDUCE_A);
-      ac::launch_walk_max(ctx->nruns_in, ctx->runs_in.p, ctx->csc_sd.p, ctx->CE.p, ctx->VR.p, Q, Qs, qs_new,
-                          ctx->FMaxN.p, ctx->carry_in.p, st);
+      ac::launch_walk_max(ctx->nruns_in, ctx->runs_in.p, ctx->csc_sd.p, ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs,
+                          qs_new, ctx->FMaxN.p, ctx->carry_in.p, st);
       MARK(AC_STAGE_EDGE_WALK);
       ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_new, ctx->FMaxN.p, st);
       MARK(AC_STAGE_WALK_FIXUP);

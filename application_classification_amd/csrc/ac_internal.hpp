@@ -4,7 +4,8 @@
 //   - per-vertex x pattern-edge arrays are row-major [n][Qs], Qs = Q rounded up to a multiple of 4, so a row is a
 //     whole number of float4s (the edge walk gathers whole rows);
 //   - per-vertex x pattern-vertex arrays are row-major [n][P];
-//   - the edge-cost matrix CE is [m][Qs] in CSC (destination-sorted) edge order;
+//   - the edge-cost matrix CE holds m (rounded up to 4) x Qs costs in CSC (destination-sorted) edge order, in
+//     "quad" layout: CE(k, q) = CE[(k / 4) * 4 * Qs + 4 q + k % 4];
 //   - edge ids travel as int2 pairs: CSC order {src, dst}, CSR order {CSC position, src}.
 #pragma once
 
@@ -58,7 +59,8 @@ struct VertexArgs {
   float* VFm;           // [n][Qs] VF - min out-edge cost (-inf: no out-edge)
   const float* minCEo;  // [n][Qs] min out-edge cost (+inf: none)
   const float* lsO;     // [n][Qs] log1p(sum over other out-edges of exp(min - cost))
-  float* VR;            // [n][Qs]
+  float* VR;            // [n][VRs]
+  int VRs;              // row stride of VR: a power of two up to 32 floats, else a multiple of 32 (whole lines)
   float* out;           // [n][P]
   const int* ps;
   const int* pd;
@@ -74,9 +76,8 @@ struct VertexArgs {
 enum VertexMode { kModeA = 0, kModeC = 1, kModeOut = 2 };
 
 // Launchers (ac_kernels.hip).  All asynchronous on `st`.
-void launch_cdist_vertex(int64_t n, int P, int dV, const float* fd, const double* pf, float* CV, hipStream_t st);
-void launch_cdist_edge(int64_t m, int Q, int Qs, int dE, const float* fe, const int* eid, const double* pf,
-                       float* CE, hipStream_t st);
+void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, const double* pat, int K, int Kp,
+                  int Kstore, float* out, int out_stride, int quad, hipStream_t st);
 void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks, hipStream_t st);
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st);
 void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st);
@@ -87,8 +88,8 @@ void launch_walk_src(int64_t nruns, const Run* runs, const int2* ks, const float
                      float* lsO, float* VFm, double* carry, Lse* gpart, int nblocks, hipStream_t st);
 void launch_walk_src_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, float* minCEo,
                            float* lsO, float* VFm, hipStream_t st);
-void launch_walk_max(int64_t nruns, const Run* runs, const int2* sd, const float* CE, const float* VR, int Q, int Qs,
-                     const QStat* qs, float* FMaxN, double* carry, hipStream_t st);
+void launch_walk_max(int64_t nruns, const Run* runs, const int2* sd, const float* CE, const float* VR, int VRs, int Q,
+                     int Qs, const QStat* qs, float* FMaxN, double* carry, hipStream_t st);
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
                            float* FMaxN, hipStream_t st);
 void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st);

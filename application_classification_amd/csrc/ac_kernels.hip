@@ -13,7 +13,9 @@
 // every later log-softmax removes again, so the raw costs are used.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "ac_internal.hpp"
 
@@ -23,14 +25,42 @@ namespace {
 
 constexpr double kInf = __builtin_huge_val();
 
-__device__ __forceinline__ void lse_add(Lse& s, double x) {
-  if (x == -kInf) return;
-  if (x > s.m) {
-    s.t = (s.m == -kInf) ? 0.0 : (1.0 + s.t) * (double)expf((float)(s.m - x));
-    s.m = x;
-  } else {
-    s.t += (double)expf((float)(x - s.m));
+// Edge costs are row-major [m][Qs] (CSC order).  The edge walk reads them in batches of B consecutive rows starting at
+// multiples of B, where B rows are a whole number of 128-byte lines, so no line is shared by two batches.
+__device__ __forceinline__ int64_t ce_index(int64_t k, int q, int Qs) { return k * Qs + q; }
+
+int walk_batch(int Qs) {
+  if (const char* e = std::getenv("AC_WALK_BATCH")) {  // tuning override: 1, 2, 4 or 8
+    const int b = std::atoi(e);
+    if (b == 1 || b == 2 || b == 4 || b == 8) return b;
   }
+  int b = 1;
+  while ((b * Qs * 4) % 128 != 0 && b < 8) b *= 2;
+  return b;
+}
+
+// Streaming loads: data read once per pass (edge costs, edge ids) is loaded non-temporally so that it does not
+// evict the gathered rows that are reused from L2.
+__device__ __forceinline__ float4 nt_load_float4(const float4* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 x = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+  return make_float4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ int2 nt_load_int2(const int2* p) {
+  typedef int v2 __attribute__((ext_vector_type(2)));
+  const v2 x = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  return make_int2(x.x, x.y);
+}
+
+// One exp per value and no branches: the exponent is always min - max of (x, m), which rescales t when x is a new
+// maximum and is the new term otherwise.  exp(-inf) = 0 covers the empty accumulator.
+__device__ __forceinline__ void lse_add(Lse& s, double x) {
+  const bool valid = x != -kInf;
+  const bool up = x > s.m;
+  const double e = (double)__expf((float)(fmin(x, s.m) - fmax(x, s.m)));
+  const double t = up ? (1.0 + s.t) * e : s.t + e;
+  s.t = valid ? t : s.t;
+  s.m = valid ? fmax(x, s.m) : s.m;
 }
 
 __device__ __forceinline__ Lse lse_merge(Lse a, Lse b) {
@@ -49,66 +79,93 @@ __device__ __forceinline__ Lse lse_empty() { return Lse{-kInf, 0.0}; }
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------------------------
-// cdist (src/ac.hxx:33-56), FP64 accumulation of (a - b)^2, rounded once to FP32.
+// cdist (src/ac.hxx:33-56): out[r][c] = || feats[row r] - pat_c ||, FP64 accumulation of (a - b)^2 rounded once to
+// FP32.  Pattern features are [dim][Kp] doubles (Kp = K rounded up to 8, zero padded); row r of the output reads
+// feature row idx[r] (idx == nullptr: row r).  Columns K <= c < Kstore of the output are written as 0.
 
-__global__ void k_cdist_vertex(int64_t n, int P, int dV, const float* __restrict__ fd,
-                               const double* __restrict__ pf, float* __restrict__ CV) {
-  const int64_t total = n * P;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = i / P;
-    const int p = (int)(i - v * P);
-    const float* a = fd + v * dV;
-    const double* b = pf + (int64_t)p * dV;
-    double acc = 0.0;
-    for (int k = 0; k < dV; ++k) {
-      double d = (double)a[k] - b[k];
-      acc = fma(d, d, acc);
+constexpr int kCdTile = 128;
+
+// A tile of kCdTile feature rows is staged in LDS with row-contiguous (coalesced) loads; each thread then computes
+// one row's distances to eight patterns at a time, the pattern values being wave-uniform (scalar loads).
+__global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int* __restrict__ idx,
+                                                        const float* __restrict__ feats, int dim,
+                                                        const double* __restrict__ pat, int K, int Kp, int Kstore,
+                                                        float* __restrict__ out, int out_stride, int tile,
+                                                        int quad) {
+  extern __shared__ float rows[];  // [tile][dim + 1], tile <= kCdTile
+  const int ld = dim + 1;
+  for (int64_t base = blockIdx.x * (int64_t)tile; base < nrows; base += (int64_t)gridDim.x * tile) {
+    const int nt = (int)((nrows - base) < tile ? (nrows - base) : tile);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nt * dim; e += kCdTile) {
+      const int r = e / dim, c = e - r * dim;
+      const int64_t src = idx ? (int64_t)idx[base + r] : base + r;
+      rows[r * ld + c] = feats[src * dim + c];
     }
-    CV[i] = (float)sqrt(acc);
-  }
-}
-
-void launch_cdist_vertex(int64_t n, int P, int dV, const float* fd, const double* pf, float* CV, hipStream_t st) {
-  int64_t total = n * P;
-  if (total == 0) return;
-  int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
-  hipLaunchKernelGGL(k_cdist_vertex, dim3(blocks), dim3(256), 0, st, n, P, dV, fd, pf, CV);
-}
-
-// One thread per CSC edge k computes the whole cost row CE[k][0..Qs) for edge eid[k], four pattern edges at a time;
-// pattern features pf are [dE][Qs] doubles (zero in padding columns, whose costs are written as 0).
-__global__ void k_cdist_edge(int64_t m, int Q, int Qs, int dE, const float* __restrict__ fe,
-                             const int* __restrict__ eid, const double* __restrict__ pf, float* __restrict__ CE) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
-    const float* g = fe + (int64_t)eid[k] * dE;
-    float4* out = reinterpret_cast<float4*>(CE + k * Qs);
-    for (int q0 = 0; q0 < Qs; q0 += 4) {
-      double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-      const double* b = pf + q0;
-      for (int i = 0; i < dE; ++i) {
-        const double x = (double)g[i];
-        const double* bi = b + (int64_t)i * Qs;
-        double d0 = x - bi[0], d1 = x - bi[1], d2 = x - bi[2], d3 = x - bi[3];
-        a0 = fma(d0, d0, a0);
-        a1 = fma(d1, d1, a1);
-        a2 = fma(d2, d2, a2);
-        a3 = fma(d3, d3, a3);
+    __syncthreads();
+    if ((int)threadIdx.x < nt) {
+      const float* x = rows + threadIdx.x * ld;
+      const int64_t row = base + threadIdx.x;
+      // Row-major: column c at o[c]; quad layout: column c at o[4 c] (see ce_index).
+      float* o = out + row * out_stride;
+      const int cs = 1;
+      (void)quad;
+      for (int k0 = 0; k0 < Kstore; k0 += 8) {
+        double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < dim; ++i) {
+          const double xi = (double)x[i];
+          const double* pv = pat + (int64_t)i * Kp + k0;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            const double d = xi - pv[c];
+            a[c] = fma(d, d, a[c]);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int k = k0 + c;
+          if (k < Kstore) o[k * cs] = k < K ? (float)sqrt(a[c]) : 0.f;
+        }
       }
-      float4 r;
-      r.x = q0 + 0 < Q ? (float)sqrt(a0) : 0.f;
-      r.y = q0 + 1 < Q ? (float)sqrt(a1) : 0.f;
-      r.z = q0 + 2 < Q ? (float)sqrt(a2) : 0.f;
-      r.w = q0 + 3 < Q ? (float)sqrt(a3) : 0.f;
-      out[q0 >> 2] = r;
     }
   }
 }
 
-void launch_cdist_edge(int64_t m, int Q, int Qs, int dE, const float* fe, const int* eid, const double* pf,
-                       float* CE, hipStream_t st) {
-  if (m == 0) return;
-  int blocks = (int)std::min<int64_t>((m + 255) / 256, 1 << 20);
-  hipLaunchKernelGGL(k_cdist_edge, dim3(blocks), dim3(256), 0, st, m, Q, Qs, dE, fe, eid, pf, CE);
+// Fallback for feature widths too large to stage: one thread per output element, rows read from global memory.
+__global__ void k_cdist_direct(int64_t nrows, const int* __restrict__ idx, const float* __restrict__ feats, int dim,
+                               const double* __restrict__ pat, int K, int Kp, int Kstore, float* __restrict__ out,
+                               int out_stride, int quad) {
+  const int64_t total = nrows * Kstore;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / Kstore;
+    const int k = (int)(e - r * Kstore);
+    const float* x = feats + (idx ? (int64_t)idx[r] : r) * dim;
+    double a = 0.0;
+    if (k < K)
+      for (int i = 0; i < dim; ++i) {
+        const double d = (double)x[i] - pat[(int64_t)i * Kp + k];
+        a = fma(d, d, a);
+      }
+    const int64_t o = quad ? ce_index(r, k, out_stride) : r * out_stride + k;
+    out[o] = k < K ? (float)sqrt(a) : 0.f;
+  }
+}
+
+void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, const double* pat, int K, int Kp,
+                  int Kstore, float* out, int out_stride, int quad, hipStream_t st) {
+  if (nrows == 0 || Kstore == 0) return;
+  if (dim <= 255) {
+    // At most 64 KB of LDS: 128 rows up to 127 features, fewer rows beyond.
+    const int tile = std::min(kCdTile, 16384 / (dim + 1));
+    const size_t lds = (size_t)tile * (dim + 1) * sizeof(float);
+    const int blocks = (int)std::min<int64_t>((nrows + tile - 1) / tile, 1 << 16);
+    hipLaunchKernelGGL(k_cdist_rows, dim3(blocks), dim3(kCdTile), lds, st, nrows, idx, feats, dim, pat, K, Kp,
+                       Kstore, out, out_stride, tile, quad);
+  } else {
+    const int blocks = (int)std::min<int64_t>((nrows * Kstore + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_cdist_direct, dim3(blocks), dim3(256), 0, st, nrows, idx, feats, dim, pat, K, Kp, Kstore,
+                       out, out_stride, quad);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -250,7 +307,6 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_src(int64_t nruns, const Ru
   Lse glob[4] = {lse_empty(), lse_empty(), lse_empty(), lse_empty()};
   if (g < G && r < nruns) {
     const Run R = runs[r];
-    const float4* CE4 = reinterpret_cast<const float4*>(CE);
     Lse s[4] = {lse_empty(), lse_empty(), lse_empty(), lse_empty()};
     int cur = ks[R.kb].y;
     auto emit = [&](int v) {
@@ -273,7 +329,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_src(int64_t nruns, const Ru
         emit(cur);
         cur = e.y;
       }
-      const float4 c = CE4[(int64_t)e.x * L4 + j];
+      const float4 c = reinterpret_cast<const float4*>(CE + ce_index(e.x, 4 * j, Qs))[0];
       lse_add(s[0], -(double)c.x);
       lse_add(s[1], -(double)c.y);
       lse_add(s[2], -(double)c.z);
@@ -309,49 +365,62 @@ void launch_walk_src(int64_t nruns, const Run* runs, const int2* ks, const float
                      VFm, carry, gpart);
 }
 
-__global__ void k_walk_src_fixup(int nsplit, const Split* __restrict__ splits, const double* __restrict__ carry,
-                                 int Q, int Qs, float* __restrict__ minCEo, float* __restrict__ lsO,
-                                 float* __restrict__ VFm) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= (int64_t)nsplit * Qs) return;
-  const int sidx = (int)(i / Qs), q = (int)(i - (int64_t)sidx * Qs);
-  const Split S = splits[sidx];
-  Lse s = lse_empty();
-  for (int c = 0; c < S.count; ++c) {
-    const int64_t o = ((int64_t)(S.first + c) * Qs + q) * 2;
-    s = lse_merge(s, Lse{carry[o], carry[o + 1]});
-  }
-  const int64_t o = (int64_t)S.v * Qs + q;
-  if (q < Q) {
-    minCEo[o] = (float)(-s.m);
-    lsO[o] = (float)log1p(s.t);
-    VFm[o] = (float)s.m;
-  } else {
-    minCEo[o] = 0.f;
-    lsO[o] = 0.f;
-    VFm[o] = 0.f;
+// One wave per split vertex: groups of Qs/4 lanes take its pieces round-robin (lane j owns pattern edges
+// 4j..4j+3), then the groups' results are merged in group order through LDS.  Deterministic.
+__global__ __launch_bounds__(256) void k_walk_src_fixup(int nsplit, const Split* __restrict__ splits,
+                                                        const double* __restrict__ carry, int Q, int Qs,
+                                                        float* __restrict__ minCEo, float* __restrict__ lsO,
+                                                        float* __restrict__ VFm) {
+  __shared__ Lse red[4][64][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int L4 = Qs / 4, G = 64 / L4;
+  const int g = lane / L4, j = lane - g * L4;
+  const int sidx = blockIdx.x * 4 + wave;
+  const bool active = sidx < nsplit && g < G;
+  Lse s[4] = {lse_empty(), lse_empty(), lse_empty(), lse_empty()};
+  Split S{0, 0, 0, 0};
+  if (sidx < nsplit) S = splits[sidx];
+  if (active)
+    for (int c = g; c < S.count; c += G) {
+      const double* o = carry + ((int64_t)(S.first + c) * Qs + 4 * j) * 2;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[u] = lse_merge(s[u], Lse{o[2 * u], o[2 * u + 1]});
+    }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) red[wave][lane][u] = s[u];
+  __syncthreads();
+  if (sidx < nsplit && g == 0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      Lse acc = lse_empty();
+      for (int gg = 0; gg < G; ++gg) acc = lse_merge(acc, red[wave][gg * L4 + j][u]);
+      const int q = 4 * j + u;
+      const int64_t o = (int64_t)S.v * Qs + q;
+      minCEo[o] = q < Q ? (float)(-acc.m) : 0.f;
+      lsO[o] = q < Q ? (float)log1p(acc.t) : 0.f;
+      VFm[o] = q < Q ? (float)acc.m : 0.f;
+    }
   }
 }
 
 void launch_walk_src_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, float* minCEo,
                            float* lsO, float* VFm, hipStream_t st) {
-  int64_t total = (int64_t)nsplit * Qs;
-  if (total == 0) return;
-  hipLaunchKernelGGL(k_walk_src_fixup, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, nsplit, splits,
-                     carry, Q, Qs, minCEo, lsO, VFm);
+  if (nsplit == 0) return;
+  hipLaunchKernelGGL(k_walk_src_fixup, dim3((unsigned)((nsplit + 3) / 4)), dim3(256), 0, st, nsplit, splits, carry,
+                     Q, Qs, minCEo, lsO, VFm);
 }
 
 // Destination maxima (CSC walk), src/main.cu:220-224 with the log-softmax of src/main.cu:218 folded in:
 //   FMaxN[v][q] = max over in-edges e of v of (VR[src e][q] - CE[e][q])  -  log sum_e exp(FE[q][e])
 // computed exactly in FP64 (a difference of two floats is exact in double) and rounded once.  With VR == nullptr
 // it is the prep's max over in-edges of -CE (src/main.cu:143-146).
-template <bool kWithVR>
+template <bool kWithVR, int B>
 __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nruns, const Run* __restrict__ runs,
                                                          const int2* __restrict__ sd, const float* __restrict__ CE,
-                                                         const float* __restrict__ VR, int Q, int Qs,
+                                                         const float* __restrict__ VR, int VRs, int Q, int Qs,
                                                          const QStat* __restrict__ qs, float* __restrict__ FMaxN,
                                                          double* __restrict__ carry) {
-  const int L4 = Qs / 4, G = 64 / L4;
+  const int L4 = Qs / 4, G = 64 / L4, VR4s = VRs / 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane / L4, j = lane - g * L4;
   const int64_t r = ((int64_t)blockIdx.x * (kWalkBlock / 64) + wave) * G + g;
@@ -359,38 +428,43 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nruns, const Ru
   const Run R = runs[r];
   const float4* CE4 = reinterpret_cast<const float4*>(CE);
   const float4* VR4 = reinterpret_cast<const float4*>(VR);
-  double sm[4], sl[4];
+  // log sum_e exp(FE[q][e]) as one FP64 value (its FP64 rounding is far below the FP32 rounding of the result).
+  double sl[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int q = 4 * j + c;
-    sm[c] = q < Q ? qs[q].mFE : 0.0;
-    sl[c] = q < Q ? qs[q].lFE : 0.0;
+    sl[c] = q < Q ? qs[q].mFE + qs[q].lFE : 0.0;
   }
   double mx0 = -kInf, mx1 = -kInf, mx2 = -kInf, mx3 = -kInf;
   int cur = sd[R.kb].y;
   auto emit = [&](int v) {
     float4 o;
-    o.x = (float)((mx0 - sm[0]) - sl[0]);
-    o.y = (float)((mx1 - sm[1]) - sl[1]);
-    o.z = (float)((mx2 - sm[2]) - sl[2]);
-    o.w = (float)((mx3 - sm[3]) - sl[3]);
+    o.x = (float)(mx0 - sl[0]);
+    o.y = (float)(mx1 - sl[1]);
+    o.z = (float)(mx2 - sl[2]);
+    o.w = (float)(mx3 - sl[3]);
     reinterpret_cast<float4*>(FMaxN)[(int64_t)v * L4 + j] = o;
     mx0 = mx1 = mx2 = mx3 = -kInf;
   };
-  int k = R.kb;
-  // Four edges per step: their ids, costs and gathered rows are all requested before any is used.
-  for (; k + 4 <= R.ke; k += 4) {
-    int2 e[4];
-    float4 c[4], w[4];
+  // One batch of B consecutive edges per step (batches start at multiples of B, so their cost rows are whole cache
+  // lines); the costs, ids and gathered rows of the batch are all requested before any is used.  Edges of the
+  // batch outside the run are skipped.
+  for (int64_t bt = R.kb / B; bt <= (int64_t)(R.ke - 1) / B; ++bt) {
+    int2 e[B];
+    float4 c[B], w[B];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) e[u] = sd[k + u];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      c[u] = CE4[(int64_t)(k + u) * L4 + j];
-      w[u] = kWithVR ? VR4[(int64_t)e[u].x * L4 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int u = 0; u < B; ++u) {
+      const int64_t k = bt * B + u;
+      const bool in = k >= R.kb && k < R.ke;
+      e[u] = in ? nt_load_int2(sd + k) : make_int2(0, -1);
+      c[u] = in ? nt_load_float4(CE4 + k * L4 + j) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < B; ++u)
+      w[u] = (kWithVR && e[u].y >= 0) ? VR4[(int64_t)e[u].x * VR4s + j] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      if (e[u].y < 0) continue;
       if (e[u].y != cur) {
         emit(cur);
         cur = e[u].y;
@@ -400,19 +474,6 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nruns, const Ru
       mx2 = fmax(mx2, (double)w[u].z - (double)c[u].z);
       mx3 = fmax(mx3, (double)w[u].w - (double)c[u].w);
     }
-  }
-  for (; k < R.ke; ++k) {
-    const int2 e = sd[k];
-    const float4 c = CE4[(int64_t)k * L4 + j];
-    const float4 w = kWithVR ? VR4[(int64_t)e.x * L4 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
-    if (e.y != cur) {
-      emit(cur);
-      cur = e.y;
-    }
-    mx0 = fmax(mx0, (double)w.x - (double)c.x);
-    mx1 = fmax(mx1, (double)w.y - (double)c.y);
-    mx2 = fmax(mx2, (double)w.z - (double)c.z);
-    mx3 = fmax(mx3, (double)w.w - (double)c.w);
   }
   if (R.slot >= 0) {
     double* o = carry + (int64_t)R.slot * Qs + 4 * j;
@@ -425,144 +486,254 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nruns, const Ru
   }
 }
 
-void launch_walk_max(int64_t nruns, const Run* runs, const int2* sd, const float* CE, const float* VR, int Q, int Qs,
-                     const QStat* qs, float* FMaxN, double* carry, hipStream_t st) {
-  if (nruns == 0) return;
-  const int nb = walk_blocks(nruns, Qs);
-  if (VR)
-    hipLaunchKernelGGL(k_walk_max<true>, dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, Q, Qs, qs,
-                       FMaxN, carry);
-  else
-    hipLaunchKernelGGL(k_walk_max<false>, dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, Q, Qs, qs,
-                       FMaxN, carry);
+template <bool kWithVR>
+static void launch_walk_max_t(int B, int nb, hipStream_t st, int64_t nruns, const Run* runs, const int2* sd,
+                              const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
+                              double* carry) {
+  switch (B) {
+    case 1:
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 1>), dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, VRs, Q,
+                         Qs, qs, FMaxN, carry);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 2>), dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, VRs, Q,
+                         Qs, qs, FMaxN, carry);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 4>), dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, VRs, Q,
+                         Qs, qs, FMaxN, carry);
+      break;
+    default:
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 8>), dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, VRs, Q,
+                         Qs, qs, FMaxN, carry);
+      break;
+  }
 }
 
-__global__ void k_walk_max_fixup(int nsplit, const Split* __restrict__ splits, const double* __restrict__ carry,
-                                 int Q, int Qs, const QStat* __restrict__ qs, float* __restrict__ FMaxN) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= (int64_t)nsplit * Qs) return;
-  const int sidx = (int)(i / Qs), q = (int)(i - (int64_t)sidx * Qs);
-  const Split S = splits[sidx];
-  double mx = -kInf;
-  for (int c = 0; c < S.count; ++c) mx = fmax(mx, carry[(int64_t)(S.first + c) * Qs + q]);
-  FMaxN[(int64_t)S.v * Qs + q] = q < Q ? (float)((mx - qs[q].mFE) - qs[q].lFE) : 0.f;
+void launch_walk_max(int64_t nruns, const Run* runs, const int2* sd, const float* CE, const float* VR, int VRs, int Q,
+                     int Qs, const QStat* qs, float* FMaxN, double* carry, hipStream_t st) {
+  if (nruns == 0) return;
+  const int nb = walk_blocks(nruns, Qs);
+  const int B = walk_batch(Qs);
+  if (VR)
+    launch_walk_max_t<true>(B, nb, st, nruns, runs, sd, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+  else
+    launch_walk_max_t<false>(B, nb, st, nruns, runs, sd, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+}
+
+// One wave per split vertex, as k_walk_src_fixup.
+__global__ __launch_bounds__(256) void k_walk_max_fixup(int nsplit, const Split* __restrict__ splits,
+                                                        const double* __restrict__ carry, int Q, int Qs,
+                                                        const QStat* __restrict__ qs, float* __restrict__ FMaxN) {
+  __shared__ double red[4][64][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int L4 = Qs / 4, G = 64 / L4;
+  const int g = lane / L4, j = lane - g * L4;
+  const int sidx = blockIdx.x * 4 + wave;
+  const bool active = sidx < nsplit && g < G;
+  double mx[4] = {-kInf, -kInf, -kInf, -kInf};
+  Split S{0, 0, 0, 0};
+  if (sidx < nsplit) S = splits[sidx];
+  if (active)
+    for (int c = g; c < S.count; c += G) {
+      const double4 o = *reinterpret_cast<const double4*>(carry + (int64_t)(S.first + c) * Qs + 4 * j);
+      mx[0] = fmax(mx[0], o.x);
+      mx[1] = fmax(mx[1], o.y);
+      mx[2] = fmax(mx[2], o.z);
+      mx[3] = fmax(mx[3], o.w);
+    }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) red[wave][lane][u] = mx[u];
+  __syncthreads();
+  if (sidx < nsplit && g == 0) {
+    float4 out;
+    float* po = reinterpret_cast<float*>(&out);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double acc = -kInf;
+      for (int gg = 0; gg < G; ++gg) acc = fmax(acc, red[wave][gg * L4 + j][u]);
+      const int q = 4 * j + u;
+      po[u] = q < Q ? (float)((acc - qs[q].mFE) - qs[q].lFE) : 0.f;
+    }
+    reinterpret_cast<float4*>(FMaxN)[(int64_t)S.v * L4 + j] = out;
+  }
 }
 
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
                            float* FMaxN, hipStream_t st) {
-  int64_t total = (int64_t)nsplit * Qs;
-  if (total == 0) return;
-  hipLaunchKernelGGL(k_walk_max_fixup, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, nsplit, splits,
-                     carry, Q, Qs, qs, FMaxN);
+  if (nsplit == 0) return;
+  hipLaunchKernelGGL(k_walk_max_fixup, dim3((unsigned)((nsplit + 3) / 4)), dim3(256), 0, st, nsplit, splits, carry,
+                     Q, Qs, qs, FMaxN);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Vertex passes.  A block walks tiles of VT consecutive vertices in three phases:
-//   1. threads mapped to (q, row): FMax = max(fillF, FMaxN), RMax = max(fillR, VFm - LSE_RE) into LDS (FP64);
-//   2. threads mapped to (p, row): MU_pre = -CV + sum of the FMax / RMax terms of p in ascending q (FMax before
-//      RMax, src/ac.hxx:164-170 made deterministic); MU = MU_pre - LSE_MU;
-//   3. (mode A) threads mapped to (q, row): VF = MU[pd] - FMax, VR = MU[ps] - RMax (src/main.cu:165-176).
-// Every global access is row-contiguous in the thread index, so loads and stores coalesce.
+// Vertex passes.  A block takes tiles of VT (<= 64) consecutive vertices; the tile's FMaxN, VFm and CV rows
+// (contiguous in HBM) are staged in LDS as FP32 with coalesced float4 loads, into rows padded to an odd stride.
+//   p-phase : wave w handles pattern vertices p = w, w + 4, ...; lane r is tile row r, so the pattern's incidence
+//             structure is wave-uniform (no divergence).  MU_pre = -CV + sum over the incidences of p in ascending q
+//             (FMax term before RMax term; src/ac.hxx:164-170 made deterministic), FMax = max(fillF, FMaxN),
+//             RMax = max(fillR, VFm - LSE_RE).  Mode A keeps MU = MU_pre - LSE_MU in LDS (FP64), mode OUT writes it
+//             in dn x pn order, mode C reduces the column's log-sum-exp across the wave.
+//   q-phase : (mode A) a thread per (q, row): VF = MU[pd] - FMax, VR = MU[ps] - RMax (src/main.cu:165-176), stored
+//             with coalesced writes, and the per-q maxima / edge log-sum-exps accumulated in registers.
+
+constexpr int kVTMax = 64;
+
+static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
+  size_t b = (size_t)(4 * Q + 2 * P) * sizeof(double);                       // tables
+  if (mode == kModeA) b += (size_t)P * VT * sizeof(double);                  // MU
+  if (mode == kModeC) b += (size_t)P * sizeof(Lse);                          // column accumulators
+  b += (size_t)VT * (2 * (Qs + 1) + (P + 1)) * sizeof(float);                // staged rows
+  b += (size_t)(2 * Q + P + 1) * sizeof(int);                                // incidence lists
+  return (b + 15) / 16 * 16;
+}
 
 int vertex_tile(int P, int Q) {
-  int vt = 65536 / ((2 * Q + P) * 8);
-  return vt < 1 ? 1 : (vt > 64 ? 64 : vt);
+  const int Qs = std::max(4, (Q + 3) / 4 * 4);
+  int vt = kVTMax;
+  while (vt > 8 && vertex_lds_bytes(kModeA, vt, P, Q, Qs) > 65536) vt /= 2;
+  return vt;
+}
+
+// log-sum-exp of one value per lane over the wave (-inf lanes contribute nothing); fixed butterfly order.
+__device__ __forceinline__ Lse wave_lse(double x) {
+  double m = x;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (m == -kInf) return lse_empty();
+  const unsigned long long at_max = __ballot(x == m);
+  const int first = __ffsll((long long)at_max) - 1;
+  double e = ((int)(threadIdx.x & 63) == first) ? 0.0 : (double)__expf((float)(x - m));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+  return Lse{m, e};
 }
 
 template <int kMode>
 __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
   extern __shared__ double lds[];
   const int Q = a.Q, P = a.P, Qs = a.Qs, VT = a.VT;
-  double* Fm = lds;              // [VT][Q]
-  double* Rm = Fm + VT * Q;      // [VT][Q]
-  double* MUs = Rm + VT * Q;     // [VT][P]
-  const int t = threadIdx.x;
-  const int RQ = Q > 0 ? kVertexBlock / Q : 0;
-  const int RP = kVertexBlock / P;
-  const bool qthread = t < RQ * Q;
-  const int q = qthread ? t % Q : 0, rq = qthread ? t / Q : 0;
-  const bool pthread = t < RP * P;
-  const int p = pthread ? t % P : 0, rp = pthread ? t / P : 0;
+  const int ldq = Qs + 1, ldp = P + 1;
+  double* tab = lds;                                   // [4][Q]: fillF, fillR, mRE, lRE
+  double* pmu = tab + 4 * Q;                           // [2][P]: m, l of LSE_MU
+  double* MUs = pmu + 2 * P;                           // [P][VT] (mode A)
+  Lse* lacc = reinterpret_cast<Lse*>(MUs + (kMode == kModeA ? P * VT : 0));  // [P] (mode C)
+  float* Fs = reinterpret_cast<float*>(lacc + (kMode == kModeC ? P : 0));    // [VT][ldq]
+  float* Vs = Fs + VT * ldq;                           // [VT][ldq]
+  float* Cs = Vs + VT * ldq;                           // [VT][ldp]
+  int* incs = reinterpret_cast<int*>(Cs + VT * ldp);   // [2Q]
+  int* incoff = incs + 2 * Q;                          // [P + 1]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const bool sums = !a.first;
 
-  QStat s{};
-  int q_ps = 0, q_pd = 0;
-  if (qthread) {
-    s = a.qs[q];
-    q_ps = a.ps[q];
-    q_pd = a.pd[q];
+  for (int i = t; i < 2 * Q; i += kVertexBlock) incs[i] = a.inc[i];
+  for (int i = t; i <= P; i += kVertexBlock) incoff[i] = a.inc_off[i];
+  for (int i = t; i < Q; i += kVertexBlock) {
+    const QStat x = a.qs[i];
+    tab[i] = x.fillF;
+    tab[Q + i] = x.fillR;
+    tab[2 * Q + i] = x.mRE;
+    tab[3 * Q + i] = x.lRE;
   }
-  Lse2 ms{0.0, 0.0};
-  int ib = 0, ie = 0;
-  if (pthread) {
-    if (kMode != kModeC) ms = a.pst[p];
-    ib = a.inc_off[p];
-    ie = a.inc_off[p + 1];
+  for (int i = t; i < P; i += kVertexBlock) {
+    if (kMode != kModeC) {
+      pmu[i] = a.pst[i].m;
+      pmu[P + i] = a.pst[i].l;
+    } else {
+      lacc[i] = lse_empty();
+    }
   }
-  double vfmax = -kInf, vrmax = -kInf;
-  Lse lre = lse_empty(), lfe = lse_empty(), lmu = lse_empty();
 
+  const int RQ = Q > 0 ? kVertexBlock / Q : 0;
+  const bool qthread = t < RQ * Q;
+  const int q = qthread ? t % Q : 0, rq = qthread ? t / Q : 0;
+  double vfmax = -kInf, vrmax = -kInf;
+  Lse lre = lse_empty(), lfe = lse_empty();
+
+  const float4* F4 = reinterpret_cast<const float4*>(a.FMaxN);
+  const float4* V4 = reinterpret_cast<const float4*>(a.VFm);
   const int64_t ntiles = (a.n + VT - 1) / VT;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t v0 = tile * VT;
     const int nv = (int)((a.n - v0) < VT ? (a.n - v0) : VT);
-    if (qthread) {
-      for (int r = rq; r < nv; r += RQ) {
-        const int64_t i = (v0 + r) * Qs + q;
-        const double fm = fmax(s.fillF, (double)a.FMaxN[i]);
-        const float vfm = a.VFm[i];
-        const double rn = vfm == -INFINITY ? -kInf : ((double)vfm - s.mRE) - s.lRE;
-        Fm[r * Q + q] = fm;
-        Rm[r * Q + q] = fmax(s.fillR, rn);
+    __syncthreads();  // tables written (first tile) / previous tile consumed
+    {
+      const int cnt4 = nv * Qs / 4;
+      const int64_t b4 = v0 * Qs / 4;
+      for (int i = t; i < cnt4; i += kVertexBlock) {
+        const float4 f = F4[b4 + i], v = V4[b4 + i];
+        const int e = 4 * i, r = e / Qs, c = e - r * Qs;
+        float* fr = Fs + r * ldq + c;
+        float* vr = Vs + r * ldq + c;
+        fr[0] = f.x, fr[1] = f.y, fr[2] = f.z, fr[3] = f.w;
+        vr[0] = v.x, vr[1] = v.y, vr[2] = v.z, vr[3] = v.w;
+      }
+      for (int i = t; i < nv * P; i += kVertexBlock) {
+        const int r = i / P, c = i - r * P;
+        Cs[r * ldp + c] = a.CV[v0 * P + i];
       }
     }
     __syncthreads();
-    if (pthread) {
-      for (int r = rp; r < nv; r += RP) {
-        double acc = -(double)a.CV[(v0 + r) * P + p];
+    // p-phase: wave-uniform pattern vertex, lane = tile row.
+    for (int p = wave; p < P; p += kVertexBlock / 64) {
+      const int r = lane;
+      double x = -kInf;
+      if (r < nv) {
+        double acc = -(double)Cs[r * ldp + p];
         if (sums)
-          for (int x = ib; x < ie; ++x) {
-            const int code = a.inc[x];
-            acc += (code & 1) ? Rm[r * Q + (code >> 1)] : Fm[r * Q + (code >> 1)];
+          for (int k = incoff[p]; k < incoff[p + 1]; ++k) {
+            const int code = incs[k];
+            const int qq = code >> 1;
+            if (code & 1) {
+              const float vfm = Vs[r * ldq + qq];
+              const double rn = vfm == -INFINITY ? -kInf : ((double)vfm - tab[2 * Q + qq]) - tab[3 * Q + qq];
+              acc += fmax(tab[Q + qq], rn);
+            } else {
+              acc += fmax(tab[qq], (double)Fs[r * ldq + qq]);
+            }
           }
-        if (kMode == kModeC) {
-          lse_add(lmu, acc);
-        } else {
-          const double mu = (acc - ms.m) - ms.l;
-          if (kMode == kModeA)
-            MUs[r * P + p] = mu;
-          else
-            a.out[(v0 + r) * P + p] = (float)mu;
-        }
+        x = acc;
+      }
+      if (kMode == kModeC) {
+        const Lse w = wave_lse(x);
+        if (lane == 0) lacc[p] = lse_merge(lacc[p], w);
+      } else if (r < nv) {
+        const double mu = (x - pmu[p]) - pmu[P + p];
+        if (kMode == kModeA)
+          MUs[p * VT + r] = mu;
+        else
+          a.out[(v0 + r) * P + p] = (float)mu;
       }
     }
     if (kMode == kModeA) {
       __syncthreads();
       if (qthread) {
+        const double fF = tab[q], fR = tab[Q + q], mR = tab[2 * Q + q], lR = tab[3 * Q + q];
+        const int q_ps = a.ps[q], q_pd = a.pd[q];
         for (int r = rq; r < nv; r += RQ) {
           const int64_t i = (v0 + r) * Qs + q;
-          const double vf = MUs[r * P + q_pd] - Fm[r * Q + q];
-          const double vr = MUs[r * P + q_ps] - Rm[r * Q + q];
-          a.VR[i] = (float)vr;
+          const float mo = a.minCEo[i];
+          const float lo = a.lsO[i];
+          const double fm = fmax(fF, (double)Fs[r * ldq + q]);
+          const float vfm = Vs[r * ldq + q];
+          const double rm = fmax(fR, vfm == -INFINITY ? -kInf : ((double)vfm - mR) - lR);
+          const double vf = MUs[q_pd * VT + r] - fm;
+          const double vr = MUs[q_ps * VT + r] - rm;
+          a.VR[(v0 + r) * a.VRs + q] = (float)vr;
           vfmax = fmax(vfmax, vf);
           vrmax = fmax(vrmax, vr);
-          const float mo = a.minCEo[i];
-          if (mo != INFINITY) {
-            const double lo = (double)a.lsO[i];
-            const double x = vf - (double)mo;
-            a.VFm[i] = (float)x;
-            lse_add(lre, x + lo);
-            lse_add(lfe, (vr - (double)mo) + lo);
-          } else {
-            a.VFm[i] = -INFINITY;
-          }
+          const bool out_edges = mo != INFINITY;
+          const double xx = vf - (double)mo;
+          a.VFm[i] = out_edges ? (float)xx : -INFINITY;
+          lse_add(lre, out_edges ? xx + (double)lo : -kInf);
+          lse_add(lfe, out_edges ? (vr - (double)mo) + (double)lo : -kInf);
         }
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
 
-  // Block partials, merged over the rows of each column in a fixed order.
+  // Block partials, merged in a fixed order.
   if (kMode == kModeA) {
     QPart* red = reinterpret_cast<QPart*>(lds);
     if (qthread) red[rq * Q + q] = QPart{vfmax, vrmax, lre, lfe};
@@ -579,21 +750,14 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
       a.qpart[(int64_t)blockIdx.x * Q + t] = acc;
     }
   } else if (kMode == kModeC) {
-    Lse* red = reinterpret_cast<Lse*>(lds);
-    if (pthread) red[rp * P + p] = lmu;
-    __syncthreads();
-    if (t < P) {
-      Lse acc = lse_empty();
-      for (int r = 0; r < RP; ++r) acc = lse_merge(acc, red[r * P + t]);
-      a.ppart[(int64_t)blockIdx.x * P + t] = acc;
-    }
+    for (int i = t; i < P; i += kVertexBlock) a.ppart[(int64_t)blockIdx.x * P + i] = lacc[i];
   }
 }
 
 void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st) {
-  size_t tiles = (size_t)a.VT * (2 * a.Q + a.P) * sizeof(double);
-  size_t red = std::max(sizeof(QPart) * kVertexBlock, sizeof(Lse) * kVertexBlock);
-  size_t lds = std::max(tiles, red);
+  const size_t tiles = vertex_lds_bytes(mode, a.VT, a.P, a.Q, a.Qs);
+  const size_t red = mode == kModeA ? sizeof(QPart) * kVertexBlock : 0;
+  const size_t lds = std::max(tiles, red);
   if (mode == kModeA)
     hipLaunchKernelGGL(k_vertex<kModeA>, dim3(nblocks), dim3(kVertexBlock), lds, st, a);
   else if (mode == kModeC)
