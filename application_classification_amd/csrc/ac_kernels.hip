@@ -39,6 +39,9 @@ int walk_batch(int Qs) {
   return b;
 }
 
+// Native 4-wide vector (register-resident across loop iterations, unlike the HIP float4 wrapper struct).
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 // Streaming loads: data read once per pass (edge costs, edge ids) is loaded non-temporally so that it does not
 // evict the gathered rows that are reused from L2.
 __device__ __forceinline__ float4 nt_load_float4(const float4* p) {
@@ -446,35 +449,61 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nruns, const Ru
     reinterpret_cast<float4*>(FMaxN)[(int64_t)v * L4 + j] = o;
     mx0 = mx1 = mx2 = mx3 = -kInf;
   };
-  // One batch of B consecutive edges per step (batches start at multiples of B, so their cost rows are whole cache
-  // lines); the costs, ids and gathered rows of the batch are all requested before any is used.  Edges of the
-  // batch outside the run are skipped.
-  for (int64_t bt = R.kb / B; bt <= (int64_t)(R.ke - 1) / B; ++bt) {
-    int2 e[B];
-    float4 c[B], w[B];
+  // Batches of B consecutive edges (batches start at multiples of B, so their cost rows are whole cache lines),
+  // software-pipelined so that every wait overlaps other loads: while batch bt is reduced, the costs and gathered
+  // rows of batch bt + 1 and the ids of batch bt + 2 are already in flight.  Edges outside the run are skipped.
+  typedef int i2v __attribute__((ext_vector_type(2)));
+  const f4v* CEv = reinterpret_cast<const f4v*>(CE);
+  const f4v* VRv = reinterpret_cast<const f4v*>(VR);
+  const i2v* sdv = reinterpret_cast<const i2v*>(sd);
+  const int64_t bfirst = R.kb / B, blast = (int64_t)(R.ke - 1) / B;
+#define AC_WALK_IDS(E, BT)                                                                 \
+  _Pragma("unroll") for (int u = 0; u < B; ++u) {                                          \
+    const int64_t k_ = (BT) * B + u;                                                       \
+    E[u] = (k_ >= R.kb && k_ < R.ke) ? __builtin_nontemporal_load(sdv + k_) : i2v{0, -1};  \
+  }
+#define AC_WALK_COSTS(C, BT)                                                                         \
+  _Pragma("unroll") for (int u = 0; u < B; ++u) {                                                    \
+    const int64_t k_ = (BT) * B + u;                                                                 \
+    C[u] = (k_ >= R.kb && k_ < R.ke) ? __builtin_nontemporal_load(CEv + k_ * L4 + j) : f4v{0, 0, 0, 0}; \
+  }
+#define AC_WALK_GATHER(W, E)                                                                        \
+  _Pragma("unroll") for (int u = 0; u < B; ++u) {                                                   \
+    W[u] = (kWithVR && E[u].y >= 0) ? VRv[(int64_t)E[u].x * VR4s + j] : f4v{0, 0, 0, 0};            \
+  }
+  i2v e0[B], e1[B], e2[B];
+  f4v c0[B], c1[B], w0[B], w1[B];
+  AC_WALK_IDS(e0, bfirst);
+  AC_WALK_IDS(e1, bfirst + 1);
+  AC_WALK_COSTS(c0, bfirst);
+  AC_WALK_GATHER(w0, e0);
+  for (int64_t bt = bfirst; bt <= blast; ++bt) {
+    AC_WALK_COSTS(c1, bt + 1);
+    AC_WALK_GATHER(w1, e1);
+    AC_WALK_IDS(e2, bt + 2);
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      const int64_t k = bt * B + u;
-      const bool in = k >= R.kb && k < R.ke;
-      e[u] = in ? nt_load_int2(sd + k) : make_int2(0, -1);
-      c[u] = in ? nt_load_float4(CE4 + k * L4 + j) : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (e0[u].y < 0) continue;
+      if (e0[u].y != cur) {
+        emit(cur);
+        cur = e0[u].y;
+      }
+      mx0 = fmax(mx0, (double)w0[u].x - (double)c0[u].x);
+      mx1 = fmax(mx1, (double)w0[u].y - (double)c0[u].y);
+      mx2 = fmax(mx2, (double)w0[u].z - (double)c0[u].z);
+      mx3 = fmax(mx3, (double)w0[u].w - (double)c0[u].w);
     }
 #pragma unroll
-    for (int u = 0; u < B; ++u)
-      w[u] = (kWithVR && e[u].y >= 0) ? VR4[(int64_t)e[u].x * VR4s + j] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
     for (int u = 0; u < B; ++u) {
-      if (e[u].y < 0) continue;
-      if (e[u].y != cur) {
-        emit(cur);
-        cur = e[u].y;
-      }
-      mx0 = fmax(mx0, (double)w[u].x - (double)c[u].x);
-      mx1 = fmax(mx1, (double)w[u].y - (double)c[u].y);
-      mx2 = fmax(mx2, (double)w[u].z - (double)c[u].z);
-      mx3 = fmax(mx3, (double)w[u].w - (double)c[u].w);
+      e0[u] = e1[u];
+      e1[u] = e2[u];
+      c0[u] = c1[u];
+      w0[u] = w1[u];
     }
   }
+#undef AC_WALK_IDS
+#undef AC_WALK_COSTS
+#undef AC_WALK_GATHER
   if (R.slot >= 0) {
     double* o = carry + (int64_t)R.slot * Qs + 4 * j;
     o[0] = mx0;

@@ -72,7 +72,22 @@ def main():
     ap.add_argument("--config", default="s22", choices=["s22", "s18", "small"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (scales rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = 1 << 28
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied 3 x {n * 4} bytes", flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the matcher is a single-GPU path here (multi-GPU sharding is a design "
               "note, not built); run with --gpus 1.", file=sys.stderr)
