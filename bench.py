@@ -64,6 +64,93 @@ def cpu_baseline(threads):
     }
 
 
+STAGE_OF_KERNEL = {"k_walk_max<true": "edge_walk", "k_vertex<0>": "vertex_a", "k_vertex<1>": "vertex_c",
+                   "k_vertex<2>": "output", "k_walk_max_fixup": "walk_fixup", "k_reduce_q": "reduce_a",
+                   "k_reduce_lse": "reduce_c"}
+
+
+def _short(name):
+    name = name.replace("void ", "").replace("ac::", "")
+    return name.split("(")[0]
+
+
+def summarize_profile(src, workload, out_dir):
+    """Writes profiles/<workload>_kernels.md and profiles/<workload>_traffic.json from a directory holding
+    rocprofv3 outputs: trace/ (--kernel-trace --stats), fetch/ and write/ (--pmc FETCH_SIZE / WRITE_SIZE of the
+    bench), cfetch/ and cwrite/ (the same counters for --copy-calibration, a plain copy of 3 x 1 GiB)."""
+    import collections
+    import csv
+    import glob
+
+    def rows(sub):
+        f = glob.glob(os.path.join(src, sub, "*_kernel_stats.csv")) or glob.glob(os.path.join(src, sub,
+                                                                                                "*_counter_collection.csv"))
+        return list(csv.DictReader(open(f[0]))) if f else []
+
+    def per_kernel(sub, counter):
+        agg = collections.defaultdict(list)
+        for r in rows(sub):
+            if r.get("Counter_Name") == counter:
+                agg[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]) * 1024.0)  # KiB -> bytes
+        return {k: sum(v) / len(v) for k, v in agg.items()}
+
+    copy_bytes = 3 * (1 << 30) / 3  # per copy launch
+    cf = per_kernel("cfetch", "FETCH_SIZE")
+    cw = per_kernel("cwrite", "WRITE_SIZE")
+    fk = next((k for k in cf if "copyBuffer" in k), None)
+    wk = next((k for k in cw if "copyBuffer" in k), None)
+    f_scale = copy_bytes / cf[fk] if fk else None
+    w_scale = copy_bytes / cw[wk] if wk else None
+    fetch = per_kernel("fetch", "FETCH_SIZE")
+    write = per_kernel("write", "WRITE_SIZE")
+    stats = rows("trace")
+    lines = [f"# Kernel profile: {workload}", "",
+             "Commands (run from the repository root on one MI355X):", "",
+             "    rocprofv3 --kernel-trace --stats --output-format csv -d <dir>/trace -o s22 -- "
+             "python -u bench.py --steps 2 --warmup 1 --no-cpu-baseline",
+             "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/fetch -o s22 -- "
+             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile",
+             "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir>/write -o s22 -- "
+             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile",
+             "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/cfetch -o copy -- python -u bench.py --copy-calibration",
+             "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir>/cwrite -o copy -- python -u bench.py --copy-calibration",
+             f"    python bench.py --summarize-profile <dir>", "",
+             f"Counter calibration on a plain 1 GiB device copy: FETCH_SIZE x {f_scale:.4f}, WRITE_SIZE x {w_scale:.4f} "
+             "(bytes per counted byte)." if f_scale and w_scale else "Counter calibration: missing.", "",
+             "| kernel | calls | avg us | total ms | % | HBM read GB/launch | HBM write GB/launch |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    traffic = {"workload": workload, "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"])):
+        k = _short(r["Name"])
+        fr = fetch.get(k)
+        wr = write.get(k)
+        fb = fr * f_scale if fr is not None and f_scale else None
+        wb = wr * w_scale if wr is not None and w_scale else None
+        lines.append(f"| {k} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | {float(r['TotalDurationNs']) / 1e6:.3f} "
+                     f"| {float(r['Percentage']):.1f} | {fb / 1e9 if fb else float('nan'):.3f} | "
+                     f"{wb / 1e9 if wb else float('nan'):.3f} |")
+        traffic["kernels"][k] = {"avg_us": float(r["AverageNs"]) / 1e3, "calls": int(r["Calls"]),
+                                 "read_bytes": fb, "write_bytes": wb}
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, f"{workload}_kernels.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(out_dir, f"{workload}_traffic.json"), "w") as f:
+        json.dump(traffic, f, indent=1)
+    print("\n".join(lines))
+
+
+def measured_traffic(workload, stage):
+    """HBM bytes (read + write) per launch of a stage's kernel from the committed profile summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    prefixes = [p for p, st in STAGE_OF_KERNEL.items() if st == stage]
+    for k, v in json.load(open(path)).get("kernels", {}).items():
+        if any(k.startswith(p) for p in prefixes) and v.get("read_bytes") is not None and v.get("write_bytes") is not None:
+            return v["read_bytes"] + v["write_bytes"]
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,10 +159,18 @@ def main():
     ap.add_argument("--config", default="s22", choices=["s22", "s18", "small"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--summarize-profile", metavar="DIR",
+                    help="summarise rocprofv3 outputs under DIR into profiles/ and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (scales rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
+    if args.summarize_profile:
+        from application_classification_amd import workloads as W
+
+        name = {"s22": "rmat22-d64-p16x24", "s18": "rmat18-d32-p12x16", "small": "small-er1000"}[args.config]
+        summarize_profile(args.summarize_profile, name, os.path.join(ROOT, "profiles"))
+        return
     if args.copy_calibration:
         import torch
 
@@ -156,7 +251,7 @@ def main():
         avg_ms = stages[dom] / pn
         achieved = B[dom] / (avg_ms * 1e-3)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(w.name, dom),
                     "alg_bytes_per_launch": B[dom], "avg_launch_ms": avg_ms}
         log(f"stages (ms, summed over {pn} iterations): {json.dumps({k: round(v, 4) for k, v in stages.items()})}")
 
