@@ -18,6 +18,7 @@ using ac::Lse;
 using ac::Lse2;
 using ac::QPart;
 using ac::QStat;
+using ac::Bundle;
 using ac::Run;
 using ac::Split;
 
@@ -130,6 +131,7 @@ struct ac_ctx {
   DevBuf<int> csc_eid;
   DevBuf<Run> runs_in, runs_out;
   DevBuf<Split> splits_in, splits_out;
+  std::vector<Run> h_runs_in;
   int64_t nruns_in = 0, nruns_out = 0;
   int nsplit_in = 0, nsplit_out = 0, nslots_in = 0, nslots_out = 0;
 
@@ -143,6 +145,12 @@ struct ac_ctx {
 
   // Work buffers.
   DevBuf<float> CE, CV, FMaxN, VFm, minCEo, lsO, VR, OUT;
+  // Bundle layout of the in-edge runs (depends on the pattern through Qs): rows of CE, their endpoints and edge ids,
+  // the row of every CSC position, and the CSR walk's {row, src} pairs.
+  DevBuf<Bundle> bundles;
+  DevBuf<int> brun, gslot, bsrc, bdst, beid, rowmap;
+  DevBuf<int2> csr_kb;
+  int64_t nbundles = 0, brows = 0;
   DevBuf<double> carry_in, carry_out;
   DevBuf<QPart> qpart;
   DevBuf<Lse> ppart, lpart;
@@ -277,6 +285,8 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   ctx->nsplit_out = (int)sout.size();
   ctx->nslots_in = slin;
   ctx->nslots_out = slout;
+  ctx->h_runs_in = std::move(rin);
+  ctx->work_key[0] = -1;  // the bundle layout follows the graph, not only its sizes
   ctx->has_data = true;
   ctx->have_result = false;
   return AC_OK;
@@ -297,14 +307,14 @@ static int set_pattern(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int
   ctx->pefeat.assign(ef, ef + nE * dE);
   ctx->ps.assign(src, src + nE);
   ctx->pd.assign(dst, dst + nE);
-  // Pattern features in FP64, transposed to [dim][count rounded up to 8] with zero padding (cdist reads eight
-  // patterns per step).
+  // Pattern features in FP64 as [count rounded up to 8 / 8][dim][8] with zero padding: cdist reads the eight
+  // patterns of a group together, one feature at a time.
   const int Pp = round8((int)nV), Qp = round8((int)nE);
   std::vector<double> pv((size_t)dV * Pp, 0.0), pe((size_t)dE * Qp, 0.0);
   for (int64_t p = 0; p < nV; ++p)
-    for (int64_t k = 0; k < dV; ++k) pv[k * Pp + p] = vf[p * dV + k];
+    for (int64_t k = 0; k < dV; ++k) pv[((p / 8) * dV + k) * 8 + p % 8] = vf[p * dV + k];
   for (int64_t q = 0; q < nE; ++q)
-    for (int64_t k = 0; k < dE; ++k) pe[k * Qp + q] = ef[q * dE + k];
+    for (int64_t k = 0; k < dE; ++k) pe[((q / 8) * dE + k) * 8 + q % 8] = ef[q * dE + k];
   // Incidence of each pattern vertex in ascending q: the FMax term (q ends at p) before the RMax term (q starts at p).
   std::vector<int> inc_off(nV + 1, 0), inc;
   for (int64_t p = 0; p < nV; ++p) {
@@ -350,6 +360,54 @@ extern "C" int ac_set_graph(ac_ctx* ctx, int which, int64_t nV, int64_t dV, cons
   return set_pattern(ctx, nV, dV, vf, nE, dE, ef, src, dst);
 }
 
+// Deals the in-edge runs, longest first (ties in CSC order), out to bundles of G = 64 / (Qs / 4) runs, then lays
+// out the rows of every bundle on the device (k_build_bundles) and re-targets the CSR walk at them.
+static int build_bundles(ac_ctx* ctx) {
+  const int G = ac::walk_groups(ctx->Qs), Bt = ac::walk_batch();
+  const std::vector<Run>& R = ctx->h_runs_in;
+  const int64_t nr = (int64_t)R.size();
+  std::vector<int> order(nr);
+  for (int64_t i = 0; i < nr; ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return R[a].ke - R[a].kb > R[b].ke - R[b].kb; });
+  const int64_t nb = (nr + G - 1) / G;
+  std::vector<Bundle> bun(nb);
+  std::vector<int> brun((size_t)std::max<int64_t>(1, nb * G), -1), gslot((size_t)std::max<int64_t>(1, nb * G), -1);
+  int64_t base = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    int len = 0;
+    for (int g = 0; g < G; ++g) {
+      const int64_t i = b * G + g;
+      if (i >= nr) break;
+      const Run& r = R[order[i]];
+      brun[i] = order[i];
+      gslot[i] = r.slot;
+      len = std::max(len, r.ke - r.kb);
+    }
+    len = (len + Bt - 1) / Bt * Bt;
+    bun[b] = Bundle{base, len, 0};
+    base += (int64_t)len * G;
+  }
+  if (base >= (int64_t)INT32_MAX) return ctx->fail(AC_ERR_ARG, "ac_run: the padded edge layout exceeds 2^31 rows");
+  hipStream_t st = ctx->stream;
+  ctx->nbundles = nb;
+  ctx->brows = base;
+  HIPCHK(ctx, ctx->bundles.upload(bun.data(), bun.size(), st));
+  HIPCHK(ctx, ctx->brun.upload(brun.data(), brun.size(), st));
+  HIPCHK(ctx, ctx->gslot.upload(gslot.data(), gslot.size(), st));
+  const size_t rows = (size_t)std::max<int64_t>(1, base), m = (size_t)std::max<int64_t>(1, ctx->de);
+  HIPCHK(ctx, ctx->bsrc.alloc(rows));
+  HIPCHK(ctx, ctx->bdst.alloc(rows));
+  HIPCHK(ctx, ctx->beid.alloc(rows));
+  HIPCHK(ctx, ctx->rowmap.alloc(m));
+  HIPCHK(ctx, ctx->csr_kb.alloc(m));
+  ac::launch_build_bundles(nb, ctx->bundles.p, ctx->brun.p, ctx->runs_in.p, ctx->csc_sd.p, ctx->csc_eid.p, G,
+                           ctx->bsrc.p, ctx->bdst.p, ctx->beid.p, ctx->rowmap.p, st);
+  ac::launch_remap_ks(ctx->de, ctx->csr_ks.p, ctx->rowmap.p, ctx->csr_kb.p, st);
+  HIPCHK(ctx, hipGetLastError());
+  return AC_OK;
+}
+
 // Sizes every work buffer for the current (data, pattern) pair.
 static int ensure_work(ac_ctx* ctx) {
   const int64_t key[4] = {ctx->dn, ctx->de, ctx->P, ctx->Q};
@@ -362,7 +420,9 @@ static int ensure_work(ac_ctx* ctx) {
   ctx->nb_col = (int)std::min<int64_t>((n + 255) / 256, 1024);
   ctx->nb_src = std::max(1, ac::walk_blocks(ctx->nruns_out, Qs));
   const int nb_l = std::max(ctx->nb_col, ctx->nb_src);
-  HIPCHK(ctx, ctx->CE.alloc((size_t)std::max<int64_t>(1, (m + 3) / 4 * 4 * Qs)));
+  int rc = build_bundles(ctx);
+  if (rc != AC_OK) return rc;
+  HIPCHK(ctx, ctx->CE.alloc((size_t)std::max<int64_t>(1, ctx->brows * Qs)));
   HIPCHK(ctx, ctx->CV.alloc((size_t)(n * P)));
   HIPCHK(ctx, ctx->FMaxN.alloc((size_t)(n * Qs)));
   HIPCHK(ctx, ctx->VFm.alloc((size_t)(n * Qs)));
@@ -429,19 +489,19 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
   ctx->pev_used = 0;
   MARK(-1);
   // ---- prep (src/main.cu:107-148)
-  ac::launch_cdist(n, nullptr, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P, 0, st);
+  ac::launch_cdist(n, nullptr, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P, st);
   ac::launch_col_lse_neg(n, P, ctx->CV.p, ctx->lpart.p, ctx->nb_col, st);
   ac::launch_reduce_lse(ctx->nb_col, P, ctx->lpart.p, ctx->lse_cv.p, st);
   HIPCHK(ctx, hipMemcpyAsync(ctx->pstat.p, ctx->lse_cv.p, sizeof(Lse2) * P, hipMemcpyDeviceToDevice, st));
   if (Q > 0) {
-    ac::launch_cdist(m, ctx->csc_eid.p, ctx->efeat.p, (int)ctx->dE, ctx->pf_e.p, Q, round8(Q), Qs, ctx->CE.p, Qs, 1,
-                     st);
+    ac::launch_cdist(ctx->brows, ctx->beid.p, ctx->efeat.p, (int)ctx->dE, ctx->pf_e.p, Q, round8(Q), Qs, ctx->CE.p,
+                     Qs, st);
     ac::launch_fill(ctx->minCEo.p, n * Qs, INFINITY, st);
     ac::launch_fill(ctx->lsO.p, n * Qs, 0.f, st);
     ac::launch_fill(ctx->VFm.p, n * Qs, -INFINITY, st);
     ac::launch_fill(ctx->FMaxN.p, n * Qs, -INFINITY, st);
     if (ctx->nruns_out > 0) {
-      ac::launch_walk_src(ctx->nruns_out, ctx->runs_out.p, ctx->csr_ks.p, ctx->CE.p, Q, Qs, ctx->minCEo.p,
+      ac::launch_walk_src(ctx->nruns_out, ctx->runs_out.p, ctx->csr_kb.p, ctx->CE.p, Q, Qs, ctx->minCEo.p,
                           ctx->lsO.p, ctx->VFm.p, ctx->carry_out.p, ctx->lpart.p, ctx->nb_src, st);
       ac::launch_walk_src_fixup(ctx->nsplit_out, ctx->splits_out.p, ctx->carry_out.p, Q, Qs, ctx->minCEo.p,
                                 ctx->lsO.p, ctx->VFm.p, st);
@@ -452,8 +512,8 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
       HIPCHK(ctx, hipStreamSynchronize(st));
     }
     ac::launch_prep_scalars(P, Q, ctx->d_ps.p, ctx->d_pd.p, ctx->lse_cv.p, ctx->lse_ce.p, qs_prep, st);
-    ac::launch_walk_max(ctx->nruns_in, ctx->runs_in.p, ctx->csc_sd.p, ctx->CE.p, nullptr, ctx->VRs, Q, Qs, qs_prep,
-                        ctx->FMaxN.p, ctx->carry_in.p, st);
+    ac::launch_walk_max(ctx->nbundles, ctx->bundles.p, ctx->gslot.p, ctx->bsrc.p, ctx->bdst.p, ctx->CE.p, nullptr,
+                        ctx->VRs, Q, Qs, qs_prep, ctx->FMaxN.p, ctx->carry_in.p, st);
     ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_prep, ctx->FMaxN.p, st);
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
@@ -491,8 +551,8 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
       MARK(AC_STAGE_VERTEX_A);
       ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_new, st);
       MARK(AC_STAGE_REDUCE_A);
-      ac::launch_walk_max(ctx->nruns_in, ctx->runs_in.p, ctx->csc_sd.p, ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs,
-                          qs_new, ctx->FMaxN.p, ctx->carry_in.p, st);
+      ac::launch_walk_max(ctx->nbundles, ctx->bundles.p, ctx->gslot.p, ctx->bsrc.p, ctx->bdst.p, ctx->CE.p,
+                          ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p, st);
       MARK(AC_STAGE_EDGE_WALK);
       ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_new, ctx->FMaxN.p, st);
       MARK(AC_STAGE_WALK_FIXUP);

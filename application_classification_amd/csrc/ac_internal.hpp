@@ -4,9 +4,11 @@
 //   - per-vertex x pattern-edge arrays are row-major [n][Qs], Qs = Q rounded up to a multiple of 4, so a row is a
 //     whole number of float4s (the edge walk gathers whole rows);
 //   - per-vertex x pattern-vertex arrays are row-major [n][P];
-//   - the edge-cost matrix CE holds m (rounded up to 4) x Qs costs in CSC (destination-sorted) edge order, in
-//     "quad" layout: CE(k, q) = CE[(k / 4) * 4 * Qs + 4 q + k % 4];
-//   - edge ids travel as int2 pairs: CSC order {src, dst}, CSR order {CSC position, src}.
+//   - the edge-cost matrix CE is row-major [rows][Qs] in "bundle" order: the in-edge runs (CSC order) are dealt
+//     out G = 64 / (Qs / 4) to a bundle, and step s of the bundle's run g is row base + s * G + g, so one step of
+//     the wave that walks a bundle reads G adjacent rows.  Runs shorter than their bundle are padded with rows of
+//     +inf cost; bsrc / bdst hold each row's edge endpoints (padding: src 0, dst repeating the run's last one);
+//   - CSR order travels as int2 pairs {row of the edge in CE, src}.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -47,6 +49,12 @@ struct Run {
 struct Split {
   int v, first, count, pad;
 };
+// G runs walked side by side by one wave: rows [base, base + len * G) of CE / bsrc / bdst; len is a multiple of
+// the walk's batch size.
+struct Bundle {
+  int64_t base;
+  int len, pad;
+};
 
 constexpr int kVertexBlock = 256;
 constexpr int kWalkBlock = 256;
@@ -77,7 +85,7 @@ enum VertexMode { kModeA = 0, kModeC = 1, kModeOut = 2 };
 
 // Launchers (ac_kernels.hip).  All asynchronous on `st`.
 void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, const double* pat, int K, int Kp,
-                  int Kstore, float* out, int out_stride, int quad, hipStream_t st);
+                  int Kstore, float* out, int out_stride, hipStream_t st);
 void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks, hipStream_t st);
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st);
 void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st);
@@ -88,12 +96,18 @@ void launch_walk_src(int64_t nruns, const Run* runs, const int2* ks, const float
                      float* lsO, float* VFm, double* carry, Lse* gpart, int nblocks, hipStream_t st);
 void launch_walk_src_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, float* minCEo,
                            float* lsO, float* VFm, hipStream_t st);
-void launch_walk_max(int64_t nruns, const Run* runs, const int2* sd, const float* CE, const float* VR, int VRs, int Q,
-                     int Qs, const QStat* qs, float* FMaxN, double* carry, hipStream_t st);
+void launch_build_bundles(int64_t nbundles, const Bundle* bundles, const int* brun, const Run* runs, const int2* sd,
+                          const int* eid, int G, int* bsrc, int* bdst, int* beid, int* rowmap, hipStream_t st);
+void launch_remap_ks(int64_t m, const int2* ks, const int* rowmap, int2* out, hipStream_t st);
+void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
+                     const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
+                     double* carry, hipStream_t st);
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
                            float* FMaxN, hipStream_t st);
 void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st);
 int walk_blocks(int64_t nruns, int Qs);
+int walk_groups(int Qs);  // runs per bundle: 64 / (Qs / 4)
+int walk_batch();         // steps per batch of the bundle walk (bundle lengths are multiples of it)
 int vertex_tile(int P, int Q);
 
 }  // namespace ac

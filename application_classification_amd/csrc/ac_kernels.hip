@@ -25,19 +25,8 @@ namespace {
 
 constexpr double kInf = __builtin_huge_val();
 
-// Edge costs are row-major [m][Qs] (CSC order).  The edge walk reads them in batches of B consecutive rows starting at
-// multiples of B, where B rows are a whole number of 128-byte lines, so no line is shared by two batches.
+// Edge costs are row-major [rows][Qs].
 __device__ __forceinline__ int64_t ce_index(int64_t k, int q, int Qs) { return k * Qs + q; }
-
-int walk_batch(int Qs) {
-  if (const char* e = std::getenv("AC_WALK_BATCH")) {  // tuning override: 1, 2, 4 or 8
-    const int b = std::atoi(e);
-    if (b == 1 || b == 2 || b == 4 || b == 8) return b;
-  }
-  int b = 1;
-  while ((b * Qs * 4) % 128 != 0 && b < 8) b *= 2;
-  return b;
-}
 
 // Native 4-wide vector (register-resident across loop iterations, unlike the HIP float4 wrapper struct).
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -83,8 +72,10 @@ __device__ __forceinline__ Lse lse_empty() { return Lse{-kInf, 0.0}; }
 
 // ---------------------------------------------------------------------------------------------------------------
 // cdist (src/ac.hxx:33-56): out[r][c] = || feats[row r] - pat_c ||, FP64 accumulation of (a - b)^2 rounded once to
-// FP32.  Pattern features are [dim][Kp] doubles (Kp = K rounded up to 8, zero padded); row r of the output reads
-// feature row idx[r] (idx == nullptr: row r).  Columns K <= c < Kstore of the output are written as 0.
+// FP32.  Pattern features are [Kp / 8][dim][8] doubles (Kp = K rounded up to 8, zero padded: each group of eight
+// patterns is one contiguous block, eight values per feature); row r of the output reads
+// feature row idx[r] (idx == nullptr: row r).  Columns K <= c < Kstore of the output are written as 0, and rows with
+// idx[r] < 0 (padding of the bundle layout) as +inf.
 
 constexpr int kCdTile = 128;
 
@@ -93,8 +84,7 @@ constexpr int kCdTile = 128;
 __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int* __restrict__ idx,
                                                         const float* __restrict__ feats, int dim,
                                                         const double* __restrict__ pat, int K, int Kp, int Kstore,
-                                                        float* __restrict__ out, int out_stride, int tile,
-                                                        int quad) {
+                                                        float* __restrict__ out, int out_stride, int tile) {
   extern __shared__ float rows[];  // [tile][dim + 1], tile <= kCdTile
   const int ld = dim + 1;
   for (int64_t base = blockIdx.x * (int64_t)tile; base < nrows; base += (int64_t)gridDim.x * tile) {
@@ -103,21 +93,20 @@ __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int
     for (int e = threadIdx.x; e < nt * dim; e += kCdTile) {
       const int r = e / dim, c = e - r * dim;
       const int64_t src = idx ? (int64_t)idx[base + r] : base + r;
-      rows[r * ld + c] = feats[src * dim + c];
+      rows[r * ld + c] = src >= 0 ? feats[src * dim + c] : 0.f;
     }
     __syncthreads();
     if ((int)threadIdx.x < nt) {
       const float* x = rows + threadIdx.x * ld;
       const int64_t row = base + threadIdx.x;
-      // Row-major: column c at o[c]; quad layout: column c at o[4 c] (see ce_index).
+      const bool pad = idx && idx[row] < 0;
       float* o = out + row * out_stride;
-      const int cs = 1;
-      (void)quad;
       for (int k0 = 0; k0 < Kstore; k0 += 8) {
         double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
         for (int i = 0; i < dim; ++i) {
           const double xi = (double)x[i];
-          const double* pv = pat + (int64_t)i * Kp + k0;
+          const double* pv = pat + ((int64_t)(k0 / 8) * dim + i) * 8;
 #pragma unroll
           for (int c = 0; c < 8; ++c) {
             const double d = xi - pv[c];
@@ -127,7 +116,7 @@ __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
           const int k = k0 + c;
-          if (k < Kstore) o[k * cs] = k < K ? (float)sqrt(a[c]) : 0.f;
+          if (k < Kstore) o[k] = pad ? INFINITY : k < K ? (float)sqrt(a[c]) : 0.f;
         }
       }
     }
@@ -137,25 +126,25 @@ __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int
 // Fallback for feature widths too large to stage: one thread per output element, rows read from global memory.
 __global__ void k_cdist_direct(int64_t nrows, const int* __restrict__ idx, const float* __restrict__ feats, int dim,
                                const double* __restrict__ pat, int K, int Kp, int Kstore, float* __restrict__ out,
-                               int out_stride, int quad) {
+                               int out_stride) {
   const int64_t total = nrows * Kstore;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = e / Kstore;
     const int k = (int)(e - r * Kstore);
-    const float* x = feats + (idx ? (int64_t)idx[r] : r) * dim;
+    const int64_t src = idx ? (int64_t)idx[r] : r;
+    const float* x = feats + src * dim;
     double a = 0.0;
-    if (k < K)
+    if (k < K && src >= 0)
       for (int i = 0; i < dim; ++i) {
-        const double d = (double)x[i] - pat[(int64_t)i * Kp + k];
+        const double d = (double)x[i] - pat[((int64_t)(k / 8) * dim + i) * 8 + k % 8];
         a = fma(d, d, a);
       }
-    const int64_t o = quad ? ce_index(r, k, out_stride) : r * out_stride + k;
-    out[o] = k < K ? (float)sqrt(a) : 0.f;
+    out[r * out_stride + k] = src < 0 ? INFINITY : k < K ? (float)sqrt(a) : 0.f;
   }
 }
 
 void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, const double* pat, int K, int Kp,
-                  int Kstore, float* out, int out_stride, int quad, hipStream_t st) {
+                  int Kstore, float* out, int out_stride, hipStream_t st) {
   if (nrows == 0 || Kstore == 0) return;
   if (dim <= 255) {
     // At most 64 KB of LDS: 128 rows up to 127 features, fewer rows beyond.
@@ -163,11 +152,11 @@ void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, co
     const size_t lds = (size_t)tile * (dim + 1) * sizeof(float);
     const int blocks = (int)std::min<int64_t>((nrows + tile - 1) / tile, 1 << 16);
     hipLaunchKernelGGL(k_cdist_rows, dim3(blocks), dim3(kCdTile), lds, st, nrows, idx, feats, dim, pat, K, Kp,
-                       Kstore, out, out_stride, tile, quad);
+                       Kstore, out, out_stride, tile);
   } else {
     const int blocks = (int)std::min<int64_t>((nrows * Kstore + 255) / 256, 1 << 16);
     hipLaunchKernelGGL(k_cdist_direct, dim3(blocks), dim3(256), 0, st, nrows, idx, feats, dim, pat, K, Kp, Kstore,
-                       out, out_stride, quad);
+                       out, out_stride);
   }
 }
 
@@ -413,24 +402,43 @@ void launch_walk_src_fixup(int nsplit, const Split* splits, const double* carry,
                      Q, Qs, minCEo, lsO, VFm);
 }
 
+// ---------------------------------------------------------------------------------------------------------------
 // Destination maxima (CSC walk), src/main.cu:220-224 with the log-softmax of src/main.cu:218 folded in:
 //   FMaxN[v][q] = max over in-edges e of v of (VR[src e][q] - CE[e][q])  -  log sum_e exp(FE[q][e])
 // computed exactly in FP64 (a difference of two floats is exact in double) and rounded once.  With VR == nullptr
 // it is the prep's max over in-edges of -CE (src/main.cu:143-146).
+//
+// One wave walks one bundle of G runs, group g (Qs/4 lanes) walking run g, lane j owning pattern edges 4j..4j+3.
+// Step s of the bundle reads rows base + s*G + g of CE (one contiguous block of G*Qs floats for the wave), of bsrc
+// and of bdst, and gathers row bsrc of VR.  The steps go in batches of B, software-pipelined two batches deep with
+// two register sets used in turn (no register copies, so no wait for loads in flight at a loop back-edge): while
+// batch t is reduced, the costs, destinations and gathered rows of batch t + 1 and the sources of batches t + 2 and
+// t + 3 are in flight.  Loads past the bundle's last batch are clamped to it and never used.
+
+int walk_groups(int Qs) { return 64 / (Qs / 4); }
+
+int walk_batch() {
+  if (const char* e = std::getenv("AC_WALK_BATCH")) {  // tuning override: 1, 2 or 4
+    const int b = std::atoi(e);
+    if (b == 1 || b == 2 || b == 4) return b;
+  }
+  return 2;
+}
+
 template <bool kWithVR, int B>
-__global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nruns, const Run* __restrict__ runs,
-                                                         const int2* __restrict__ sd, const float* __restrict__ CE,
+__global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const Bundle* __restrict__ bundles,
+                                                         const int* __restrict__ gslot, const int* __restrict__ bsrc,
+                                                         const int* __restrict__ bdst, const float* __restrict__ CE,
                                                          const float* __restrict__ VR, int VRs, int Q, int Qs,
                                                          const QStat* __restrict__ qs, float* __restrict__ FMaxN,
                                                          double* __restrict__ carry) {
   const int L4 = Qs / 4, G = 64 / L4, VR4s = VRs / 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane / L4, j = lane - g * L4;
-  const int64_t r = ((int64_t)blockIdx.x * (kWalkBlock / 64) + wave) * G + g;
-  if (g >= G || r >= nruns) return;
-  const Run R = runs[r];
-  const float4* CE4 = reinterpret_cast<const float4*>(CE);
-  const float4* VR4 = reinterpret_cast<const float4*>(VR);
+  const int64_t b = (int64_t)blockIdx.x * (kWalkBlock / 64) + wave;
+  if (g >= G || b >= nbundles) return;
+  const Bundle Bd = bundles[b];
+  const int slot = gslot[b * G + g];
   // log sum_e exp(FE[q][e]) as one FP64 value (its FP64 rounding is far below the FP32 rounding of the result).
   double sl[4];
 #pragma unroll
@@ -439,7 +447,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nruns, const Ru
     sl[c] = q < Q ? qs[q].mFE + qs[q].lFE : 0.0;
   }
   double mx0 = -kInf, mx1 = -kInf, mx2 = -kInf, mx3 = -kInf;
-  int cur = sd[R.kb].y;
+  int cur = -1;
   auto emit = [&](int v) {
     float4 o;
     o.x = (float)(mx0 - sl[0]);
@@ -449,105 +457,153 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nruns, const Ru
     reinterpret_cast<float4*>(FMaxN)[(int64_t)v * L4 + j] = o;
     mx0 = mx1 = mx2 = mx3 = -kInf;
   };
-  // Batches of B consecutive edges (batches start at multiples of B, so their cost rows are whole cache lines),
-  // software-pipelined so that every wait overlaps other loads: while batch bt is reduced, the costs and gathered
-  // rows of batch bt + 1 and the ids of batch bt + 2 are already in flight.  Edges outside the run are skipped.
-  typedef int i2v __attribute__((ext_vector_type(2)));
-  const f4v* CEv = reinterpret_cast<const f4v*>(CE);
+  const int GL4 = G * L4;
+  const int* srcp = bsrc + Bd.base + g;
+  const int* dstp = bdst + Bd.base + g;
+  const f4v* cep = reinterpret_cast<const f4v*>(CE) + (Bd.base + g) * L4 + j;
   const f4v* VRv = reinterpret_cast<const f4v*>(VR);
-  const i2v* sdv = reinterpret_cast<const i2v*>(sd);
-  const int64_t bfirst = R.kb / B, blast = (int64_t)(R.ke - 1) / B;
-#define AC_WALK_IDS(E, BT)                                                                 \
-  _Pragma("unroll") for (int u = 0; u < B; ++u) {                                          \
-    const int64_t k_ = (BT) * B + u;                                                       \
-    E[u] = (k_ >= R.kb && k_ < R.ke) ? __builtin_nontemporal_load(sdv + k_) : i2v{0, -1};  \
+  const int last = Bd.len / B - 1;
+#define AC_WK_SRC(S, T)                                                          \
+  if (kWithVR) {                                                                 \
+    const int t_ = (T) < last ? (T) : last;                                      \
+    _Pragma("unroll") for (int u = 0; u < B; ++u) S[u] =                         \
+        __builtin_nontemporal_load(srcp + (t_ * B + u) * G);                     \
   }
-#define AC_WALK_COSTS(C, BT)                                                                         \
-  _Pragma("unroll") for (int u = 0; u < B; ++u) {                                                    \
-    const int64_t k_ = (BT) * B + u;                                                                 \
-    C[u] = (k_ >= R.kb && k_ < R.ke) ? __builtin_nontemporal_load(CEv + k_ * L4 + j) : f4v{0, 0, 0, 0}; \
+#define AC_WK_BATCH(C, D, W, S, T)                                                                       \
+  {                                                                                                      \
+    const int t_ = (T) < last ? (T) : last;                                                              \
+    _Pragma("unroll") for (int u = 0; u < B; ++u) C[u] = __builtin_nontemporal_load(cep + (t_ * B + u) * GL4); \
+    _Pragma("unroll") for (int u = 0; u < B; ++u) D[u] = __builtin_nontemporal_load(dstp + (t_ * B + u) * G);  \
+    _Pragma("unroll") for (int u = 0; u < B; ++u) W[u] =                                                 \
+        kWithVR ? VRv[(int64_t)S[u] * VR4s + j] : f4v{0.f, 0.f, 0.f, 0.f};                               \
   }
-#define AC_WALK_GATHER(W, E)                                                                        \
-  _Pragma("unroll") for (int u = 0; u < B; ++u) {                                                   \
-    W[u] = (kWithVR && E[u].y >= 0) ? VRv[(int64_t)E[u].x * VR4s + j] : f4v{0, 0, 0, 0};            \
+#define AC_WK_REDUCE(C, D, W)                                 \
+  _Pragma("unroll") for (int u = 0; u < B; ++u) {             \
+    if (D[u] != cur) {                                        \
+      if (cur >= 0) emit(cur);                                \
+      cur = D[u];                                             \
+    }                                                         \
+    mx0 = fmax(mx0, (double)W[u].x - (double)C[u].x);         \
+    mx1 = fmax(mx1, (double)W[u].y - (double)C[u].y);         \
+    mx2 = fmax(mx2, (double)W[u].z - (double)C[u].z);         \
+    mx3 = fmax(mx3, (double)W[u].w - (double)C[u].w);         \
   }
-  i2v e0[B], e1[B], e2[B];
-  f4v c0[B], c1[B], w0[B], w1[B];
-  AC_WALK_IDS(e0, bfirst);
-  AC_WALK_IDS(e1, bfirst + 1);
-  AC_WALK_COSTS(c0, bfirst);
-  AC_WALK_GATHER(w0, e0);
-  for (int64_t bt = bfirst; bt <= blast; ++bt) {
-    AC_WALK_COSTS(c1, bt + 1);
-    AC_WALK_GATHER(w1, e1);
-    AC_WALK_IDS(e2, bt + 2);
+  int SA[B], SB[B], DA[B], DB[B];
+  f4v CA[B], CB[B], WA[B], WB[B];
 #pragma unroll
-    for (int u = 0; u < B; ++u) {
-      if (e0[u].y < 0) continue;
-      if (e0[u].y != cur) {
-        emit(cur);
-        cur = e0[u].y;
-      }
-      mx0 = fmax(mx0, (double)w0[u].x - (double)c0[u].x);
-      mx1 = fmax(mx1, (double)w0[u].y - (double)c0[u].y);
-      mx2 = fmax(mx2, (double)w0[u].z - (double)c0[u].z);
-      mx3 = fmax(mx3, (double)w0[u].w - (double)c0[u].w);
-    }
-#pragma unroll
-    for (int u = 0; u < B; ++u) {
-      e0[u] = e1[u];
-      e1[u] = e2[u];
-      c0[u] = c1[u];
-      w0[u] = w1[u];
-    }
+  for (int u = 0; u < B; ++u) SA[u] = SB[u] = 0;
+  AC_WK_SRC(SA, 0);
+  AC_WK_SRC(SB, 1);
+  AC_WK_BATCH(CA, DA, WA, SA, 0);
+  AC_WK_SRC(SA, 2);
+  for (int t = 0;; t += 2) {
+    AC_WK_BATCH(CB, DB, WB, SB, t + 1);
+    AC_WK_SRC(SB, t + 3);
+    AC_WK_REDUCE(CA, DA, WA);
+    if (t + 1 > last) break;
+    AC_WK_BATCH(CA, DA, WA, SA, t + 2);
+    AC_WK_SRC(SA, t + 4);
+    AC_WK_REDUCE(CB, DB, WB);
+    if (t + 2 > last) break;
   }
-#undef AC_WALK_IDS
-#undef AC_WALK_COSTS
-#undef AC_WALK_GATHER
-  if (R.slot >= 0) {
-    double* o = carry + (int64_t)R.slot * Qs + 4 * j;
+#undef AC_WK_SRC
+#undef AC_WK_BATCH
+#undef AC_WK_REDUCE
+  if (slot >= 0) {
+    double* o = carry + (int64_t)slot * Qs + 4 * j;
     o[0] = mx0;
     o[1] = mx1;
     o[2] = mx2;
     o[3] = mx3;
-  } else {
+  } else if (cur >= 0) {
     emit(cur);
   }
 }
 
 template <bool kWithVR>
-static void launch_walk_max_t(int B, int nb, hipStream_t st, int64_t nruns, const Run* runs, const int2* sd,
-                              const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
-                              double* carry) {
+static void launch_walk_max_t(int B, int nb, hipStream_t st, int64_t nbundles, const Bundle* bundles,
+                              const int* gslot, const int* bsrc, const int* bdst, const float* CE, const float* VR,
+                              int VRs, int Q, int Qs, const QStat* qs, float* FMaxN, double* carry) {
   switch (B) {
     case 1:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 1>), dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, VRs, Q,
-                         Qs, qs, FMaxN, carry);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 2>), dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, VRs, Q,
-                         Qs, qs, FMaxN, carry);
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 1>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot, bsrc,
+                         bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
       break;
     case 4:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 4>), dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, VRs, Q,
-                         Qs, qs, FMaxN, carry);
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 4>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot, bsrc,
+                         bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
       break;
     default:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 8>), dim3(nb), dim3(kWalkBlock), 0, st, nruns, runs, sd, CE, VR, VRs, Q,
-                         Qs, qs, FMaxN, carry);
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 2>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot, bsrc,
+                         bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
       break;
   }
 }
 
-void launch_walk_max(int64_t nruns, const Run* runs, const int2* sd, const float* CE, const float* VR, int VRs, int Q,
-                     int Qs, const QStat* qs, float* FMaxN, double* carry, hipStream_t st) {
-  if (nruns == 0) return;
-  const int nb = walk_blocks(nruns, Qs);
-  const int B = walk_batch(Qs);
+void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
+                     const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
+                     double* carry, hipStream_t st) {
+  if (nbundles == 0) return;
+  const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
+  const int B = walk_batch();
   if (VR)
-    launch_walk_max_t<true>(B, nb, st, nruns, runs, sd, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+    launch_walk_max_t<true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
   else
-    launch_walk_max_t<false>(B, nb, st, nruns, runs, sd, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+    launch_walk_max_t<false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+}
+
+// Bundle layout from the CSC arrays: one block per bundle writes its rows (step-major, run-minor).  rowmap[k] is
+// the row of CSC position k.  brun[b * G + g] is the run walked by group g of bundle b (-1: none).
+__global__ __launch_bounds__(256) void k_build_bundles(const Bundle* __restrict__ bundles, const int* __restrict__ brun,
+                                                       const Run* __restrict__ runs, const int2* __restrict__ sd,
+                                                       const int* __restrict__ eid, int G, int* __restrict__ bsrc,
+                                                       int* __restrict__ bdst, int* __restrict__ beid,
+                                                       int* __restrict__ rowmap) {
+  const int64_t b = blockIdx.x;
+  const Bundle Bd = bundles[b];
+  const int nrow = Bd.len * G;
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+    const int s = i / G, g = i - s * G;
+    const int r = brun[b * G + g];
+    const int64_t row = Bd.base + i;
+    int src = 0, dst = -1, e = -1;
+    if (r >= 0) {
+      const Run R = runs[r];
+      if (R.kb + s < R.ke) {
+        const int k = R.kb + s;
+        src = sd[k].x;
+        dst = sd[k].y;
+        e = eid[k];
+        rowmap[k] = (int)row;
+      } else {
+        dst = sd[R.ke - 1].y;
+      }
+    }
+    bsrc[row] = src;
+    bdst[row] = dst;
+    beid[row] = e;
+  }
+}
+
+void launch_build_bundles(int64_t nbundles, const Bundle* bundles, const int* brun, const Run* runs, const int2* sd,
+                          const int* eid, int G, int* bsrc, int* bdst, int* beid, int* rowmap, hipStream_t st) {
+  if (nbundles == 0) return;
+  hipLaunchKernelGGL(k_build_bundles, dim3((unsigned)nbundles), dim3(256), 0, st, bundles, brun, runs, sd, eid, G,
+                     bsrc, bdst, beid, rowmap);
+}
+
+__global__ void k_remap_ks(int64_t m, const int2* __restrict__ ks, const int* __restrict__ rowmap,
+                           int2* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    const int2 e = ks[i];
+    out[i] = make_int2(rowmap[e.x], e.y);
+  }
+}
+
+void launch_remap_ks(int64_t m, const int2* ks, const int* rowmap, int2* out, hipStream_t st) {
+  if (m == 0) return;
+  const int blocks = (int)std::min<int64_t>((m + 255) / 256, 1 << 16);
+  hipLaunchKernelGGL(k_remap_ks, dim3(blocks), dim3(256), 0, st, m, ks, rowmap, out);
 }
 
 // One wave per split vertex, as k_walk_src_fixup.
@@ -596,8 +652,10 @@ void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry,
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Vertex passes.  A block takes tiles of VT (<= 64) consecutive vertices; the tile's FMaxN, VFm and CV rows
-// (contiguous in HBM) are staged in LDS as FP32 with coalesced float4 loads, into rows padded to an odd stride.
+// Vertex passes.  A block takes tiles of VT (<= 64) consecutive vertices.  The tile's rows of FMaxN and VFm (and,
+// in mode A, of minCEo and lsO), contiguous in HBM, and of CV are staged in LDS as FP32, into rows padded to an odd
+// stride; the next tile's rows are loaded into registers (coalesced float4 loads) while the current tile is
+// computed, so the loads of one tile overlap the arithmetic of the previous one.
 //   p-phase : wave w handles pattern vertices p = w, w + 4, ...; lane r is tile row r, so the pattern's incidence
 //             structure is wave-uniform (no divergence).  MU_pre = -CV + sum over the incidences of p in ascending q
 //             (FMax term before RMax term; src/ac.hxx:164-170 made deterministic), FMax = max(fillF, FMaxN),
@@ -607,12 +665,15 @@ void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry,
 //             with coalesced writes, and the per-q maxima / edge log-sum-exps accumulated in registers.
 
 constexpr int kVTMax = 64;
+constexpr int kPF4 = 2;  // float4s per thread of each staged [VT][Qs] array: VT * Qs <= 4 * kPF4 * kVertexBlock
+constexpr int kPFC = 8;  // floats per thread of the staged CV tile:       VT * P  <= kPFC * kVertexBlock
 
 static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
   size_t b = (size_t)(4 * Q + 2 * P) * sizeof(double);                       // tables
-  if (mode == kModeA) b += (size_t)P * VT * sizeof(double);                  // MU
+  if (mode == kModeA) b += (size_t)(P + 1) * VT * sizeof(double);            // MU
   if (mode == kModeC) b += (size_t)P * sizeof(Lse);                          // column accumulators
-  b += (size_t)VT * (2 * (Qs + 1) + (P + 1)) * sizeof(float);                // staged rows
+  const int nst = mode == kModeA ? 4 : 2;                                    // staged [VT][Qs] arrays
+  b += (size_t)VT * (nst * (Qs + 1) + (P + 1)) * sizeof(float);              // staged rows
   b += (size_t)(2 * Q + P + 1) * sizeof(int);                                // incidence lists
   return (b + 15) / 16 * 16;
 }
@@ -620,7 +681,9 @@ static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
 int vertex_tile(int P, int Q) {
   const int Qs = std::max(4, (Q + 3) / 4 * 4);
   int vt = kVTMax;
-  while (vt > 8 && vertex_lds_bytes(kModeA, vt, P, Q, Qs) > 65536) vt /= 2;
+  while (vt > 4 && (vertex_lds_bytes(kModeA, vt, P, Q, Qs) > 65536 || vt * Qs > 4 * kPF4 * kVertexBlock ||
+                    vt * P > kPFC * kVertexBlock))
+    vt /= 2;
   return vt;
 }
 
@@ -638,18 +701,34 @@ __device__ __forceinline__ Lse wave_lse(double x) {
   return Lse{m, e};
 }
 
-template <int kMode>
+// Butterfly merge of one log-sum-exp accumulator per lane over the wave (fixed order; lane 0's result is used).
+__device__ __forceinline__ Lse wave_lse_merge(Lse s) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const Lse w{__shfl_xor(s.m, o, 64), __shfl_xor(s.t, o, 64)};
+    s = lse_merge(s, w);
+  }
+  return s;
+}
+
+// kPW > 0 (mode C with P <= 4 * kPW): each lane accumulates its rows' log-sum-exp of its wave's kPW pattern vertices
+// in registers over all tiles, merged across the wave once at the end.  kPW == 0: one wave-level reduction per tile.
+template <int kMode, int kPW>
 __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
+  constexpr bool kA = kMode == kModeA;
   extern __shared__ double lds[];
   const int Q = a.Q, P = a.P, Qs = a.Qs, VT = a.VT;
   const int ldq = Qs + 1, ldp = P + 1;
   double* tab = lds;                                   // [4][Q]: fillF, fillR, mRE, lRE
   double* pmu = tab + 4 * Q;                           // [2][P]: m, l of LSE_MU
-  double* MUs = pmu + 2 * P;                           // [P][VT] (mode A)
-  Lse* lacc = reinterpret_cast<Lse*>(MUs + (kMode == kModeA ? P * VT : 0));  // [P] (mode C)
-  float* Fs = reinterpret_cast<float*>(lacc + (kMode == kModeC ? P : 0));    // [VT][ldq]
-  float* Vs = Fs + VT * ldq;                           // [VT][ldq]
-  float* Cs = Vs + VT * ldq;                           // [VT][ldp]
+  double* MUs = pmu + 2 * P;                           // [VT][P + 1] (mode A)
+  const int ldm = P + 1;
+  Lse* lacc = reinterpret_cast<Lse*>(MUs + (kA ? ldm * VT : 0));              // [P] (mode C)
+  float* Fs = reinterpret_cast<float*>(lacc + (kMode == kModeC ? P : 0));    // [VT][ldq] FMaxN
+  float* Vs = Fs + VT * ldq;                           // [VT][ldq] VFm
+  float* Ms = Vs + VT * ldq;                           // [VT][ldq] minCEo (mode A)
+  float* Ls = Ms + (kA ? VT * ldq : 0);                // [VT][ldq] lsO (mode A)
+  float* Cs = Ls + (kA ? VT * ldq : 0);                // [VT][ldp] CV
   int* incs = reinterpret_cast<int*>(Cs + VT * ldp);   // [2Q]
   int* incoff = incs + 2 * Q;                          // [P + 1]
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -679,75 +758,135 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
   double vfmax = -kInf, vrmax = -kInf;
   Lse lre = lse_empty(), lfe = lse_empty();
 
-  const float4* F4 = reinterpret_cast<const float4*>(a.FMaxN);
-  const float4* V4 = reinterpret_cast<const float4*>(a.VFm);
+  const f4v* F4 = reinterpret_cast<const f4v*>(a.FMaxN);
+  const f4v* V4 = reinterpret_cast<const f4v*>(a.VFm);
+  const f4v* M4 = reinterpret_cast<const f4v*>(a.minCEo);
+  const f4v* L4v = reinterpret_cast<const f4v*>(a.lsO);
   const int64_t ntiles = (a.n + VT - 1) / VT;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  Lse pacc[kPW > 0 ? kPW : 1];
+#pragma unroll
+  for (int pi = 0; pi < (kPW > 0 ? kPW : 1); ++pi) pacc[pi] = lse_empty();
+  f4v pf[4][kPF4];
+  float pc[kPFC];
+  // Loads of a tile into registers; indexes past the tile are clamped to its last element (loaded, never used).
+#define AC_VX_PREFETCH(TILE)                                                         \
+  {                                                                                  \
+    const int64_t v0_ = (TILE) * VT;                                                 \
+    const int nv_ = (int)((a.n - v0_) < VT ? (a.n - v0_) : VT);                      \
+    const int last4_ = nv_ * Qs / 4 - 1, lastc_ = nv_ * P - 1;                       \
+    const int64_t b4_ = v0_ * Qs / 4, bc_ = v0_ * P;                                 \
+    _Pragma("unroll") for (int k = 0; k < kPF4; ++k) {                               \
+      const int i_ = min(t + k * kVertexBlock, last4_);                              \
+      pf[0][k] = F4[b4_ + i_];                                                       \
+      pf[1][k] = V4[b4_ + i_];                                                       \
+      if (kA) {                                                                      \
+        pf[2][k] = M4[b4_ + i_];                                                     \
+        pf[3][k] = L4v[b4_ + i_];                                                    \
+      }                                                                              \
+    }                                                                                \
+    _Pragma("unroll") for (int k = 0; k < kPFC; ++k) pc[k] = a.CV[bc_ + min(t + k * kVertexBlock, lastc_)]; \
+  }
+  int64_t tile = blockIdx.x;
+  if (tile < ntiles) AC_VX_PREFETCH(tile);
+  for (; tile < ntiles; tile += gridDim.x) {
     const int64_t v0 = tile * VT;
     const int nv = (int)((a.n - v0) < VT ? (a.n - v0) : VT);
     __syncthreads();  // tables written (first tile) / previous tile consumed
     {
-      const int cnt4 = nv * Qs / 4;
-      const int64_t b4 = v0 * Qs / 4;
-      for (int i = t; i < cnt4; i += kVertexBlock) {
-        const float4 f = F4[b4 + i], v = V4[b4 + i];
-        const int e = 4 * i, r = e / Qs, c = e - r * Qs;
-        float* fr = Fs + r * ldq + c;
-        float* vr = Vs + r * ldq + c;
-        fr[0] = f.x, fr[1] = f.y, fr[2] = f.z, fr[3] = f.w;
-        vr[0] = v.x, vr[1] = v.y, vr[2] = v.z, vr[3] = v.w;
+      const int cnt4 = nv * Qs / 4, cntc = nv * P;
+#pragma unroll
+      for (int k = 0; k < kPF4; ++k) {
+        const int i = t + k * kVertexBlock;
+        if (i < cnt4) {
+          const int e = 4 * i, r = e / Qs, c = e - r * Qs;
+          const int o = r * ldq + c;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            Fs[o + u] = pf[0][k][u];
+            Vs[o + u] = pf[1][k][u];
+            if (kA) {
+              Ms[o + u] = pf[2][k][u];
+              Ls[o + u] = pf[3][k][u];
+            }
+          }
+        }
       }
-      for (int i = t; i < nv * P; i += kVertexBlock) {
-        const int r = i / P, c = i - r * P;
-        Cs[r * ldp + c] = a.CV[v0 * P + i];
+#pragma unroll
+      for (int k = 0; k < kPFC; ++k) {
+        const int i = t + k * kVertexBlock;
+        if (i < cntc) {
+          const int r = i / P, c = i - r * P;
+          Cs[r * ldp + c] = pc[k];
+        }
       }
     }
     __syncthreads();
+    if (tile + gridDim.x < ntiles) AC_VX_PREFETCH(tile + gridDim.x);
     // p-phase: wave-uniform pattern vertex, lane = tile row.
-    for (int p = wave; p < P; p += kVertexBlock / 64) {
-      const int r = lane;
-      double x = -kInf;
-      if (r < nv) {
-        double acc = -(double)Cs[r * ldp + p];
-        if (sums)
-          for (int k = incoff[p]; k < incoff[p + 1]; ++k) {
-            const int code = incs[k];
-            const int qq = code >> 1;
-            if (code & 1) {
-              const float vfm = Vs[r * ldq + qq];
-              const double rn = vfm == -INFINITY ? -kInf : ((double)vfm - tab[2 * Q + qq]) - tab[3 * Q + qq];
-              acc += fmax(tab[Q + qq], rn);
-            } else {
-              acc += fmax(tab[qq], (double)Fs[r * ldq + qq]);
-            }
-          }
-        x = acc;
+    // MU_pre of row `r` and pattern vertex `p` (-inf for rows past the tile).
+#define AC_VX_MU_PRE(X, P_, R_)                                                                              \
+    {                                                                                                      \
+      X = -kInf;                                                                                           \
+      if ((R_) < nv) {                                                                                     \
+        double acc_ = -(double)Cs[(R_) * ldp + (P_)];                                                      \
+        if (sums)                                                                                          \
+          for (int k_ = incoff[P_]; k_ < incoff[(P_) + 1]; ++k_) {                                        \
+            const int code_ = incs[k_];                                                                    \
+            const int qq_ = code_ >> 1;                                                                    \
+            if (code_ & 1) {                                                                               \
+              const float vfm_ = Vs[(R_) * ldq + qq_];                                                     \
+              const double rn_ =                                                                           \
+                  vfm_ == -INFINITY ? -kInf : ((double)vfm_ - tab[2 * Q + qq_]) - tab[3 * Q + qq_];        \
+              acc_ += fmax(tab[Q + qq_], rn_);                                                             \
+            } else {                                                                                       \
+              acc_ += fmax(tab[qq_], (double)Fs[(R_) * ldq + qq_]);                                        \
+            }                                                                                              \
+          }                                                                                                \
+        X = acc_;                                                                                          \
+      }                                                                                                    \
+    }
+    if (kMode == kModeC && kPW > 0) {
+#pragma unroll
+      for (int pi = 0; pi < kPW; ++pi) {
+        const int p = wave + 4 * pi;
+        if (p < P) {
+          double x;
+          AC_VX_MU_PRE(x, p, lane);
+          lse_add(pacc[pi], x);
+        }
       }
-      if (kMode == kModeC) {
-        const Lse w = wave_lse(x);
-        if (lane == 0) lacc[p] = lse_merge(lacc[p], w);
-      } else if (r < nv) {
-        const double mu = (x - pmu[p]) - pmu[P + p];
-        if (kMode == kModeA)
-          MUs[p * VT + r] = mu;
-        else
-          a.out[(v0 + r) * P + p] = (float)mu;
+    } else {
+      for (int p = wave; p < P; p += kVertexBlock / 64) {
+        const int r = lane;
+        double x;
+        AC_VX_MU_PRE(x, p, r);
+        if (kMode == kModeC) {
+          const Lse w = wave_lse(x);
+          if (lane == 0) lacc[p] = lse_merge(lacc[p], w);
+        } else if (r < nv) {
+          const double mu = (x - pmu[p]) - pmu[P + p];
+          if (kA)
+            MUs[r * ldm + p] = mu;
+          else
+            a.out[(v0 + r) * P + p] = (float)mu;
+        }
       }
     }
-    if (kMode == kModeA) {
+#undef AC_VX_MU_PRE
+    if (kA) {
       __syncthreads();
       if (qthread) {
         const double fF = tab[q], fR = tab[Q + q], mR = tab[2 * Q + q], lR = tab[3 * Q + q];
         const int q_ps = a.ps[q], q_pd = a.pd[q];
         for (int r = rq; r < nv; r += RQ) {
           const int64_t i = (v0 + r) * Qs + q;
-          const float mo = a.minCEo[i];
-          const float lo = a.lsO[i];
+          const float mo = Ms[r * ldq + q];
+          const float lo = Ls[r * ldq + q];
           const double fm = fmax(fF, (double)Fs[r * ldq + q]);
           const float vfm = Vs[r * ldq + q];
           const double rm = fmax(fR, vfm == -INFINITY ? -kInf : ((double)vfm - mR) - lR);
-          const double vf = MUs[q_pd * VT + r] - fm;
-          const double vr = MUs[q_ps * VT + r] - rm;
+          const double vf = MUs[r * ldm + q_pd] - fm;
+          const double vr = MUs[r * ldm + q_ps] - rm;
           a.VR[(v0 + r) * a.VRs + q] = (float)vr;
           vfmax = fmax(vfmax, vf);
           vrmax = fmax(vrmax, vr);
@@ -760,6 +899,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
       }
     }
   }
+#undef AC_VX_PREFETCH
   __syncthreads();
 
   // Block partials, merged in a fixed order.
@@ -779,7 +919,16 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
       a.qpart[(int64_t)blockIdx.x * Q + t] = acc;
     }
   } else if (kMode == kModeC) {
-    for (int i = t; i < P; i += kVertexBlock) a.ppart[(int64_t)blockIdx.x * P + i] = lacc[i];
+    if (kPW > 0) {
+#pragma unroll
+      for (int pi = 0; pi < kPW; ++pi) {
+        const int p = wave + 4 * pi;
+        const Lse w = wave_lse_merge(pacc[pi]);
+        if (p < P && lane == 0) a.ppart[(int64_t)blockIdx.x * P + p] = w;
+      }
+    } else {
+      for (int i = t; i < P; i += kVertexBlock) a.ppart[(int64_t)blockIdx.x * P + i] = lacc[i];
+    }
   }
 }
 
@@ -788,11 +937,15 @@ void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st) {
   const size_t red = mode == kModeA ? sizeof(QPart) * kVertexBlock : 0;
   const size_t lds = std::max(tiles, red);
   if (mode == kModeA)
-    hipLaunchKernelGGL(k_vertex<kModeA>, dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+    hipLaunchKernelGGL((k_vertex<kModeA, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+  else if (mode == kModeC && a.P <= 16)
+    hipLaunchKernelGGL((k_vertex<kModeC, 4>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+  else if (mode == kModeC && a.P <= 32)
+    hipLaunchKernelGGL((k_vertex<kModeC, 8>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
   else if (mode == kModeC)
-    hipLaunchKernelGGL(k_vertex<kModeC>, dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+    hipLaunchKernelGGL((k_vertex<kModeC, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
   else
-    hipLaunchKernelGGL(k_vertex<kModeOut>, dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+    hipLaunchKernelGGL((k_vertex<kModeOut, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
 }
 
 }  // namespace ac

@@ -148,13 +148,15 @@ def test_single_data_vertex():
     np.testing.assert_array_equal(mu, 0.0)
 
 
-def test_large_pattern(small):
-    """A pattern wider than one float4 row chunk per lane group and than the default vertex tile."""
+@pytest.mark.parametrize("pv,pe", [(24, 36), (40, 70), (64, 120)])
+def test_large_pattern(small, pv, pe):
+    """Patterns past 16 and 32 vertices (the column log-sum-exp switches from register to per-tile reduction) and
+    wider than the default vertex tile."""
     d = small.data
-    p = W.planted_pattern(d, 40, 70, seed=3)
+    p = W.planted_pattern(d, pv, pe, seed=3)
     mu = gpu_mu(d, p, 4)
     ref = O.run(g64(d), g64(p), n_iter=4)
-    assert_parity(mu, ref, "40-vertex / 70-edge pattern")
+    assert_parity(mu, ref, f"{pv}-vertex / {pe}-edge pattern")
 
 
 def test_errors_are_reported():
