@@ -20,6 +20,9 @@ AC_ERR_IO = 5
 AC_ERR_NODEV = 6
 AC_DATA = 0
 AC_PATTERN = 1
+AC_OPT_WALK_BATCH = 1
+AC_OPT_GRAPH = 2
+AC_OPT_SLAB_EDGES = 3
 STAGES = ["prep", "vertex_a", "reduce_a", "edge_walk", "walk_fixup", "vertex_c", "reduce_c", "output"]
 
 ERROR_NAMES = {
@@ -84,6 +87,14 @@ SIGNATURES = {
     "ac_get_timing": (I, [P, ctypes.POINTER(D), ctypes.POINTER(D)]),
     "ac_stream": (P, [P]),
     "ac_run_profiled": (I, [P, I, P]),
+    "ac_set_option": (I, [P, I, I]),
+    "ac_get_option": (I, [P, I, P]),
+    "ac_batch_set_patterns": (I, [P, I, P, P, I64, P, I64, P, P, P]),
+    "ac_batch_run": (I, [P, P]),
+    "ac_batch_get_mu": (I, [P, I, P]),
+    "ac_batch_info": (I, [P, P]),
+    "ac_batch_timing": (I, [P, ctypes.POINTER(D), ctypes.POINTER(D)]),
+    "ac_batch_autotune": (I, [P, I, P, P]),
     "ac_cdist": (I, [I64, I64, I64, P, P, P, P]),
     "ac_row_log_softmax": (I, [I64, I64, P, P]),
     "ac_row_max": (I, [I64, I64, P, P, P]),

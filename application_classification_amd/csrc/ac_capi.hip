@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -145,22 +146,46 @@ struct ac_ctx {
 
   // Work buffers.
   DevBuf<float> CE, CV, FMaxN, VFm, minCEo, lsO, VR, OUT;
-  // Bundle layout of the in-edge runs (depends on the pattern through Qs): rows of CE, their endpoints and edge ids,
-  // the row of every CSC position, and the CSR walk's {row, src} pairs.
-  DevBuf<Bundle> bundles;
-  DevBuf<int> brun, gslot, bsrc, bdst, beid, rowmap;
-  DevBuf<int2> csr_kb;
-  int64_t nbundles = 0, brows = 0;
+  // Bundle layouts of the in-edge runs, one per number of runs per bundle G: G follows the pattern through Qs,
+  // so a few are kept (most recently used last) for runs of patterns of different widths.  Each holds the rows' edge
+  // endpoints and edge ids and the CSR walk's {row, src} pairs.
+  struct BundleSet {
+    int G = 0;
+    DevBuf<Bundle> bundles;
+    DevBuf<int> gslot, bsrc, bdst, beid;
+    DevBuf<int2> csr_kb;
+    int64_t nbundles = 0, brows = 0;
+  };
+  std::vector<std::unique_ptr<BundleSet>> bsets;
+  BundleSet* bs = nullptr;
+  int walk_B = 0;          // steps per batch of the edge walk (0: default)
+  bool use_graph = false;  // replay iteration pairs from a captured graph
   DevBuf<double> carry_in, carry_out;
   DevBuf<QPart> qpart;
   DevBuf<Lse> ppart, lpart;
   DevBuf<QStat> qstat;  // [3][Q]: prep state, then by iteration parity
   DevBuf<Lse2> pstat, lse_cv, lse_ce;
   int nb_vertex = 0, nb_col = 0, nb_src = 0, VT = 0, VRs = 0;
-  int64_t work_key[4] = {-1, -1, -1, -1};
+  int64_t work_key[5] = {-1, -1, -1, -1, -1};
 
   bool have_result = false;
   int last_iters = 0;
+
+  // Batched scoring: the patterns (host copies, concatenated), where each one's result lives in `bout`, and the
+  // statistics of the last batched run.
+  struct BatchPattern {
+    int P, Q;
+    int64_t v0, e0, out_off;
+  };
+  std::vector<BatchPattern> bp;
+  std::vector<float> b_vfeat, b_efeat;
+  std::vector<int> b_src, b_dst;
+  int64_t b_dV = 0, b_dE = 0;
+  int slab_edges = 128;
+  DevBuf<float> bout;
+  bool have_batch = false;
+  int64_t b_info[4] = {0, 0, 0, 0};
+  double b_prep_ms = 0.0, b_loop_ms = 0.0;
 
   // Profiling: one event after every launch, tagged with the stage it closes.
   bool profiling = false;
@@ -286,7 +311,9 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   ctx->nslots_in = slin;
   ctx->nslots_out = slout;
   ctx->h_runs_in = std::move(rin);
-  ctx->work_key[0] = -1;  // the bundle layout follows the graph, not only its sizes
+  ctx->bsets.clear();  // bundle layouts follow the graph, not only its sizes
+  ctx->bs = nullptr;
+  ctx->work_key[0] = -1;
   ctx->has_data = true;
   ctx->have_result = false;
   return AC_OK;
@@ -360,10 +387,26 @@ extern "C" int ac_set_graph(ac_ctx* ctx, int which, int64_t nV, int64_t dV, cons
   return set_pattern(ctx, nV, dV, vf, nE, dE, ef, src, dst);
 }
 
-// Deals the in-edge runs, longest first (ties in CSC order), out to bundles of G = 64 / (Qs / 4) runs, then lays
-// out the rows of every bundle on the device (k_build_bundles) and re-targets the CSR walk at them.
-static int build_bundles(ac_ctx* ctx) {
-  const int G = ac::walk_groups(ctx->Qs), Bt = ac::walk_batch();
+static int walk_batch_of(const ac_ctx* ctx) { return ctx->walk_B > 0 ? ctx->walk_B : ac::walk_batch(); }
+
+// Bundle lengths are padded to a multiple of the largest walk batch, so one layout serves every batch size.
+constexpr int kMaxWalkBatch = 4;
+// Bundle layouts kept per data graph (one per distinct G among recent patterns).
+constexpr size_t kBundleCache = 8;
+
+// Selects the bundle layout for the current pattern width, building it if it is not cached: the
+// in-edge runs, longest first (ties in CSC order), are dealt out to bundles of G = 64 / (Qs / 4) runs, then the rows
+// of every bundle are laid out on the device (k_build_bundles) and the CSR walk is re-targeted at them.
+static int select_bundles(ac_ctx* ctx) {
+  const int G = ac::walk_groups(ctx->Qs), Bt = kMaxWalkBatch;
+  for (size_t i = 0; i < ctx->bsets.size(); ++i)
+    if (ctx->bsets[i]->G == G) {
+      std::unique_ptr<ac_ctx::BundleSet> hit = std::move(ctx->bsets[i]);
+      ctx->bsets.erase(ctx->bsets.begin() + i);
+      ctx->bsets.push_back(std::move(hit));
+      ctx->bs = ctx->bsets.back().get();
+      return AC_OK;
+    }
   const std::vector<Run>& R = ctx->h_runs_in;
   const int64_t nr = (int64_t)R.size();
   std::vector<int> order(nr);
@@ -390,28 +433,51 @@ static int build_bundles(ac_ctx* ctx) {
   }
   if (base >= (int64_t)INT32_MAX) return ctx->fail(AC_ERR_ARG, "ac_run: the padded edge layout exceeds 2^31 rows");
   hipStream_t st = ctx->stream;
-  ctx->nbundles = nb;
-  ctx->brows = base;
-  HIPCHK(ctx, ctx->bundles.upload(bun.data(), bun.size(), st));
-  HIPCHK(ctx, ctx->brun.upload(brun.data(), brun.size(), st));
-  HIPCHK(ctx, ctx->gslot.upload(gslot.data(), gslot.size(), st));
+  ctx->bs = nullptr;
+  if (ctx->bsets.size() >= kBundleCache) ctx->bsets.erase(ctx->bsets.begin());
+  ctx->bsets.push_back(std::unique_ptr<ac_ctx::BundleSet>(new ac_ctx::BundleSet()));
+  ac_ctx::BundleSet* S = ctx->bsets.back().get();
+  S->G = G;
+  S->nbundles = nb;
+  S->brows = base;
+  DevBuf<int> d_brun, rowmap;
   const size_t rows = (size_t)std::max<int64_t>(1, base), m = (size_t)std::max<int64_t>(1, ctx->de);
-  HIPCHK(ctx, ctx->bsrc.alloc(rows));
-  HIPCHK(ctx, ctx->bdst.alloc(rows));
-  HIPCHK(ctx, ctx->beid.alloc(rows));
-  HIPCHK(ctx, ctx->rowmap.alloc(m));
-  HIPCHK(ctx, ctx->csr_kb.alloc(m));
-  ac::launch_build_bundles(nb, ctx->bundles.p, ctx->brun.p, ctx->runs_in.p, ctx->csc_sd.p, ctx->csc_eid.p, G,
-                           ctx->bsrc.p, ctx->bdst.p, ctx->beid.p, ctx->rowmap.p, st);
-  ac::launch_remap_ks(ctx->de, ctx->csr_ks.p, ctx->rowmap.p, ctx->csr_kb.p, st);
-  HIPCHK(ctx, hipGetLastError());
+  int rc = AC_OK;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == AC_OK)
+      rc = ctx->fail(e == hipErrorOutOfMemory ? AC_ERR_OOM : AC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  chk(S->bundles.upload(bun.data(), bun.size(), st), "bundle upload");
+  chk(d_brun.upload(brun.data(), brun.size(), st), "bundle upload");
+  chk(S->gslot.upload(gslot.data(), gslot.size(), st), "bundle upload");
+  chk(S->bsrc.alloc(rows), "bundle rows");
+  chk(S->bdst.alloc(rows), "bundle rows");
+  chk(S->beid.alloc(rows), "bundle rows");
+  chk(rowmap.alloc(m), "bundle rows");
+  chk(S->csr_kb.alloc(m), "bundle rows");
+  if (rc != AC_OK) {
+    ctx->bsets.pop_back();
+    return rc;
+  }
+  ac::launch_build_bundles(nb, S->bundles.p, d_brun.p, ctx->runs_in.p, ctx->csc_sd.p, ctx->csc_eid.p, G, S->bsrc.p,
+                           S->bdst.p, S->beid.p, rowmap.p, st);
+  ac::launch_remap_ks(ctx->de, ctx->csr_ks.p, rowmap.p, S->csr_kb.p, st);
+  chk(hipGetLastError(), "k_build_bundles");
+  chk(hipStreamSynchronize(st), "k_build_bundles");  // d_brun / rowmap are freed on return
+  if (rc != AC_OK) {
+    ctx->bsets.pop_back();
+    return rc;
+  }
+  ctx->bs = S;
   return AC_OK;
 }
 
 // Sizes every work buffer for the current (data, pattern) pair.
 static int ensure_work(ac_ctx* ctx) {
-  const int64_t key[4] = {ctx->dn, ctx->de, ctx->P, ctx->Q};
-  if (std::equal(key, key + 4, ctx->work_key)) return AC_OK;
+  int rc = select_bundles(ctx);
+  if (rc != AC_OK) return rc;
+  const int64_t key[5] = {ctx->dn, ctx->de, ctx->P, ctx->Q, ctx->bs->brows};
+  if (std::equal(key, key + 5, ctx->work_key)) return AC_OK;
   const int64_t n = ctx->dn, m = ctx->de;
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
   ctx->VT = ac::vertex_tile(P, Q);
@@ -420,9 +486,7 @@ static int ensure_work(ac_ctx* ctx) {
   ctx->nb_col = (int)std::min<int64_t>((n + 255) / 256, 1024);
   ctx->nb_src = std::max(1, ac::walk_blocks(ctx->nruns_out, Qs));
   const int nb_l = std::max(ctx->nb_col, ctx->nb_src);
-  int rc = build_bundles(ctx);
-  if (rc != AC_OK) return rc;
-  HIPCHK(ctx, ctx->CE.alloc((size_t)std::max<int64_t>(1, ctx->brows * Qs)));
+  HIPCHK(ctx, ctx->CE.alloc((size_t)std::max<int64_t>(1, ctx->bs->brows * Qs)));
   HIPCHK(ctx, ctx->CV.alloc((size_t)(n * P)));
   HIPCHK(ctx, ctx->FMaxN.alloc((size_t)(n * Qs)));
   HIPCHK(ctx, ctx->VFm.alloc((size_t)(n * Qs)));
@@ -443,7 +507,7 @@ static int ensure_work(ac_ctx* ctx) {
   // Padding columns of the row-major [n][Qs] arrays are never read as data; keep them zero.
   HIPCHK(ctx, hipMemsetAsync(ctx->VR.p, 0, ctx->VR.n * sizeof(float), ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->qstat.p, 0, ctx->qstat.n * sizeof(QStat), ctx->stream));
-  std::copy(key, key + 4, ctx->work_key);
+  std::copy(key, key + 5, ctx->work_key);
   return AC_OK;
 }
 
@@ -467,7 +531,8 @@ static int mark(ac_ctx* ctx, int stage) {
     if (_rc != AC_OK) return _rc;    \
   } while (0)
 
-static int enqueue_run(ac_ctx* ctx, int n_iter) {
+// Checks the context and sizes the work buffers for a run of the current (data, pattern) pair.
+static int prepare_run(ac_ctx* ctx) {
   if (!ctx) {
     ac::set_global_error("ac_run: null context");
     return AC_ERR_ARG;
@@ -476,17 +541,17 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
   if (ctx->pdV != ctx->dV || ctx->pdE != ctx->dE)
     return ctx->fail(AC_ERR_ARG, "ac_run: feature dimensions of the data and pattern graphs differ");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  int rc = ensure_work(ctx);
-  if (rc != AC_OK) return rc;
-  if (n_iter < 0) n_iter = ctx->P;  // src/main.cu:160
+  return ensure_work(ctx);
+}
+
+// Prep (src/main.cu:107-148), between events ev[0] and ev[1].
+static int enqueue_prep(ac_ctx* ctx) {
   hipStream_t st = ctx->stream;
-  const int64_t n = ctx->dn, m = ctx->de;
+  const int64_t n = ctx->dn;
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
   QStat* qs_prep = ctx->qstat.p;
-  QStat* qs_par[2] = {ctx->qstat.p + std::max(1, Q), ctx->qstat.p + 2 * std::max(1, Q)};
 
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
-  ctx->pev_used = 0;
   MARK(-1);
   // ---- prep (src/main.cu:107-148)
   ac::launch_cdist(n, nullptr, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P, st);
@@ -494,14 +559,14 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
   ac::launch_reduce_lse(ctx->nb_col, P, ctx->lpart.p, ctx->lse_cv.p, st);
   HIPCHK(ctx, hipMemcpyAsync(ctx->pstat.p, ctx->lse_cv.p, sizeof(Lse2) * P, hipMemcpyDeviceToDevice, st));
   if (Q > 0) {
-    ac::launch_cdist(ctx->brows, ctx->beid.p, ctx->efeat.p, (int)ctx->dE, ctx->pf_e.p, Q, round8(Q), Qs, ctx->CE.p,
+    ac::launch_cdist(ctx->bs->brows, ctx->bs->beid.p, ctx->efeat.p, (int)ctx->dE, ctx->pf_e.p, Q, round8(Q), Qs, ctx->CE.p,
                      Qs, st);
     ac::launch_fill(ctx->minCEo.p, n * Qs, INFINITY, st);
     ac::launch_fill(ctx->lsO.p, n * Qs, 0.f, st);
     ac::launch_fill(ctx->VFm.p, n * Qs, -INFINITY, st);
     ac::launch_fill(ctx->FMaxN.p, n * Qs, -INFINITY, st);
     if (ctx->nruns_out > 0) {
-      ac::launch_walk_src(ctx->nruns_out, ctx->runs_out.p, ctx->csr_kb.p, ctx->CE.p, Q, Qs, ctx->minCEo.p,
+      ac::launch_walk_src(ctx->nruns_out, ctx->runs_out.p, ctx->bs->csr_kb.p, ctx->CE.p, Q, Qs, ctx->minCEo.p,
                           ctx->lsO.p, ctx->VFm.p, ctx->carry_out.p, ctx->lpart.p, ctx->nb_src, st);
       ac::launch_walk_src_fixup(ctx->nsplit_out, ctx->splits_out.p, ctx->carry_out.p, Q, Qs, ctx->minCEo.p,
                                 ctx->lsO.p, ctx->VFm.p, st);
@@ -512,19 +577,23 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
       HIPCHK(ctx, hipStreamSynchronize(st));
     }
     ac::launch_prep_scalars(P, Q, ctx->d_ps.p, ctx->d_pd.p, ctx->lse_cv.p, ctx->lse_ce.p, qs_prep, st);
-    ac::launch_walk_max(ctx->nbundles, ctx->bundles.p, ctx->gslot.p, ctx->bsrc.p, ctx->bdst.p, ctx->CE.p, nullptr,
-                        ctx->VRs, Q, Qs, qs_prep, ctx->FMaxN.p, ctx->carry_in.p, st);
+    ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
+                        ctx->CE.p, nullptr, ctx->VRs, Q, Qs, qs_prep, ctx->FMaxN.p, ctx->carry_in.p,
+                        walk_batch_of(ctx), st);
     ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_prep, ctx->FMaxN.p, st);
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
   MARK(AC_STAGE_PREP);
+  return AC_OK;
+}
 
-  // ---- loop (src/main.cu:160-245)
+
+static ac::VertexArgs vertex_args(ac_ctx* ctx) {
   ac::VertexArgs va{};
-  va.n = n;
-  va.P = P;
-  va.Q = Q;
-  va.Qs = Qs;
+  va.n = ctx->dn;
+  va.P = ctx->P;
+  va.Q = ctx->Q;
+  va.Qs = ctx->Qs;
   va.VT = ctx->VT;
   va.CV = ctx->CV.p;
   va.FMaxN = ctx->FMaxN.p;
@@ -541,38 +610,134 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
   va.pst = ctx->pstat.p;
   va.qpart = ctx->qpart.p;
   va.ppart = ctx->ppart.p;
-  const QStat* qs_state = qs_prep;  // scalars finishing FMaxN / VFm as they currently are
-  for (int it = 0; it < n_iter; ++it) {
-    QStat* qs_new = qs_par[it & 1];
-    if (Q > 0) {
-      va.first = it == 0;
-      va.qs = qs_state;
-      ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
-      MARK(AC_STAGE_VERTEX_A);
-      ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_new, st);
-      MARK(AC_STAGE_REDUCE_A);
-      ac::launch_walk_max(ctx->nbundles, ctx->bundles.p, ctx->gslot.p, ctx->bsrc.p, ctx->bdst.p, ctx->CE.p,
-                          ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p, st);
-      MARK(AC_STAGE_EDGE_WALK);
-      ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_new, ctx->FMaxN.p, st);
-      MARK(AC_STAGE_WALK_FIXUP);
-      qs_state = qs_new;
-    }
-    va.first = 0;
+  return va;
+}
+
+// One iteration (src/main.cu:160-245) reading the scalars of state `qs_state`, writing those of the new state into
+// the parity slot of `it`.  Kernels only (no copies or fills), so it can be captured into a graph.
+static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QStat*& qs_state) {
+  hipStream_t st = ctx->stream;
+  const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
+  QStat* qs_new = ctx->qstat.p + (1 + (it & 1)) * std::max(1, Q);
+  if (Q > 0) {
+    va.first = it == 0;
     va.qs = qs_state;
-    ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex, st);
-    MARK(AC_STAGE_VERTEX_C);
-    ac::launch_reduce_lse(ctx->nb_vertex, P, ctx->ppart.p, ctx->pstat.p, st);
-    MARK(AC_STAGE_REDUCE_C);
+    ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
+    MARK(AC_STAGE_VERTEX_A);
+    ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_new, st);
+    MARK(AC_STAGE_REDUCE_A);
+    ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
+                        ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p,
+                        walk_batch_of(ctx), st);
+    MARK(AC_STAGE_EDGE_WALK);
+    ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_new, ctx->FMaxN.p, st);
+    MARK(AC_STAGE_WALK_FIXUP);
+    qs_state = qs_new;
   }
-  // ---- output (src/main.cu:247): MU[v][p] written directly in dn x pn order; timed with the loop, as the
-  // reference's timer covers its final transpose.
-  va.first = n_iter == 0;
+  va.first = 0;
   va.qs = qs_state;
-  ac::launch_vertex(ac::kModeOut, va, ctx->nb_vertex, st);
+  ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex, st);
+  MARK(AC_STAGE_VERTEX_C);
+  ac::launch_reduce_lse(ctx->nb_vertex, P, ctx->ppart.p, ctx->pstat.p, st);
+  MARK(AC_STAGE_REDUCE_C);
+  return AC_OK;
+}
+
+// Output of the current state (src/main.cu:247): MU[v][p] written directly in dn x pn order into OUT.
+static int enqueue_output(ac_ctx* ctx, ac::VertexArgs& va, int done, const QStat* qs_state) {
+  va.first = done == 0;
+  va.qs = qs_state;
+  ac::launch_vertex(ac::kModeOut, va, ctx->nb_vertex, ctx->stream);
+  return AC_OK;
+}
+
+// A copy of the output columns [c0, c0 + ncol) taken after `t` iterations into dst[dn][ncol].
+struct Snapshot {
+  int t, c0, ncol;
+  float* dst;
+};
+
+// The loop (src/main.cu:160-245) from the prep state: n_iter iterations, the snapshots (sorted by t) taken as their
+// iteration counts are reached, and, without snapshots, the final output into OUT.  With `graph`, iterations
+// 1, 2, ... are replayed in pairs (odd, even: one per parity of the scalar slots) from a captured graph wherever no
+// snapshot falls inside the pair.  Ends with event ev[2].
+static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& snaps, bool graph) {
+  hipStream_t st = ctx->stream;
+  ac::VertexArgs va = vertex_args(ctx);
+  const QStat* qs_state = ctx->qstat.p;  // scalars finishing FMaxN / VFm as they currently are
+  size_t si = 0;
+  auto take = [&](int t) -> int {
+    while (si < snaps.size() && snaps[si].t == t) {
+      const Snapshot& S = snaps[si++];
+      int rc = enqueue_output(ctx, va, t, qs_state);
+      if (rc != AC_OK) return rc;
+      ac::launch_copy_cols(ctx->dn, ctx->P, S.c0, S.ncol, ctx->OUT.p, S.dst, st);
+    }
+    return AC_OK;
+  };
+  int rc = take(0);
+  if (rc != AC_OK) return rc;
+  hipGraphExec_t exec = nullptr;
+  int it = 0;
+  while (it < n_iter) {
+    const bool pair = graph && it >= 1 && it + 1 < n_iter && !(si < snaps.size() && snaps[si].t == it + 1);
+    if (pair) {
+      if (!exec) {
+        hipGraph_t g = nullptr;
+        HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const QStat* q0 = qs_state;
+        int rc1 = enqueue_iteration(ctx, va, it, q0);
+        if (rc1 == AC_OK) rc1 = enqueue_iteration(ctx, va, it + 1, q0);
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc1 != AC_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc1;
+        }
+        HIPCHK(ctx, e);
+        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ctx, ei);
+      }
+      const hipError_t el = hipGraphLaunch(exec, st);
+      if (el != hipSuccess) {
+        (void)hipGraphExecDestroy(exec);
+        HIPCHK(ctx, el);
+      }
+      qs_state = ctx->qstat.p + (1 + ((it + 1) & 1)) * std::max(1, ctx->Q);
+      if (ctx->Q == 0) qs_state = ctx->qstat.p;
+      it += 2;
+    } else {
+      rc = enqueue_iteration(ctx, va, it, qs_state);
+      if (rc != AC_OK) break;
+      ++it;
+    }
+    rc = take(it);
+    if (rc != AC_OK) break;
+  }
+  if (exec) {
+    // The graph's launches stay valid after the executable is destroyed once they have been enqueued.
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    (void)hipGraphExecDestroy(exec);
+  }
+  if (rc != AC_OK) return rc;
+  if (snaps.empty()) {
+    rc = enqueue_output(ctx, va, n_iter, qs_state);
+    if (rc != AC_OK) return rc;
+  }
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
   MARK(AC_STAGE_OUTPUT);
+  return AC_OK;
+}
+
+static int enqueue_run(ac_ctx* ctx, int n_iter) {
+  int rc = prepare_run(ctx);
+  if (rc != AC_OK) return rc;
+  if (n_iter < 0) n_iter = ctx->P;  // src/main.cu:160
+  ctx->pev_used = 0;
+  rc = enqueue_prep(ctx);
+  if (rc == AC_OK) rc = enqueue_loop(ctx, n_iter, {}, ctx->use_graph && !ctx->profiling);
+  if (rc != AC_OK) return rc;
   ctx->have_result = true;
   ctx->last_iters = n_iter;
   return AC_OK;
@@ -654,5 +819,296 @@ extern "C" int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms) {
     const int s = ctx->pev_stage[i];
     if (s >= 0 && s < AC_NSTAGES) stage_ms[s] += ms;
   }
+  return AC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Options.
+
+extern "C" int ac_set_option(ac_ctx* ctx, int option, int value) {
+  if (!ctx) return AC_ERR_ARG;
+  switch (option) {
+    case AC_OPT_WALK_BATCH:
+      if (value != 0 && value != 1 && value != 2 && value != 4)
+        return ctx->fail(AC_ERR_ARG, "ac_set_option: the walk batch is 1, 2 or 4 (0: default)");
+      ctx->walk_B = value;
+      return AC_OK;
+    case AC_OPT_GRAPH:
+      ctx->use_graph = value != 0;
+      return AC_OK;
+    case AC_OPT_SLAB_EDGES:
+      if (value < 4 || value > 256) return ctx->fail(AC_ERR_ARG, "ac_set_option: slab edges must be in [4, 256]");
+      ctx->slab_edges = value;
+      return AC_OK;
+    default:
+      return ctx->fail(AC_ERR_ARG, "ac_set_option: unknown option " + std::to_string(option));
+  }
+}
+
+extern "C" int ac_get_option(const ac_ctx* ctx, int option, int* value) {
+  if (!ctx || !value) return AC_ERR_ARG;
+  switch (option) {
+    case AC_OPT_WALK_BATCH:
+      *value = walk_batch_of(ctx);
+      return AC_OK;
+    case AC_OPT_GRAPH:
+      *value = ctx->use_graph ? 1 : 0;
+      return AC_OK;
+    case AC_OPT_SLAB_EDGES:
+      *value = ctx->slab_edges;
+      return AC_OK;
+    default:
+      return AC_ERR_ARG;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Batched scoring.
+
+extern "C" int ac_batch_set_patterns(ac_ctx* ctx, int k, const int32_t* v_off, const int32_t* e_off, int64_t dV,
+                                     const float* vfeat, int64_t dE, const float* efeat, const int32_t* src,
+                                     const int32_t* dst) {
+  if (!ctx) {
+    ac::set_global_error("ac_batch_set_patterns: null context");
+    return AC_ERR_ARG;
+  }
+  if (k < 1 || !v_off || !e_off || dV < 0 || dE < 0)
+    return ctx->fail(AC_ERR_ARG, "ac_batch_set_patterns: need k >= 1, offsets and non-negative widths");
+  if (v_off[0] != 0 || e_off[0] != 0) return ctx->fail(AC_ERR_ARG, "ac_batch_set_patterns: offsets start at 0");
+  std::vector<ac_ctx::BatchPattern> bp((size_t)k);
+  for (int i = 0; i < k; ++i) {
+    const int64_t P = (int64_t)v_off[i + 1] - v_off[i], Q = (int64_t)e_off[i + 1] - e_off[i];
+    if (P < 1 || P > 256 || Q < 0 || Q > 256)
+      return ctx->fail(AC_ERR_ARG, "ac_batch_set_patterns: pattern " + std::to_string(i) +
+                                       " needs 1..256 vertices and at most 256 edges");
+    bp[i] = ac_ctx::BatchPattern{(int)P, (int)Q, v_off[i], e_off[i], 0};
+  }
+  const int64_t nv = v_off[k], ne = e_off[k];
+  if ((nv * dV > 0 && !vfeat) || (ne * dE > 0 && !efeat) || (ne > 0 && (!src || !dst)))
+    return ctx->fail(AC_ERR_ARG, "ac_batch_set_patterns: null array for non-empty patterns");
+  for (int i = 0; i < k; ++i)
+    for (int64_t e = bp[i].e0; e < bp[i].e0 + bp[i].Q; ++e)
+      if (src[e] < 0 || src[e] >= bp[i].P || dst[e] < 0 || dst[e] >= bp[i].P)
+        return ctx->fail(AC_ERR_ARG, "ac_batch_set_patterns: edge " + std::to_string(e - bp[i].e0) + " of pattern " +
+                                         std::to_string(i) + " has an endpoint outside the pattern");
+  ctx->bp = std::move(bp);
+  ctx->b_vfeat.assign(vfeat, vfeat + nv * dV);
+  ctx->b_efeat.assign(efeat, efeat + ne * dE);
+  ctx->b_src.assign(src, src + ne);
+  ctx->b_dst.assign(dst, dst + ne);
+  ctx->b_dV = dV;
+  ctx->b_dE = dE;
+  ctx->have_batch = false;
+  return AC_OK;
+}
+
+// Slabs: patterns in descending order of iterations (ties by index), each slab filled while its edges stay within
+// `cap` and its vertices within 256.
+static std::vector<std::vector<int>> pack_slabs(const ac_ctx* ctx, const std::vector<int>& iters, int cap) {
+  const int k = (int)ctx->bp.size();
+  std::vector<int> order(k);
+  for (int i = 0; i < k; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return iters[a] > iters[b]; });
+  std::vector<std::vector<int>> slabs;
+  int sq = 0, sp = 0;
+  for (int i : order) {
+    const auto& B = ctx->bp[i];
+    if (slabs.empty() || sq + B.Q > cap || sp + B.P > 256) {
+      slabs.emplace_back();
+      sq = sp = 0;
+    }
+    slabs.back().push_back(i);
+    sq += B.Q;
+    sp += B.P;
+  }
+  return slabs;
+}
+
+// Makes the disjoint union of a slab's patterns the context's pattern; col0[j] receives member j's first column.
+static int set_slab_pattern(ac_ctx* ctx, const std::vector<int>& members, std::vector<int>* col0) {
+  const int64_t dV = ctx->b_dV, dE = ctx->b_dE;
+  std::vector<float> vf, ef;
+  std::vector<int32_t> s, d;
+  int base = 0;
+  if (col0) col0->clear();
+  for (int i : members) {
+    const auto& B = ctx->bp[i];
+    if (col0) col0->push_back(base);
+    vf.insert(vf.end(), ctx->b_vfeat.begin() + B.v0 * dV, ctx->b_vfeat.begin() + (B.v0 + B.P) * dV);
+    ef.insert(ef.end(), ctx->b_efeat.begin() + B.e0 * dE, ctx->b_efeat.begin() + (B.e0 + B.Q) * dE);
+    for (int64_t e = B.e0; e < B.e0 + B.Q; ++e) {
+      s.push_back(base + ctx->b_src[e]);
+      d.push_back(base + ctx->b_dst[e]);
+    }
+    base += B.P;
+  }
+  return ac_set_graph(ctx, AC_PATTERN, base, dV, vf.data(), (int64_t)s.size(), dE, ef.data(), s.data(), d.data());
+}
+
+static int batch_iters(ac_ctx* ctx, const int32_t* n_iter, std::vector<int>& iters) {
+  iters.resize(ctx->bp.size());
+  for (size_t i = 0; i < ctx->bp.size(); ++i) iters[i] = (n_iter && n_iter[i] >= 0) ? n_iter[i] : ctx->bp[i].P;
+  return AC_OK;
+}
+
+static int batch_check(ac_ctx* ctx, const char* who) {
+  if (!ctx->has_data) return ctx->fail(AC_ERR_STATE, std::string(who) + ": set the data graph first");
+  if (ctx->bp.empty()) return ctx->fail(AC_ERR_STATE, std::string(who) + ": set the patterns first");
+  if (ctx->b_dV != ctx->dV || ctx->b_dE != ctx->dE)
+    return ctx->fail(AC_ERR_ARG, std::string(who) + ": feature dimensions of the data graph and the patterns differ");
+  return AC_OK;
+}
+
+extern "C" int ac_batch_run(ac_ctx* ctx, const int32_t* n_iter) {
+  if (!ctx) return AC_ERR_ARG;
+  int rc = batch_check(ctx, "ac_batch_run");
+  if (rc != AC_OK) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::vector<int> iters;
+  batch_iters(ctx, n_iter, iters);
+  const std::vector<std::vector<int>> slabs = pack_slabs(ctx, iters, ctx->slab_edges);
+  int64_t total = 0;
+  for (auto& B : ctx->bp) {
+    B.out_off = total;
+    total += ctx->dn * B.P;
+  }
+  ctx->have_batch = false;
+  HIPCHK(ctx, ctx->bout.alloc((size_t)total));
+  double prep = 0.0, loop = 0.0;
+  int64_t upd = 0, done = 0;
+  for (const auto& members : slabs) {
+    std::vector<int> col0;
+    rc = set_slab_pattern(ctx, members, &col0);
+    if (rc != AC_OK) return rc;
+    rc = prepare_run(ctx);
+    if (rc != AC_OK) return rc;
+    std::vector<Snapshot> snaps;
+    int n_max = 0, sq = 0;
+    for (size_t j = 0; j < members.size(); ++j) {
+      const auto& B = ctx->bp[members[j]];
+      snaps.push_back(Snapshot{iters[members[j]], col0[j], B.P, ctx->bout.p + B.out_off});
+      n_max = std::max(n_max, iters[members[j]]);
+      sq += B.Q;
+      upd += ctx->de * B.Q * iters[members[j]];
+    }
+    std::stable_sort(snaps.begin(), snaps.end(), [](const Snapshot& a, const Snapshot& b) { return a.t < b.t; });
+    done += ctx->de * sq * n_max;
+    ctx->pev_used = 0;
+    rc = enqueue_prep(ctx);
+    if (rc == AC_OK) rc = enqueue_loop(ctx, n_max, snaps, ctx->use_graph);
+    if (rc != AC_OK) return rc;
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[2]));
+    float a = 0.f, b = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+    HIPCHK(ctx, hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+    prep += a;
+    loop += b;
+  }
+  ctx->have_result = false;  // OUT holds the last slab, not a pattern set with ac_set_graph
+  ctx->b_info[0] = (int64_t)ctx->bp.size();
+  ctx->b_info[1] = (int64_t)slabs.size();
+  ctx->b_info[2] = upd;
+  ctx->b_info[3] = done;
+  ctx->b_prep_ms = prep;
+  ctx->b_loop_ms = loop;
+  ctx->have_batch = true;
+  return AC_OK;
+}
+
+extern "C" int ac_batch_get_mu(ac_ctx* ctx, int i, float* host_out) {
+  if (!ctx || !host_out) return ctx ? ctx->fail(AC_ERR_ARG, "ac_batch_get_mu: null output") : AC_ERR_ARG;
+  if (!ctx->have_batch) return ctx->fail(AC_ERR_STATE, "ac_batch_get_mu: no result; call ac_batch_run first");
+  if (i < 0 || i >= (int)ctx->bp.size()) return ctx->fail(AC_ERR_ARG, "ac_batch_get_mu: no pattern " + std::to_string(i));
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const auto& B = ctx->bp[i];
+  HIPCHK(ctx, hipMemcpyAsync(host_out, ctx->bout.p + B.out_off, sizeof(float) * ctx->dn * B.P, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return AC_OK;
+}
+
+extern "C" int ac_batch_info(const ac_ctx* ctx, int64_t* out4) {
+  if (!ctx || !out4) return AC_ERR_ARG;
+  if (!ctx->have_batch) return AC_ERR_STATE;
+  std::copy(ctx->b_info, ctx->b_info + 4, out4);
+  return AC_OK;
+}
+
+extern "C" int ac_batch_timing(const ac_ctx* ctx, double* prep_ms, double* loop_ms) {
+  if (!ctx) return AC_ERR_ARG;
+  if (!ctx->have_batch) return AC_ERR_STATE;
+  if (prep_ms) *prep_ms = ctx->b_prep_ms;
+  if (loop_ms) *loop_ms = ctx->b_loop_ms;
+  return AC_OK;
+}
+
+extern "C" int ac_batch_autotune(ac_ctx* ctx, int probe_iters, int32_t* choice_out, double* est_ms_out) {
+  if (!ctx) return AC_ERR_ARG;
+  int rc = batch_check(ctx, "ac_batch_autotune");
+  if (rc != AC_OK) return rc;
+  if (probe_iters < 1 || probe_iters > 64) return ctx->fail(AC_ERR_ARG, "ac_batch_autotune: probe_iters in [1, 64]");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::vector<int> iters;
+  batch_iters(ctx, nullptr, iters);
+  const int caps[3] = {64, 128, 256};
+  const int batches[3] = {1, 2, 4};
+  const int saved_B = ctx->walk_B;
+  double best = std::numeric_limits<double>::infinity();
+  int best_cap = ctx->slab_edges, best_B = walk_batch_of(ctx);
+  hipStream_t st = ctx->stream;
+  for (int cap : caps) {
+    const std::vector<std::vector<int>> slabs = pack_slabs(ctx, iters, cap);
+    if (cap > caps[0] && slabs.size() == pack_slabs(ctx, iters, cap / 2).size()) continue;  // same packing
+    rc = set_slab_pattern(ctx, slabs[0], nullptr);
+    if (rc == AC_OK) rc = prepare_run(ctx);
+    if (rc == AC_OK) {
+      ctx->pev_used = 0;
+      rc = enqueue_prep(ctx);
+    }
+    if (rc != AC_OK) break;
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+    float prep_ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&prep_ms, ctx->ev[0], ctx->ev[1]));
+    const int q0 = std::max(1, ctx->Q);
+    for (int B : batches) {
+      ctx->walk_B = B;
+      ac::VertexArgs va = vertex_args(ctx);
+      const QStat* qs_state = ctx->qstat.p;
+      HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
+      for (int it = 0; it < probe_iters && rc == AC_OK; ++it) rc = enqueue_iteration(ctx, va, it, qs_state);
+      if (rc != AC_OK) break;
+      HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
+      HIPCHK(ctx, hipEventSynchronize(ctx->ev[2]));
+      float ms = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+      // Per slab: prep in proportion to its edges, iterations in proportion to edges x iterations (probe rates).
+      const double it_ms_per_q = ms / probe_iters / q0, prep_ms_per_q = prep_ms / q0;
+      double est = 0.0;
+      for (const auto& members : slabs) {
+        int sq = 0, n_max = 0;
+        for (int i : members) {
+          sq += ctx->bp[i].Q;
+          n_max = std::max(n_max, iters[i]);
+        }
+        est += prep_ms_per_q * std::max(1, sq) + it_ms_per_q * std::max(1, sq) * n_max;
+      }
+      if (est < best) {
+        best = est;
+        best_cap = cap;
+        best_B = B;
+      }
+    }
+    if (rc != AC_OK) break;
+  }
+  ctx->walk_B = saved_B;
+  ctx->have_result = false;
+  if (rc != AC_OK) return rc;
+  ctx->slab_edges = best_cap;
+  ctx->walk_B = best_B;
+  if (choice_out) {
+    choice_out[0] = best_cap;
+    choice_out[1] = best_B;
+  }
+  if (est_ms_out) *est_ms_out = best;
   return AC_OK;
 }

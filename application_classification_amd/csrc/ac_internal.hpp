@@ -101,13 +101,14 @@ void launch_build_bundles(int64_t nbundles, const Bundle* bundles, const int* br
 void launch_remap_ks(int64_t m, const int2* ks, const int* rowmap, int2* out, hipStream_t st);
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
-                     double* carry, hipStream_t st);
+                     double* carry, int B, hipStream_t st);
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
                            float* FMaxN, hipStream_t st);
 void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st);
+void launch_copy_cols(int64_t n, int P, int c0, int ncol, const float* src, float* dst, hipStream_t st);
 int walk_blocks(int64_t nruns, int Qs);
 int walk_groups(int Qs);  // runs per bundle: 64 / (Qs / 4)
-int walk_batch();         // steps per batch of the bundle walk (bundle lengths are multiples of it)
+int walk_batch();         // default steps per batch of the bundle walk: 1, 2 or 4 (bundle lengths are multiples of it)
 int vertex_tile(int P, int Q);
 
 }  // namespace ac

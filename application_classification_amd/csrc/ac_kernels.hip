@@ -542,10 +542,9 @@ static void launch_walk_max_t(int B, int nb, hipStream_t st, int64_t nbundles, c
 
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
-                     double* carry, hipStream_t st) {
+                     double* carry, int B, hipStream_t st) {
   if (nbundles == 0) return;
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
-  const int B = walk_batch();
   if (VR)
     launch_walk_max_t<true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
   else
@@ -946,6 +945,21 @@ void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st) {
     hipLaunchKernelGGL((k_vertex<kModeC, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
   else
     hipLaunchKernelGGL((k_vertex<kModeOut, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+}
+
+// Columns [c0, c0 + ncol) of src[n][P] into dst[n][ncol].
+__global__ void k_copy_cols(int64_t n, int P, int c0, int ncol, const float* __restrict__ src, float* __restrict__ dst) {
+  const int64_t total = n * ncol;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = i / ncol;
+    dst[i] = src[v * P + c0 + (int)(i - v * ncol)];
+  }
+}
+
+void launch_copy_cols(int64_t n, int P, int c0, int ncol, const float* src, float* dst, hipStream_t st) {
+  if (n == 0 || ncol == 0) return;
+  const int blocks = (int)std::min<int64_t>((n * ncol + 255) / 256, 1 << 16);
+  hipLaunchKernelGGL(k_copy_cols, dim3(blocks), dim3(256), 0, st, n, P, c0, ncol, src, dst);
 }
 
 }  // namespace ac

@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import AC_DATA, AC_PATTERN, ACError, HostGraph  # noqa: F401
+from ._lib import AC_DATA, AC_OPT_GRAPH, AC_OPT_SLAB_EDGES, AC_OPT_WALK_BATCH, AC_PATTERN, ACError, HostGraph  # noqa: F401
 
 
 @dataclass
@@ -188,6 +188,59 @@ class Matcher:
     @property
     def stream(self) -> int:
         return self._L.ac_stream(self._h) or 0
+
+    # -- options (ac_set_option) --------------------------------------------------------------------------------
+    def set_option(self, option: int, value: int) -> None:
+        self._check(self._L.ac_set_option(self._h, option, int(value)))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int()
+        self._check(self._L.ac_get_option(self._h, option, ctypes.byref(v)))
+        return v.value
+
+    # -- batched scoring (ac_batch_*) ----------------------------------------------------------------------------
+    def batch_set_patterns(self, patterns) -> None:
+        """Patterns (GraphArrays-like, local vertex ids) scored together against the data graph by batch_run."""
+        pats = [_as_graph(p) for p in patterns]
+        v_off = np.cumsum([0] + [p.num_nodes for p in pats]).astype(np.int32)
+        e_off = np.cumsum([0] + [p.num_edges for p in pats]).astype(np.int32)
+        dV = pats[0].vfeat.shape[1]
+        dE = next((p.efeat.shape[1] for p in pats if p.efeat.ndim == 2 and p.num_edges), 0)
+        vfeat = np.ascontiguousarray(np.concatenate([p.vfeat.reshape(-1, dV) for p in pats]), np.float32)
+        efeat = np.ascontiguousarray(np.concatenate([p.efeat.reshape(-1, dE) for p in pats]), np.float32)
+        src = np.ascontiguousarray(np.concatenate([p.src for p in pats]), np.int32)
+        dst = np.ascontiguousarray(np.concatenate([p.dst for p in pats]), np.int32)
+        self._check(self._L.ac_batch_set_patterns(self._h, len(pats), v_off.ctypes.data, e_off.ctypes.data, dV,
+                                                  vfeat.ctypes.data, dE, efeat.ctypes.data, src.ctypes.data,
+                                                  dst.ctypes.data))
+        self._bpn = [p.num_nodes for p in pats]
+
+    def batch_run(self, n_iter=None) -> None:
+        """Runs every pattern (n_iter: per-pattern iteration counts, default each pattern's vertex count)."""
+        arr = None if n_iter is None else np.ascontiguousarray(n_iter, np.int32)
+        self._check(self._L.ac_batch_run(self._h, None if arr is None else arr.ctypes.data))
+
+    def batch_mu(self, i: int) -> np.ndarray:
+        out = np.empty((self._dn, self._bpn[i]), np.float32)
+        self._check(self._L.ac_batch_get_mu(self._h, i, out.ctypes.data))
+        return out
+
+    def batch_info(self) -> dict:
+        out = (ctypes.c_int64 * 4)()
+        self._check(self._L.ac_batch_info(self._h, out))
+        return {"patterns": out[0], "slabs": out[1], "updates": out[2], "updates_computed": out[3]}
+
+    def batch_timing(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self._L.ac_batch_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def batch_autotune(self, probe_iters: int = 2):
+        """Picks the slab size and walk batch for the current patterns; returns ((slab_edges, walk_batch), est_ms)."""
+        ch = (ctypes.c_int32 * 2)()
+        est = ctypes.c_double()
+        self._check(self._L.ac_batch_autotune(self._h, probe_iters, ch, ctypes.byref(est)))
+        return (ch[0], ch[1]), est.value
 
 
 class ac:

@@ -130,4 +130,30 @@ def s22(seed: int = 0) -> Workload:
     return rmat(22, 64, 16, 24, seed)
 
 
+def uneven_patterns(data: Graph, count: int, seed: int, v_range=(4, 32)) -> list:
+    """`count` planted patterns with V uniform in v_range and E uniform in [V, 2V] (the batched workload's shape);
+    a pattern whose edge count cannot be planted is drawn again from the next seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    k = 0
+    while len(out) < count:
+        nv = int(rng.integers(v_range[0], v_range[1] + 1))
+        ne = int(rng.integers(nv, 2 * nv + 1))
+        for attempt in range(8):
+            try:
+                out.append(planted_pattern(data, nv, ne, seed * 100003 + k * 17 + attempt))
+                break
+            except _lib.ACError:
+                ne = max(nv - 1, ne - max(1, nv // 4))
+        k += 1
+    return out
+
+
+def batched(seed: int = 0, count: int = 256, scale: int = 20, dim: int = 32):
+    """R-MAT scale 20, 32-dim features, with `count` uneven planted patterns: (data graph, patterns)."""
+    src, dst = rmat_edges(scale, 16, RMAT_ABC, seed)
+    g = data_graph(src, dst, 1 << scale, dim, dim, seed)
+    return g, uneven_patterns(g, count, seed)
+
+
 CONFIGS = {"small": small, "s18": s18, "s22": s22}

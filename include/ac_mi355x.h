@@ -135,6 +135,46 @@ void* ac_stream(ac_ctx* ctx);
 int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms);
 
 /* -------------------------------------------------------------------------------------------------------------
+ * Tuning options.  No counterpart in the reference (its launch shapes are fixed); used by the autotuner below and
+ * by callers that pin a configuration.
+ * ------------------------------------------------------------------------------------------------------------- */
+#define AC_OPT_WALK_BATCH 1  /* edge-walk steps per software-pipelined batch: 1, 2 or 4 (0 = default, 2)         */
+#define AC_OPT_GRAPH 2       /* 1: replay iteration pairs from a captured HIP graph (fewer launches); 0: launch each */
+#define AC_OPT_SLAB_EDGES 3  /* batched runs: most pattern edges packed into one slab, 4..256 (default 128)         */
+int ac_set_option(ac_ctx* ctx, int option, int value);
+int ac_get_option(const ac_ctx* ctx, int option, int* value);
+
+/* -------------------------------------------------------------------------------------------------------------
+ * Batched scoring: many patterns against the context's resident data graph.  No counterpart in the reference,
+ * whose driver scores one pattern per process (src/main.cu:66-257); each pattern's result equals that of a plain
+ * run of the same pattern (ac_set_graph + ac_run) to within rounding of the log-sum-exp reductions.
+ *
+ * Patterns are packed into slabs (disjoint unions of patterns: the matcher's normalisations are per pattern vertex
+ * and per pattern edge, so patterns in one slab do not interact), longest iteration count first; each slab runs the
+ * fused pipeline once, and each pattern's MU is copied out after its own iteration count.  A batched run replaces
+ * the context's current pattern.
+ * ------------------------------------------------------------------------------------------------------------- */
+/* k patterns, concatenated: pattern i has vertices [v_off[i], v_off[i+1]) of vfeat (dV features each) and edges
+ * [e_off[i], e_off[i+1]) of efeat / src / dst, whose endpoints are local vertex ids in [0, v_off[i+1] - v_off[i]).
+ * Each pattern has 1..256 vertices and at most 256 edges.  Host arrays, copied. */
+int ac_batch_set_patterns(ac_ctx* ctx, int k, const int32_t* v_off, const int32_t* e_off, int64_t dV,
+                          const float* vfeat, int64_t dE, const float* efeat, const int32_t* src, const int32_t* dst);
+/* Runs every pattern: n_iter[i] iterations for pattern i (NULL: its vertex count, as src/main.cu:160). */
+int ac_batch_run(ac_ctx* ctx, const int32_t* n_iter);
+/* MU of pattern i after ac_batch_run: dn x pn_i, data-vertex-major (src/main.cu:247 order). */
+int ac_batch_get_mu(ac_ctx* ctx, int i, float* host_out);
+/* out[0] = patterns, out[1] = slabs, out[2] = sum over patterns of de * pe_i * iterations_i (edge-pair updates),
+ * out[3] = sum over slabs of de * slab edges * slab iterations (updates computed, including padding). */
+int ac_batch_info(const ac_ctx* ctx, int64_t* out4);
+/* Device time of the last ac_batch_run: prep of all slabs, and iterations + outputs of all slabs. */
+int ac_batch_timing(const ac_ctx* ctx, double* prep_ms, double* loop_ms);
+/* Picks AC_OPT_SLAB_EDGES and AC_OPT_WALK_BATCH for the patterns set by ac_batch_set_patterns: for each candidate
+ * slab size, the slab with the most iterations is prepared once and `probe_iters` iterations of it are timed with
+ * each walk batch; the candidate with the least estimated total time of all slabs is set.  choice_out (may be NULL)
+ * receives {slab edges, walk batch}; est_ms_out (may be NULL) the estimated time. */
+int ac_batch_autotune(ac_ctx* ctx, int probe_iters, int32_t* choice_out, double* est_ms_out);
+
+/* -------------------------------------------------------------------------------------------------------------
  * Stage operators on caller-owned device buffers: the ac:: operator library (src/ac.hxx) with FP32 values and
  * int32 ids, same argument meaning and pattern-major layouts.  Asynchronous on `stream`.
  * ------------------------------------------------------------------------------------------------------------- */
