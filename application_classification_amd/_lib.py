@@ -72,6 +72,7 @@ SIGNATURES = {
     "ac_gen_rmat": (I, [I, I, D, D, D, U64, ctypes.POINTER(I64), ctypes.POINTER(P), ctypes.POINTER(P)]),
     "ac_gen_normal": (I, [P, I64, U64, U64, F]),
     "ac_gen_pattern": (I, [I64, I64, P, P, I, I, U64, P, P, P, P]),
+    "ac_gen_patterns": (I, [I64, I64, P, P, I, P, P, P, P, P, P, P, P]),
     "ac_free": (None, [P]),
     "ac_create": (I, [ctypes.POINTER(P), I]),
     "ac_destroy": (I, [P]),

@@ -46,10 +46,12 @@ int gather_stride(int Qs) {
   return s;
 }
 
+// Device buffer that only grows: a smaller request reuses the allocation (n is the logical size), so runs of
+// patterns of different sizes do not free and re-allocate multi-GB buffers.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
-  size_t n = 0;
+  size_t n = 0, cap = 0;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -57,10 +59,13 @@ struct DevBuf {
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
-    n = 0;
+    n = cap = 0;
   }
   hipError_t alloc(size_t count) {
-    if (count == n && p) return hipSuccess;
+    if (count <= cap && p) {
+      n = count;
+      return hipSuccess;
+    }
     release();
     if (count == 0) return hipSuccess;
     hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
@@ -68,7 +73,7 @@ struct DevBuf {
       p = nullptr;
       return e;
     }
-    n = count;
+    n = cap = count;
     return hipSuccess;
   }
   hipError_t upload(const T* host, size_t count, hipStream_t st) {

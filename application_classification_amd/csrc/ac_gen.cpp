@@ -153,32 +153,39 @@ extern "C" int ac_gen_normal(float* out, int64_t count, uint64_t seed, uint64_t 
   return AC_OK;
 }
 
-extern "C" int ac_gen_pattern(int64_t n, int64_t m, const int32_t* srcs, const int32_t* dsts, int n_pv, int n_pe,
-                              uint64_t seed, int32_t* p_srcs, int32_t* p_dsts, int32_t* vmap, int64_t* emap) {
-  if (n < 1 || m < 1 || !srcs || !dsts || n_pv < 1 || n_pe < n_pv - 1 || !p_srcs || !p_dsts || !vmap || !emap ||
-      n_pv > n) {
-    set_global_error("ac_gen_pattern: bad arguments (need n_pe >= n_pv - 1)");
-    return AC_ERR_ARG;
-  }
-  // Incidence lists of the undirected view: for each vertex, the ids of edges touching it.
-  std::vector<int64_t> off(n + 1, 0);
+namespace {
+
+// Incidence lists of the undirected view of a graph: for each vertex, the ids of the edges touching it.
+struct Incidence {
+  std::vector<int64_t> off, inc;
+};
+
+int build_incidence(int64_t n, int64_t m, const int32_t* srcs, const int32_t* dsts, Incidence& I, const char* who) {
+  I.off.assign(n + 1, 0);
   for (int64_t e = 0; e < m; ++e) {
     if (srcs[e] < 0 || srcs[e] >= n || dsts[e] < 0 || dsts[e] >= n) {
-      set_global_error("ac_gen_pattern: edge id out of range");
+      set_global_error(std::string(who) + ": edge id out of range");
       return AC_ERR_ARG;
     }
-    off[srcs[e] + 1]++;
-    if (dsts[e] != srcs[e]) off[dsts[e] + 1]++;
+    I.off[srcs[e] + 1]++;
+    if (dsts[e] != srcs[e]) I.off[dsts[e] + 1]++;
   }
-  for (int64_t v = 0; v < n; ++v) off[v + 1] += off[v];
-  std::vector<int64_t> inc(off[n]);
-  {
-    std::vector<int64_t> fill(off.begin(), off.end() - 1);
-    for (int64_t e = 0; e < m; ++e) {
-      inc[fill[srcs[e]]++] = e;
-      if (dsts[e] != srcs[e]) inc[fill[dsts[e]]++] = e;
-    }
+  for (int64_t v = 0; v < n; ++v) I.off[v + 1] += I.off[v];
+  I.inc.resize(I.off[n]);
+  std::vector<int64_t> fill(I.off.begin(), I.off.end() - 1);
+  for (int64_t e = 0; e < m; ++e) {
+    I.inc[fill[srcs[e]]++] = e;
+    if (dsts[e] != srcs[e]) I.inc[fill[dsts[e]]++] = e;
   }
+  return AC_OK;
+}
+
+// Plants one pattern: a dense connected vertex set grown greedily from a random start, a BFS spanning tree of its
+// induced edges plus random extra induced edges, then a random pattern-vertex numbering and edge order.
+bool plant_one(int64_t n, const int32_t* srcs, const int32_t* dsts, const Incidence& I, int n_pv, int n_pe,
+               uint64_t seed, int32_t* p_srcs, int32_t* p_dsts, int32_t* vmap, int64_t* emap) {
+  const std::vector<int64_t>& off = I.off;
+  const std::vector<int64_t>& inc = I.inc;
   SeqRng rng(seed, kStreamPattern);
   const int max_scan = 8192;  // incident edges examined per member (hubs are sampled from a random offset)
   for (int attempt = 0; attempt < 2000; ++attempt) {
@@ -269,10 +276,55 @@ extern "C" int ac_gen_pattern(int64_t n, int64_t m, const int32_t* srcs, const i
       p_dsts[q] = perm[index[dsts[e]]];
       emap[q] = e;
     }
-    return AC_OK;
+    return true;
   }
+  return false;
+}
+
+}  // namespace
+
+extern "C" int ac_gen_pattern(int64_t n, int64_t m, const int32_t* srcs, const int32_t* dsts, int n_pv, int n_pe,
+                              uint64_t seed, int32_t* p_srcs, int32_t* p_dsts, int32_t* vmap, int64_t* emap) {
+  if (n < 1 || m < 1 || !srcs || !dsts || n_pv < 1 || n_pe < n_pv - 1 || !p_srcs || !p_dsts || !vmap || !emap ||
+      n_pv > n) {
+    set_global_error("ac_gen_pattern: bad arguments (need n_pe >= n_pv - 1)");
+    return AC_ERR_ARG;
+  }
+  Incidence I;
+  int rc = build_incidence(n, m, srcs, dsts, I, "ac_gen_pattern");
+  if (rc != AC_OK) return rc;
+  if (plant_one(n, srcs, dsts, I, n_pv, n_pe, seed, p_srcs, p_dsts, vmap, emap)) return AC_OK;
   set_global_error("ac_gen_pattern: no connected subgraph with that many induced edges was found");
   return AC_ERR_ARG;
+}
+
+extern "C" int ac_gen_patterns(int64_t n, int64_t m, const int32_t* srcs, const int32_t* dsts, int k,
+                               const int32_t* n_pv, const int32_t* n_pe, const uint64_t* seeds, int32_t* p_srcs,
+                               int32_t* p_dsts, int32_t* vmap, int64_t* emap, int32_t* status) {
+  if (n < 1 || m < 1 || !srcs || !dsts || k < 1 || !n_pv || !n_pe || !seeds || !p_srcs || !p_dsts || !vmap ||
+      !emap || !status) {
+    set_global_error("ac_gen_patterns: bad arguments");
+    return AC_ERR_ARG;
+  }
+  std::vector<int64_t> voff(k + 1, 0), eoff(k + 1, 0);
+  for (int i = 0; i < k; ++i) {
+    if (n_pv[i] < 1 || n_pv[i] > n || n_pe[i] < n_pv[i] - 1) {
+      set_global_error("ac_gen_patterns: pattern " + std::to_string(i) + " needs 1 <= vertices <= n and edges >= vertices - 1");
+      return AC_ERR_ARG;
+    }
+    voff[i + 1] = voff[i] + n_pv[i];
+    eoff[i + 1] = eoff[i] + n_pe[i];
+  }
+  Incidence I;
+  int rc = build_incidence(n, m, srcs, dsts, I, "ac_gen_patterns");
+  if (rc != AC_OK) return rc;
+  parallel_items(k, [&](int64_t i) {
+    status[i] = plant_one(n, srcs, dsts, I, n_pv[i], n_pe[i], seeds[i], p_srcs + eoff[i], p_dsts + eoff[i],
+                          vmap + voff[i], emap + eoff[i])
+                    ? 0
+                    : 1;
+  });
+  return AC_OK;
 }
 
 extern "C" void ac_free(void* p) { std::free(p); }

@@ -130,22 +130,58 @@ def s22(seed: int = 0) -> Workload:
     return rmat(22, 64, 16, 24, seed)
 
 
-def uneven_patterns(data: Graph, count: int, seed: int, v_range=(4, 32)) -> list:
-    """`count` planted patterns with V uniform in v_range and E uniform in [V, 2V] (the batched workload's shape);
-    a pattern whose edge count cannot be planted is drawn again from the next seed."""
-    rng = np.random.default_rng(seed)
+def planted_patterns(data: Graph, sizes, seeds, sigma: float = 0.01):
+    """Planted patterns of the given (vertices, edges) sizes (ac_gen_patterns: one incidence build, planted in
+    parallel); entries that could not be planted are None."""
+    k = len(sizes)
+    npv = np.array([s[0] for s in sizes], np.int32)
+    npe = np.array([s[1] for s in sizes], np.int32)
+    sd = np.array(seeds, np.uint64)
+    voff = np.concatenate([[0], np.cumsum(npv)]).astype(np.int64)
+    eoff = np.concatenate([[0], np.cumsum(npe)]).astype(np.int64)
+    ps = np.empty(eoff[-1], np.int32)
+    pd = np.empty(eoff[-1], np.int32)
+    vmap = np.empty(voff[-1], np.int32)
+    emap = np.empty(eoff[-1], np.int64)
+    status = np.empty(k, np.int32)
+    _lib.check(_lib.lib().ac_gen_patterns(data.num_nodes, data.num_edges, data.src.ctypes.data, data.dst.ctypes.data,
+                                          k, npv.ctypes.data, npe.ctypes.data, sd.ctypes.data, ps.ctypes.data,
+                                          pd.ctypes.data, vmap.ctypes.data, emap.ctypes.data, status.ctypes.data))
     out = []
-    k = 0
-    while len(out) < count:
+    for i in range(k):
+        if status[i]:
+            out.append(None)
+            continue
+        vm = vmap[voff[i]:voff[i + 1]]
+        em = emap[eoff[i]:eoff[i + 1]]
+        nv, ne = int(npv[i]), int(npe[i])
+        vf = data.vfeat[vm] + normal((nv, data.vfeat.shape[1]), int(sd[i]), STREAM_PV_NOISE, sigma)
+        ef = data.efeat[em] + normal((ne, data.efeat.shape[1]), int(sd[i]), STREAM_PE_NOISE, sigma)
+        out.append(Pattern(vf.astype(np.float32), ef.astype(np.float32), ps[eoff[i]:eoff[i + 1]].copy(),
+                           pd[eoff[i]:eoff[i + 1]].copy(), vmap=vm.copy(), emap=em.copy()))
+    return out
+
+
+def uneven_patterns(data: Graph, count: int, seed: int, v_range=(4, 32)) -> list:
+    """`count` planted patterns with V uniform in v_range and E uniform in [V, 2V] (the batched workload's shape).
+    A pattern whose edge count cannot be planted is drawn again with fewer edges and a new seed."""
+    rng = np.random.default_rng(seed)
+    sizes = []
+    for _ in range(count):
         nv = int(rng.integers(v_range[0], v_range[1] + 1))
-        ne = int(rng.integers(nv, 2 * nv + 1))
-        for attempt in range(8):
-            try:
-                out.append(planted_pattern(data, nv, ne, seed * 100003 + k * 17 + attempt))
-                break
-            except _lib.ACError:
-                ne = max(nv - 1, ne - max(1, nv // 4))
-        k += 1
+        sizes.append((nv, int(rng.integers(nv, 2 * nv + 1))))
+    seeds = [seed * 100003 + i * 17 for i in range(count)]
+    out = planted_patterns(data, sizes, seeds)
+    for attempt in range(1, 8):
+        miss = [i for i, p in enumerate(out) if p is None]
+        if not miss:
+            break
+        sz = [(sizes[i][0], max(sizes[i][0] - 1, sizes[i][1] - attempt * max(1, sizes[i][0] // 4))) for i in miss]
+        got = planted_patterns(data, sz, [seeds[i] + attempt for i in miss])
+        for i, p in zip(miss, got):
+            out[i] = p
+    if any(p is None for p in out):
+        raise _lib.ACError(_lib.AC_ERR_ARG, "uneven_patterns: some patterns could not be planted")
     return out
 
 

@@ -151,12 +151,110 @@ def measured_traffic(workload, stage):
     return None
 
 
+def bench_batched(args):
+    """256 uneven planted patterns (V in [4, 32], E in [V, 2V]) against one resident R-MAT scale-20 graph (32-dim):
+    the whole job (prep + iterations + output of every pattern) as 256 sequential plain runs, then as batched runs
+    (autotuned slabs, graph-replayed iterations).  A step is one pass over all 256 patterns."""
+    from application_classification_amd import workloads as W
+    from application_classification_amd.matcher import AC_OPT_GRAPH, Matcher
+
+    def allclose_report(a, b, rtol, atol):
+        err = np.abs(a.astype(np.float64) - b.astype(np.float64))
+        tol = atol + rtol * np.abs(b.astype(np.float64))
+        return int((err > tol).sum()), float((err / tol).max())
+
+    t = time.time()
+    g, pats = W.batched(0)
+    upd = sum(g.num_edges * p.num_edges * p.num_nodes for p in pats)
+    log(f"batched workload: dn={g.num_nodes} de={g.num_edges}, {len(pats)} patterns, "
+        f"sum pe={sum(p.num_edges for p in pats)}, sum pn={sum(p.num_nodes for p in pats)}, "
+        f"{upd:.3e} edge-pair updates, generated in {time.time() - t:.1f} s")
+    m = Matcher(0)
+    m.set_data(g)
+    sample = list(range(0, len(pats), 32))
+    # Sequential plain runs: one ac_set_graph + ac_run per pattern, as the reference's driver scores one pattern.
+    for p in pats[:8]:
+        m.set_pattern(p)
+        m.run()
+    seq_ms = []
+    ref = {}
+    for step in range(args.steps):
+        dev = 0.0
+        t0 = time.perf_counter()
+        for i, p in enumerate(pats):
+            m.set_pattern(p)
+            m.run_async()
+            m.sync()
+            a, b = m.timing()
+            dev += a + b
+            if step == 0 and i in sample:
+                ref[i] = m.mu()
+        seq_ms.append(((time.perf_counter() - t0) * 1e3, dev))
+        log(f"sequential step {step + 1}/{args.steps}: wall {seq_ms[-1][0]:.1f} ms, device {dev:.1f} ms")
+    m.batch_set_patterns(pats)
+    t = time.time()
+    (cap, wb), est = m.batch_autotune(2)
+    log(f"autotune: slab edges {cap}, walk batch {wb} (estimated {est:.1f} ms) in {time.time() - t:.1f} s")
+    m.set_option(AC_OPT_GRAPH, 1)
+    for i in range(args.warmup):
+        m.batch_run()
+        log(f"batched warmup {i + 1}/{args.warmup}: device {sum(m.batch_timing()):.1f} ms")
+    bat_ms = []
+    for step in range(args.steps):
+        t0 = time.perf_counter()
+        m.batch_run()
+        wall = (time.perf_counter() - t0) * 1e3
+        a, b = m.batch_timing()
+        bat_ms.append((wall, a + b, a, b))
+        log(f"batched step {step + 1}/{args.steps}: wall {wall:.1f} ms, device {a + b:.1f} ms (prep {a:.1f}, "
+            f"loop {b:.1f})")
+    info = m.batch_info()
+    worst = 0.0
+    bad = 0
+    for i in sample:
+        nb, ratio = allclose_report(m.batch_mu(i), ref[i], 1e-5, 1e-8)
+        bad += nb
+        worst = max(worst, ratio)
+    m.close()
+    t_seq = min(w for w, _ in seq_ms)
+    t_bat = min(w for w, *_ in bat_ms)
+    out = {
+        "metric": "batched edge-pair updates/s (256 patterns, whole job) vs 256 sequential plain runs",
+        "value": upd / (t_bat * 1e-3),
+        "unit": "edge-pair updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_bat,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "vs_sequential_plain": t_seq / t_bat,
+        "sequential_ms_per_step": t_seq,
+        "sequential_device_ms": min(d for _, d in seq_ms),
+        "batched_device_ms": min(d for _, d, *_ in bat_ms),
+        "batched_prep_ms": min(a for *_, a, _ in bat_ms),
+        "batched_loop_ms": min(b for *_, b in bat_ms),
+        "dtype": "f64",
+        "storage_dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "rmat20-d32-256-uneven-patterns", "data_vertices": g.num_nodes,
+                   "data_edges": g.num_edges, "patterns": len(pats),
+                   "sum_pattern_edges": int(sum(p.num_edges for p in pats)), "slab_edges": cap,
+                   "walk_batch": wb, "slabs": info["slabs"], "updates": info["updates"],
+                   "updates_computed": info["updates_computed"], "parallelism": "single-gpu"},
+        "checks": {"sampled_patterns": len(sample), "violations_vs_sequential": int(bad),
+                   "max_err_over_tol_vs_sequential": worst},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="s22", choices=["s22", "s18", "small"])
+    ap.add_argument("--config", default="s22", choices=["s22", "s18", "small", "batched"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -187,6 +285,10 @@ def main():
         print("bench.py measures one GPU: the matcher is a single-GPU path here (multi-GPU sharding is a design "
               "note, not built); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "batched":
+        bench_batched(args)
+        return
 
     from application_classification_amd import workloads as W
     from application_classification_amd.matcher import Matcher

@@ -80,6 +80,13 @@ int ac_gen_normal(float* out, int64_t count, uint64_t seed, uint64_t stream, flo
  * emap[q] = data edge index of pattern edge q.  Returns AC_ERR_ARG if no such subgraph was found. */
 int ac_gen_pattern(int64_t n, int64_t m, const int32_t* srcs, const int32_t* dsts, int n_pv, int n_pe,
                    uint64_t seed, int32_t* p_srcs, int32_t* p_dsts, int32_t* vmap, int64_t* emap);
+/* k planted patterns at once (the data graph's incidence lists are built once; patterns planted in parallel):
+ * pattern i has n_pv[i] vertices, n_pe[i] edges and seed seeds[i]; its outputs start at the sum of the preceding
+ * counts in p_srcs / p_dsts / emap (edges) and vmap (vertices).  status[i] = 0 if planted, 1 if no subgraph with
+ * that many induced edges was found. */
+int ac_gen_patterns(int64_t n, int64_t m, const int32_t* srcs, const int32_t* dsts, int k, const int32_t* n_pv,
+                    const int32_t* n_pe, const uint64_t* seeds, int32_t* p_srcs, int32_t* p_dsts, int32_t* vmap,
+                    int64_t* emap, int32_t* status);
 void ac_free(void* p);
 
 /* -------------------------------------------------------------------------------------------------------------
