@@ -23,6 +23,7 @@ AC_PATTERN = 1
 AC_OPT_WALK_BATCH = 1
 AC_OPT_GRAPH = 2
 AC_OPT_SLAB_EDGES = 3
+AC_OPT_TAPE = 4
 STAGES = ["prep", "vertex_a", "reduce_a", "edge_walk", "walk_fixup", "vertex_c", "reduce_c", "output"]
 
 ERROR_NAMES = {
@@ -89,6 +90,8 @@ SIGNATURES = {
     "ac_stream": (P, [P]),
     "ac_run_profiled": (I, [P, I, P]),
     "ac_set_option": (I, [P, I, I]),
+    "ac_set_weights": (I, [P, P, P]),
+    "ac_backward": (I, [P, P, P, P]),
     "ac_get_option": (I, [P, I, P]),
     "ac_batch_set_patterns": (I, [P, I, P, P, I64, P, I64, P, P, P]),
     "ac_batch_run": (I, [P, P]),

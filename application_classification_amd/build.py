@@ -25,7 +25,7 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{os.path.join(ROOT, 'include')}"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-HIP_SOURCES = ["ac_kernels.hip", "ac_stage_ops.hip", "ac_capi.hip"]
+HIP_SOURCES = ["ac_kernels.hip", "ac_stage_ops.hip", "ac_adjoint.hip", "ac_capi.hip"]
 HOST_SOURCES = ["ac_gen.cpp", "ac_io.cpp"]
 
 

@@ -1,0 +1,500 @@
+// Adjoint of the matcher (no counterpart in the reference): gradients of a loss on the output log-probabilities with
+// respect to the per-dimension cdist weights, by reverse mode through the prep and every iteration.
+//
+// One iteration maps the state S_i = (MU_i, FMax_i, RMax_i) to S_{i+1} (src/main.cu:160-245):
+//   VF = MU_i[pd] - FMax_i,  VR = MU_i[ps] - RMax_i,  fillR = max_v VF,  fillF = max_v VR
+//   FMax_{i+1}[v] = max(fillF, max_{e->v} (VR[src e] - CE_e) - LSE_FE),  LSE_FE = log sum_e exp(VR[src e] - CE_e)
+//   RMax_{i+1}[v] = max(fillR, max_{e: src=v} (VF[v] - CE_e) - LSE_RE)
+//   MU_{i+1} = LSM_v(-CV + sum_{q: pd=p} FMax_{i+1} + sum_{q: ps=p} RMax_{i+1})
+// Its adjoint, given (gMU, gFMax, gRMax) of S_{i+1}:
+//   gZ = gMU - exp(MU_{i+1}) colsum_v(gMU)                        (log-softmax Jacobian, FP32)
+//   gCV -= gZ;  gF = gFMax + gZ[pd],  gR = gRMax + gZ[ps]
+//   max subgradients: to the recorded argmax in-edge (FMax) / the minimum-cost out-edge (RMax) when the edge term
+//   wins, else to the fill, whose own subgradient goes to the recorded argmax vertex;
+//   log-sum-exp over edges: sum_e exp(FE_e) routes -S_F = -sum_v gE_F to every edge; its vertex part is the closed
+//   form  sum_{e: src=v} exp(VR[v] - CE_e - LSE_FE) = exp(VR[v] - LSE_FE - minCEo[v] + lsO[v]), and its edge-cost
+//   part S_F exp(VR[src] - LSE_FE) exp(-CE_e) is accumulated per (src, q) in cD, so that the final gradient of every
+//   edge cost is  gCE_e = (sparse argmax terms) + (cD[src e] + c_q) exp(-CE_e), formed once at the end.
+// The weight gradients are then  dL/dw_k = sum gC / (2 C) (x_k - p_k)^2  over vertex and edge costs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "ac_adjoint.hpp"
+
+namespace ac {
+
+namespace {
+
+constexpr double kInf = __builtin_huge_val();
+
+// MU[v][p] of a state: (Z - m) - l with Z = -CV + (first ? 0 : the incidence terms of p in ascending q, the FMax
+// term before the RMax term), exactly as the vertex pass evaluates it.
+__device__ __forceinline__ double mu_of(const BwdGraph& G, const BwdState& S, int64_t v, int p) {
+  double acc = -(double)G.CV[v * G.P + p];
+  if (!S.first)
+    for (int k = G.inc_off[p]; k < G.inc_off[p + 1]; ++k) {
+      const int code = G.inc[k];
+      const int q = code >> 1;
+      const QStat s = S.qs[q];
+      if (code & 1) {
+        const float vfm = S.VFm[v * G.Qs + q];
+        const double rn = vfm == -INFINITY ? -kInf : ((double)vfm - s.mRE) - s.lRE;
+        acc += fmax(s.fillR, rn);
+      } else {
+        acc += fmax(s.fillF, (double)S.FMaxN[v * G.Qs + q]);
+      }
+    }
+  return (acc - S.pst[p].m) - S.pst[p].l;
+}
+
+__device__ __forceinline__ double fmax_of(const BwdGraph& G, const BwdState& S, int64_t v, int q) {
+  return fmax(S.qs[q].fillF, (double)S.FMaxN[v * G.Qs + q]);
+}
+
+__device__ __forceinline__ double rn_of(const BwdGraph& G, const BwdState& S, int64_t v, int q) {
+  const float vfm = S.VFm[v * G.Qs + q];
+  return vfm == -INFINITY ? -kInf : ((double)vfm - S.qs[q].mRE) - S.qs[q].lRE;
+}
+
+}  // namespace
+
+// Column sums of X[n][C] (C <= 256): per-block partials in a fixed order.
+__global__ __launch_bounds__(256) void k_colsum(int64_t n, int C, const float* __restrict__ X, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int t = threadIdx.x, RC = 256 / C;
+  const bool active = t < RC * C;
+  const int c = active ? t % C : 0;
+  double s = 0.0;
+  if (active)
+    for (int64_t v = (int64_t)blockIdx.x * RC + t / C; v < n; v += (int64_t)gridDim.x * RC) s += (double)X[v * C + c];
+  red[t] = s;
+  __syncthreads();
+  if (t < C) {
+    double a = 0.0;
+    for (int r = 0; r < RC; ++r) a += red[r * C + t];
+    part[(int64_t)blockIdx.x * C + t] = a;
+  }
+}
+
+// out[c] = sum over parts of part[b][c], in block order.
+__global__ void k_sum_parts(int nparts, int C, const double* __restrict__ part, double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  double a = 0.0;
+  for (int b = 0; b < nparts; ++b) a += part[(int64_t)b * C + c];
+  out[c] = a;
+}
+
+// Log-softmax Jacobian of MU = LSM_v(Z) at state S: gMU <- gZ = gMU - exp(MU) colsum(gMU); gCV -= gZ.
+__global__ __launch_bounds__(256) void k_bwd_out(BwdGraph G, BwdState S, const double* __restrict__ colsum,
+                                                 float* __restrict__ gMU, float* __restrict__ gCV) {
+  const int64_t total = G.n * G.P;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = i / G.P;
+    const int p = (int)(i - v * G.P);
+    const double mu = mu_of(G, S, v, p);
+    const float gz = gMU[i] - __expf((float)mu) * (float)colsum[p];
+    gMU[i] = gz;
+    gCV[i] -= gz;
+  }
+}
+
+// Max-subgradient routing of FMax / RMax of state S (thread per (v, q), q fixed per thread).  gF = gFX + gZ[pd]
+// (with_gz), gR = gRX + gZ[ps].  The edge term takes the gradient where it is strictly above the fill.  Per-block
+// partials part[b][4][Q]: sum gE_F, sum gE_R, fill gradient of fillF, of fillR.
+__global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const float* __restrict__ gZ, int with_gz,
+                                                   const float* __restrict__ gFX, const float* __restrict__ gRX,
+                                                   float* __restrict__ gEF, float* __restrict__ gER,
+                                                   double* __restrict__ part) {
+  __shared__ double red[256][4];
+  const int t = threadIdx.x, Q = G.Q, RQ = 256 / Q;
+  const bool active = t < RQ * Q;
+  const int q = active ? t % Q : 0;
+  double sF = 0.0, sR = 0.0, fF = 0.0, fR = 0.0;
+  if (active) {
+    const int pdq = G.pd[q], psq = G.ps[q];
+    const double fillF = S.qs[q].fillF, fillR = S.qs[q].fillR;
+    for (int64_t v = (int64_t)blockIdx.x * RQ + t / Q; v < G.n; v += (int64_t)gridDim.x * RQ) {
+      const int64_t i = v * G.Qs + q;
+      float gF = gFX[i], gR = gRX[i];
+      if (with_gz) {
+        gF += gZ[v * G.P + pdq];
+        gR += gZ[v * G.P + psq];
+      }
+      const bool eF = (double)S.FMaxN[i] > fillF;
+      const bool eR = rn_of(G, S, v, q) > fillR;
+      gEF[i] = eF ? gF : 0.f;
+      gER[i] = eR ? gR : 0.f;
+      if (eF) sF += gF; else fF += gF;
+      if (eR) sR += gR; else fR += gR;
+    }
+  }
+  red[t][0] = sF;
+  red[t][1] = sR;
+  red[t][2] = fF;
+  red[t][3] = fR;
+  __syncthreads();
+  if (t < Q)
+    for (int k = 0; k < 4; ++k) {
+      double a = 0.0;
+      for (int r = 0; r < RQ; ++r) a += red[r * Q + t][k];
+      part[((int64_t)blockIdx.x * 4 + k) * Q + t] = a;
+    }
+}
+
+// Input side of one iteration (state S = S_i, scalars qs1 of S_{i+1}; red4 = {S_F, S_R, gfillF, gfillR}[Q]):
+//   gVR = -S_F exp(VR - LSE_FE - minCEo + lsO) + [v = argmax VR] gfillF
+//   gVF = gE_R - S_R exp(VF - LSE_RE - minCEo + lsO) + [v = argmax VF] gfillR
+//   cD += S_F exp(VR - LSE_FE) + S_R exp(VF - LSE_RE)            (vertices with out-edges; FP64)
+__global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const QStat* __restrict__ qs1,
+                                                const double* __restrict__ red4, const float* __restrict__ gER,
+                                                float* __restrict__ gVR, float* __restrict__ gVF,
+                                                double* __restrict__ cD) {
+  const int Q = G.Q;
+  const int64_t total = G.n * Q;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = i / Q;
+    const int q = (int)(i - v * Q);
+    const int64_t o = v * G.Qs + q;
+    const double VF = mu_of(G, S, v, G.pd[q]) - fmax_of(G, S, v, q);
+    const double VR = mu_of(G, S, v, G.ps[q]) - fmax(S.qs[q].fillR, rn_of(G, S, v, q));
+    const QStat s1 = qs1[q];
+    const double SF = red4[q], SR = red4[Q + q];
+    const float mo = G.minCEo[o];
+    double gvr = 0.0, gvf = (double)gER[o];
+    if (mo != INFINITY) {
+      const double eF = exp(VR - (s1.mFE + s1.lFE)), eR = exp(VF - (s1.mRE + s1.lRE));
+      const double wo = exp((double)G.lsO[o] - (double)mo);
+      gvr -= SF * eF * wo;
+      gvf -= SR * eR * wo;
+      cD[o] += SF * eF + SR * eR;
+    }
+    if (v == s1.vF) gvr += red4[2 * Q + q];
+    if (v == s1.vR) gvf += red4[3 * Q + q];
+    gVR[o] = (float)gvr;
+    gVF[o] = (float)gvf;
+  }
+}
+
+// Sparse edge terms: gE_F to its in-edge row FA (gVR of the row's source, -gE_F to the row's cost) and gE_R to the
+// minimum-cost out-edge row AO (-gE_R to its cost).  gVR == nullptr: the prep, whose edge terms involve no VR.
+__global__ __launch_bounds__(256) void k_bwd_scatter(BwdGraph G, const float* __restrict__ gEF,
+                                                     const float* __restrict__ gER, const int* __restrict__ FA,
+                                                     const int* __restrict__ AO, const int* __restrict__ bsrc,
+                                                     float* __restrict__ gVR, float* __restrict__ gCE) {
+  const int Q = G.Q;
+  const int64_t total = G.n * Q;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = i / Q;
+    const int q = (int)(i - v * Q);
+    const int64_t o = v * G.Qs + q;
+    const float gf = gEF[o], gr = gER[o];
+    if (gf != 0.f) {
+      const int row = FA[o];
+      if (row >= 0) {
+        if (gVR) atomicAdd(&gVR[(int64_t)bsrc[row] * G.Qs + q], gf);
+        atomicAdd(&gCE[(int64_t)row * G.Qs + q], -gf);
+      }
+    }
+    if (gr != 0.f) {
+      const int row = AO[o];
+      if (row >= 0 && row != INT_MAX) atomicAdd(&gCE[(int64_t)row * G.Qs + q], -gr);
+    }
+  }
+}
+
+// gMU[v][p] = sum over the incidences of p (ascending q) of gVF (q ends at p) or gVR (q starts at p); then
+// gFX = -gVF, gRX = -gVR (the gradients of the input state's FMax / RMax).
+__global__ __launch_bounds__(256) void k_bwd_mu(BwdGraph G, const float* __restrict__ gVR, const float* __restrict__ gVF,
+                                                float* __restrict__ gMU, float* __restrict__ gFX,
+                                                float* __restrict__ gRX) {
+  const int64_t total = G.n * G.P;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = i / G.P;
+    const int p = (int)(i - v * G.P);
+    float a = 0.f;
+    for (int k = G.inc_off[p]; k < G.inc_off[p + 1]; ++k) {
+      const int code = G.inc[k];
+      const int64_t o = v * G.Qs + (code >> 1);
+      a += (code & 1) ? gVR[o] : gVF[o];
+    }
+    gMU[i] = a;
+  }
+  const int64_t tq = G.n * G.Qs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tq; i += (int64_t)gridDim.x * blockDim.x) {
+    gFX[i] = -gVF[i];
+    gRX[i] = -gVR[i];
+  }
+}
+
+// Fill subgradients of the prep state go to the argmax of MU0 = LSM(-CV) of the source / destination pattern vertex:
+// fillF_q = max_v MU0[v][ps q], fillR_q = max_v MU0[v][pd q].  One thread, ascending q.
+__global__ void k_bwd_prep_fill(BwdGraph G, const double* __restrict__ red4, const int* __restrict__ am0,
+                                float* __restrict__ gMU) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int q = 0; q < G.Q; ++q) {
+    const int ps = G.ps[q], pd = G.pd[q];
+    gMU[(int64_t)am0[ps] * G.P + ps] += (float)red4[2 * G.Q + q];
+    gMU[(int64_t)am0[pd] * G.P + pd] += (float)red4[3 * G.Q + q];
+  }
+}
+
+// Rows of the minimum cost among out-edges: AO[src][q] = lowest CE row whose cost equals minCEo[src][q] (CSR order
+// pairs {row, src}); AO starts at INT_MAX.
+__global__ void k_argmin_out(int64_t m, int Q, int Qs, const int2* __restrict__ ks, const float* __restrict__ CE,
+                             const float* __restrict__ minCEo, int* __restrict__ AO) {
+  const int64_t total = m * Q;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i / Q;
+    const int q = (int)(i - k * Q);
+    const int2 e = ks[k];
+    if (CE[(int64_t)e.x * Qs + q] == minCEo[(int64_t)e.y * Qs + q]) atomicMin(&AO[(int64_t)e.y * Qs + q], e.x);
+  }
+}
+
+// Per column p of CV[n][P]: the lowest row of the minimum (the argmax of MU0 = LSM(-CV)).  Block partials.
+__global__ __launch_bounds__(256) void k_col_argmin(int64_t n, int P, const float* __restrict__ CV, float* __restrict__ pv,
+                                                    int* __restrict__ pi) {
+  __shared__ float rv[256];
+  __shared__ int ri[256];
+  const int t = threadIdx.x, RP = 256 / P;
+  const bool active = t < RP * P;
+  const int p = active ? t % P : 0;
+  float best = INFINITY;
+  int bi = INT_MAX;
+  if (active)
+    for (int64_t v = (int64_t)blockIdx.x * RP + t / P; v < n; v += (int64_t)gridDim.x * RP) {
+      const float x = CV[v * P + p];
+      if (x < best) {
+        best = x;
+        bi = (int)v;
+      }
+    }
+  rv[t] = best;
+  ri[t] = bi;
+  __syncthreads();
+  if (t < P) {
+    for (int r = 1; r < RP; ++r) {
+      const float x = rv[r * P + t];
+      const int xi = ri[r * P + t];
+      if (x < best || (x == best && xi < bi)) {
+        best = x;
+        bi = xi;
+      }
+    }
+    pv[(int64_t)blockIdx.x * P + t] = best;
+    pi[(int64_t)blockIdx.x * P + t] = bi;
+  }
+}
+
+__global__ void k_col_argmin_final(int nparts, int P, const float* __restrict__ pv, const int* __restrict__ pi,
+                                   int* __restrict__ am) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  float best = INFINITY;
+  int bi = 0;
+  for (int b = 0; b < nparts; ++b) {
+    const float x = pv[(int64_t)b * P + p];
+    const int xi = pi[(int64_t)b * P + p];
+    if (x < best || (x == best && xi < bi)) {
+      best = x;
+      bi = xi;
+    }
+  }
+  am[p] = bi;
+}
+
+// Edge-cost gradients divided by twice the cost: G[row][q] = (gCE + (cD[src] + cq) exp(-CE)) / (2 CE), 0 for
+// padding rows and zero costs (d sqrt(s) / ds is unbounded at 0; such a pair contributes nothing here).
+__global__ void k_bwd_edge_coef(int64_t nrows, int Q, int Qs, const int* __restrict__ beid, const int* __restrict__ bsrc,
+                                const float* __restrict__ CE, const double* __restrict__ cD,
+                                const double* __restrict__ cq, float* __restrict__ G) {
+  const int64_t total = nrows * Qs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / Qs;
+    const int q = (int)(i - row * Qs);
+    float out = 0.f;
+    if (q < Q && beid[row] >= 0) {
+      const double ce = (double)CE[i];
+      if (ce > 0.0 && ce != kInf) {
+        const double g = (double)G[i] + (cD[(int64_t)bsrc[row] * Qs + q] + cq[q]) * exp(-ce);
+        out = (float)(g / (2.0 * ce));
+      }
+    }
+    G[i] = out;
+  }
+}
+
+// Vertex-cost gradients divided by twice the cost (in place): G = gCV / (2 CV), 0 where CV = 0.
+__global__ void k_bwd_vertex_coef(int64_t total, const float* __restrict__ CV, float* __restrict__ G) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const float c = CV[i];
+    G[i] = c > 0.f ? G[i] / (2.f * c) : 0.f;
+  }
+}
+
+// Outer-product sums over rows: for q < K, k < dim,  A[q][k] = sum_r G[r][q] x_r[k]^2,  B[q][k] = sum_r G[r][q] x_r[k],
+// C[q] = sum_r G[r][q], x_r = feats[idx ? idx[r] : r] (rows with idx < 0 skipped).  A tile of 64 rows is staged in
+// LDS; each thread accumulates a 4 x 4 block of (q, k) in FP32 over the tile and in FP64 across tiles.  Per-block
+// partials part[b][2 K dim + K].
+constexpr int kOuterRows = 64;
+
+__global__ __launch_bounds__(256) void k_outer_acc(int64_t nrows, const int* __restrict__ idx,
+                                                   const float* __restrict__ feats, int dim, const float* __restrict__ G,
+                                                   int Gs, int K, double* __restrict__ part) {
+  __shared__ float gs[kOuterRows][65];
+  __shared__ float xs[kOuterRows][65];
+  const int t = threadIdx.x, tq = t / 16, tk = t % 16;
+  const int nqb = (K + 63) / 64, nkb = (dim + 63) / 64;
+  double* out = part + (int64_t)blockIdx.x * (2 * K * dim + K);
+  for (int qb = 0; qb < nqb; ++qb)
+    for (int kb = 0; kb < nkb; ++kb) {
+      double A[4][4] = {}, B[4][4] = {}, C[4] = {};
+      for (int64_t r0 = (int64_t)blockIdx.x * kOuterRows; r0 < nrows; r0 += (int64_t)gridDim.x * kOuterRows) {
+        __syncthreads();
+        for (int e = t; e < kOuterRows * 64; e += 256) {
+          const int r = e / 64, c = e % 64;
+          const int64_t row = r0 + r;
+          int64_t src = -1;
+          if (row < nrows) src = idx ? (int64_t)idx[row] : row;
+          const int q = qb * 64 + c, k = kb * 64 + c;
+          gs[r][c] = (src >= 0 && q < K) ? G[row * Gs + q] : 0.f;
+          xs[r][c] = (src >= 0 && k < dim) ? feats[src * dim + k] : 0.f;
+        }
+        __syncthreads();
+        float a[4][4] = {}, b[4][4] = {}, cc[4] = {};
+        for (int r = 0; r < kOuterRows; ++r) {
+          float gv[4], xv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            gv[u] = gs[r][tq * 4 + u];
+            xv[u] = xs[r][tk * 4 + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            cc[u] += gv[u];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+              const float gx = gv[u] * xv[w];
+              b[u][w] += gx;
+              a[u][w] = fmaf(gx, xv[w], a[u][w]);
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          C[u] += cc[u];
+#pragma unroll
+          for (int w = 0; w < 4; ++w) {
+            A[u][w] += a[u][w];
+            B[u][w] += b[u][w];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = qb * 64 + tq * 4 + u;
+        if (q >= K) continue;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          const int k = kb * 64 + tk * 4 + w;
+          if (k >= dim) continue;
+          out[(int64_t)q * dim + k] = A[u][w];
+          out[(int64_t)K * dim + (int64_t)q * dim + k] = B[u][w];
+        }
+        if (kb == 0 && tk == 0) out[2 * (int64_t)K * dim + q] = C[u];
+      }
+    }
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int64_t count, int value) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Launchers.
+
+static int grid_for(int64_t total) { return (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 1 << 15)); }
+
+void launch_fill_int(int* p, int64_t count, int value, hipStream_t st) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(count)), dim3(256), 0, st, p, count, value);
+}
+
+void launch_sum_parts(int nparts, int C, const double* part, double* out, hipStream_t st) {
+  if (C == 0) return;
+  hipLaunchKernelGGL(k_sum_parts, dim3((C + 255) / 256), dim3(256), 0, st, nparts, C, part, out);
+}
+
+void launch_colsum(int64_t n, int C, const float* X, double* part, int nparts, double* out, hipStream_t st) {
+  if (C == 0) return;
+  hipLaunchKernelGGL(k_colsum, dim3(nparts), dim3(256), 0, st, n, C, X, part);
+  hipLaunchKernelGGL(k_sum_parts, dim3((C + 255) / 256), dim3(256), 0, st, nparts, C, part, out);
+}
+
+void launch_bwd_out(const BwdGraph& G, const BwdState& S, const double* colsum, float* gMU, float* gCV, hipStream_t st) {
+  hipLaunchKernelGGL(k_bwd_out, dim3(grid_for(G.n * G.P)), dim3(256), 0, st, G, S, colsum, gMU, gCV);
+}
+
+void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gFX,
+                      const float* gRX, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st) {
+  if (G.Q == 0) return;
+  hipLaunchKernelGGL(k_bwd_route, dim3(nparts), dim3(256), 0, st, G, S, gZ, with_gz, gFX, gRX, gEF, gER, part);
+  hipLaunchKernelGGL(k_sum_parts, dim3((4 * G.Q + 255) / 256), dim3(256), 0, st, nparts, 4 * G.Q, part, red4);
+}
+
+void launch_bwd_in(const BwdGraph& G, const BwdState& S, const QStat* qs1, const double* red4, const float* gER,
+                   float* gVR, float* gVF, double* cD, hipStream_t st) {
+  if (G.Q == 0) return;
+  hipLaunchKernelGGL(k_bwd_in, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, S, qs1, red4, gER, gVR, gVF, cD);
+}
+
+void launch_bwd_scatter(const BwdGraph& G, const float* gEF, const float* gER, const int* FA, const int* AO,
+                        const int* bsrc, float* gVR, float* gCE, hipStream_t st) {
+  if (G.Q == 0) return;
+  hipLaunchKernelGGL(k_bwd_scatter, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, gEF, gER, FA, AO, bsrc, gVR, gCE);
+}
+
+void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, float* gFX, float* gRX,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_bwd_mu, dim3(grid_for(G.n * std::max(G.P, G.Qs))), dim3(256), 0, st, G, gVR, gVF, gMU, gFX,
+                     gRX);
+}
+
+void launch_bwd_prep_fill(const BwdGraph& G, const double* red4, const int* am0, float* gMU, hipStream_t st) {
+  if (G.Q == 0) return;
+  hipLaunchKernelGGL(k_bwd_prep_fill, dim3(1), dim3(64), 0, st, G, red4, am0, gMU);
+}
+
+void launch_argmin_out(int64_t m, int Q, int Qs, const int2* ks, const float* CE, const float* minCEo, int* AO,
+                       hipStream_t st) {
+  if (m == 0 || Q == 0) return;
+  hipLaunchKernelGGL(k_argmin_out, dim3(grid_for(m * Q)), dim3(256), 0, st, m, Q, Qs, ks, CE, minCEo, AO);
+}
+
+void launch_col_argmin(int64_t n, int P, const float* CV, float* pv, int* pi, int nparts, int* am, hipStream_t st) {
+  hipLaunchKernelGGL(k_col_argmin, dim3(nparts), dim3(256), 0, st, n, P, CV, pv, pi);
+  hipLaunchKernelGGL(k_col_argmin_final, dim3((P + 255) / 256), dim3(256), 0, st, nparts, P, pv, pi, am);
+}
+
+void launch_bwd_edge_coef(int64_t nrows, int Q, int Qs, const int* beid, const int* bsrc, const float* CE,
+                          const double* cD, const double* cq, float* G, hipStream_t st) {
+  if (nrows == 0) return;
+  hipLaunchKernelGGL(k_bwd_edge_coef, dim3(grid_for(nrows * Qs)), dim3(256), 0, st, nrows, Q, Qs, beid, bsrc, CE, cD,
+                     cq, G);
+}
+
+void launch_bwd_vertex_coef(int64_t total, const float* CV, float* G, hipStream_t st) {
+  hipLaunchKernelGGL(k_bwd_vertex_coef, dim3(grid_for(total)), dim3(256), 0, st, total, CV, G);
+}
+
+void launch_outer_acc(int64_t nrows, const int* idx, const float* feats, int dim, const float* G, int Gs, int K,
+                      double* part, int nparts, hipStream_t st) {
+  hipLaunchKernelGGL(k_outer_acc, dim3(nparts), dim3(256), 0, st, nrows, idx, feats, dim, G, Gs, K, part);
+}
+
+}  // namespace ac

@@ -1,0 +1,56 @@
+// Adjoint kernels (ac_adjoint.hip): device views and launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ac_internal.hpp"
+
+namespace ac {
+
+// The fixed inputs of the vertex side: costs, out-edge statistics and the pattern's incidence lists.
+struct BwdGraph {
+  int64_t n;
+  int P, Q, Qs;
+  const float* CV;      // [n][P]
+  const float* minCEo;  // [n][Qs]
+  const float* lsO;     // [n][Qs]
+  const int* ps;
+  const int* pd;
+  const int* inc_off;  // [P + 1]
+  const int* inc;      // (q << 1) | (1 if q starts at p), ascending q
+};
+
+// One state of the iteration: FMaxN / VFm rows, its scalars and the log-sum-exp of its MU columns.
+struct BwdState {
+  const float* FMaxN;  // [n][Qs]
+  const float* VFm;    // [n][Qs]
+  const QStat* qs;     // [Q]
+  const Lse2* pst;     // [P]
+  int first;           // the prep state: MU = LSM(-CV), no FMax / RMax terms
+};
+
+void launch_fill_int(int* p, int64_t count, int value, hipStream_t st);
+void launch_sum_parts(int nparts, int C, const double* part, double* out, hipStream_t st);
+void launch_colsum(int64_t n, int C, const float* X, double* part, int nparts, double* out, hipStream_t st);
+void launch_bwd_out(const BwdGraph& G, const BwdState& S, const double* colsum, float* gMU, float* gCV, hipStream_t st);
+void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gFX,
+                      const float* gRX, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st);
+void launch_bwd_in(const BwdGraph& G, const BwdState& S, const QStat* qs1, const double* red4, const float* gER,
+                   float* gVR, float* gVF, double* cD, hipStream_t st);
+void launch_bwd_scatter(const BwdGraph& G, const float* gEF, const float* gER, const int* FA, const int* AO,
+                        const int* bsrc, float* gVR, float* gCE, hipStream_t st);
+void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, float* gFX, float* gRX,
+                   hipStream_t st);
+void launch_bwd_prep_fill(const BwdGraph& G, const double* red4, const int* am0, float* gMU, hipStream_t st);
+void launch_argmin_out(int64_t m, int Q, int Qs, const int2* ks, const float* CE, const float* minCEo, int* AO,
+                       hipStream_t st);
+void launch_col_argmin(int64_t n, int P, const float* CV, float* pv, int* pi, int nparts, int* am, hipStream_t st);
+void launch_bwd_edge_coef(int64_t nrows, int Q, int Qs, const int* beid, const int* bsrc, const float* CE,
+                          const double* cD, const double* cq, float* G, hipStream_t st);
+void launch_bwd_vertex_coef(int64_t total, const float* CV, float* G, hipStream_t st);
+void launch_outer_acc(int64_t nrows, const int* idx, const float* feats, int dim, const float* G, int Gs, int K,
+                      double* part, int nparts, hipStream_t st);
+
+}  // namespace ac

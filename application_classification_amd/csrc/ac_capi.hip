@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/ac_mi355x.h"
+#include "ac_adjoint.hpp"
 #include "ac_host.hpp"
 #include "ac_internal.hpp"
 
@@ -141,6 +142,11 @@ struct ac_ctx {
   int64_t nruns_in = 0, nruns_out = 0;
   int nsplit_in = 0, nsplit_out = 0, nslots_in = 0, nslots_out = 0;
 
+  // Per-dimension cdist weights (the adjoint's learnable feature metric); absent = the reference's cdist.
+  bool has_wv = false, has_we = false;
+  DevBuf<double> wv, we;
+  std::vector<double> h_wv, h_we;
+
   // Pattern graph.
   bool has_patt = false;
   int P = 0, Q = 0, Qs = 0, pdV = 0, pdE = 0;
@@ -175,6 +181,20 @@ struct ac_ctx {
 
   bool have_result = false;
   int last_iters = 0;
+
+  // Adjoint tape: with tape_K > 0 a run records the argmax rows / vertices its max stages used and checkpoints the
+  // iteration state every tape_K iterations; ac_backward recomputes each segment from its checkpoint.
+  int tape_K = 0;
+  bool tape_valid = false;
+  int tape_iters = 0;
+  DevBuf<int> FA0, AO, am0, carg_in, tapeFA;
+  DevBuf<float> ckF, ckV, sgF, sgV;
+  DevBuf<QStat> ckQ, sgQ;
+  DevBuf<Lse2> ckP, sgP;
+  // Adjoint work buffers.
+  DevBuf<float> gMU, gCV, gFX, gRX, gEF, gER, gVR, gVF, gCE, apv;
+  DevBuf<double> cD, colsum, bpart, red4, cq, opart, osum;
+  DevBuf<int> api;
 
   // Batched scoring: the patterns (host copies, concatenated), where each one's result lives in `bout`, and the
   // statistics of the last batched run.
@@ -549,6 +569,36 @@ static int prepare_run(ac_ctx* ctx) {
   return ensure_work(ctx);
 }
 
+// Copies the live iteration state (FMaxN, VFm, the given scalars, pstat) into checkpoint c.
+static int checkpoint(ac_ctx* ctx, int c, const QStat* qs) {
+  const size_t nq = (size_t)ctx->dn * ctx->Qs, Q = (size_t)std::max(1, ctx->Q), P = (size_t)ctx->P;
+  hipStream_t st = ctx->stream;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->ckF.p + c * nq, ctx->FMaxN.p, nq * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->ckV.p + c * nq, ctx->VFm.p, nq * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->ckQ.p + c * Q, qs, Q * sizeof(QStat), hipMemcpyDeviceToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->ckP.p + c * P, ctx->pstat.p, P * sizeof(Lse2), hipMemcpyDeviceToDevice, st));
+  return AC_OK;
+}
+
+// Buffers of the adjoint tape for a run of n_iter iterations: checkpoints of iterations 0, K, 2K, ... and the
+// argmax records of the prep.
+static int ensure_tape(ac_ctx* ctx, int n_iter) {
+  const int K = ctx->tape_K;
+  const size_t nck = (size_t)std::max(1, (n_iter + K - 1) / K);
+  const size_t nq = (size_t)ctx->dn * ctx->Qs, Q = (size_t)std::max(1, ctx->Q), P = (size_t)ctx->P;
+  HIPCHK(ctx, ctx->FA0.alloc(nq));
+  HIPCHK(ctx, ctx->AO.alloc(nq));
+  HIPCHK(ctx, ctx->am0.alloc(P));
+  HIPCHK(ctx, ctx->carg_in.alloc((size_t)std::max(1, ctx->nslots_in) * ctx->Qs));
+  HIPCHK(ctx, ctx->ckF.alloc(nck * nq));
+  HIPCHK(ctx, ctx->ckV.alloc(nck * nq));
+  HIPCHK(ctx, ctx->ckQ.alloc(nck * Q));
+  HIPCHK(ctx, ctx->ckP.alloc(nck * P));
+  HIPCHK(ctx, ctx->apv.alloc((size_t)ctx->nb_col * P));
+  HIPCHK(ctx, ctx->api.alloc((size_t)ctx->nb_col * P));
+  return AC_OK;
+}
+
 // Prep (src/main.cu:107-148), between events ev[0] and ev[1].
 static int enqueue_prep(ac_ctx* ctx) {
   hipStream_t st = ctx->stream;
@@ -559,13 +609,14 @@ static int enqueue_prep(ac_ctx* ctx) {
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
   MARK(-1);
   // ---- prep (src/main.cu:107-148)
-  ac::launch_cdist(n, nullptr, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P, st);
+  ac::launch_cdist(n, nullptr, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P,
+                   ctx->has_wv ? ctx->wv.p : nullptr, st);
   ac::launch_col_lse_neg(n, P, ctx->CV.p, ctx->lpart.p, ctx->nb_col, st);
   ac::launch_reduce_lse(ctx->nb_col, P, ctx->lpart.p, ctx->lse_cv.p, st);
   HIPCHK(ctx, hipMemcpyAsync(ctx->pstat.p, ctx->lse_cv.p, sizeof(Lse2) * P, hipMemcpyDeviceToDevice, st));
   if (Q > 0) {
     ac::launch_cdist(ctx->bs->brows, ctx->bs->beid.p, ctx->efeat.p, (int)ctx->dE, ctx->pf_e.p, Q, round8(Q), Qs, ctx->CE.p,
-                     Qs, st);
+                     Qs, ctx->has_we ? ctx->we.p : nullptr, st);
     ac::launch_fill(ctx->minCEo.p, n * Qs, INFINITY, st);
     ac::launch_fill(ctx->lsO.p, n * Qs, 0.f, st);
     ac::launch_fill(ctx->VFm.p, n * Qs, -INFINITY, st);
@@ -582,10 +633,22 @@ static int enqueue_prep(ac_ctx* ctx) {
       HIPCHK(ctx, hipStreamSynchronize(st));
     }
     ac::launch_prep_scalars(P, Q, ctx->d_ps.p, ctx->d_pd.p, ctx->lse_cv.p, ctx->lse_ce.p, qs_prep, st);
+    const bool tape = ctx->tape_K > 0;
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, nullptr, ctx->VRs, Q, Qs, qs_prep, ctx->FMaxN.p, ctx->carry_in.p,
-                        walk_batch_of(ctx), st);
-    ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_prep, ctx->FMaxN.p, st);
+                        walk_batch_of(ctx), st, tape ? ctx->FA0.p : nullptr, tape ? ctx->carg_in.p : nullptr);
+    ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_prep, ctx->FMaxN.p, st,
+                              tape ? ctx->carg_in.p : nullptr, tape ? ctx->FA0.p : nullptr);
+    if (tape) {
+      // Rows of the minimum-cost out-edge of every (v, q) (the RMax argmax), for the adjoint.
+      ac::launch_fill_int(ctx->AO.p, n * Qs, INT32_MAX, st);
+      ac::launch_argmin_out(ctx->de, Q, Qs, ctx->bs->csr_kb.p, ctx->CE.p, ctx->minCEo.p, ctx->AO.p, st);
+    }
+  }
+  if (ctx->tape_K > 0) {
+    ac::launch_col_argmin(n, P, ctx->CV.p, ctx->apv.p, ctx->api.p, ctx->nb_col, ctx->am0.p, st);
+    int rc = checkpoint(ctx, 0, qs_prep);
+    if (rc != AC_OK) return rc;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
   MARK(AC_STAGE_PREP);
@@ -715,6 +778,10 @@ static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& sn
       rc = enqueue_iteration(ctx, va, it, qs_state);
       if (rc != AC_OK) break;
       ++it;
+      if (ctx->tape_K > 0 && it % ctx->tape_K == 0 && it < n_iter) {
+        rc = checkpoint(ctx, it / ctx->tape_K, qs_state);
+        if (rc != AC_OK) break;
+      }
     }
     rc = take(it);
     if (rc != AC_OK) break;
@@ -739,12 +806,19 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
   int rc = prepare_run(ctx);
   if (rc != AC_OK) return rc;
   if (n_iter < 0) n_iter = ctx->P;  // src/main.cu:160
+  ctx->tape_valid = false;
+  if (ctx->tape_K > 0) {
+    rc = ensure_tape(ctx, n_iter);
+    if (rc != AC_OK) return rc;
+  }
   ctx->pev_used = 0;
   rc = enqueue_prep(ctx);
-  if (rc == AC_OK) rc = enqueue_loop(ctx, n_iter, {}, ctx->use_graph && !ctx->profiling);
+  if (rc == AC_OK) rc = enqueue_loop(ctx, n_iter, {}, ctx->use_graph && !ctx->profiling && ctx->tape_K == 0);
   if (rc != AC_OK) return rc;
   ctx->have_result = true;
   ctx->last_iters = n_iter;
+  ctx->tape_valid = ctx->tape_K > 0;
+  ctx->tape_iters = n_iter;
   return AC_OK;
 }
 
@@ -828,6 +902,35 @@ extern "C" int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Weights.
+
+extern "C" int ac_set_weights(ac_ctx* ctx, const float* wv, const float* we) {
+  if (!ctx) {
+    ac::set_global_error("ac_set_weights: null context");
+    return AC_ERR_ARG;
+  }
+  if (!ctx->has_data) return ctx->fail(AC_ERR_STATE, "ac_set_weights: set the data graph first (it fixes the widths)");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto set = [&](const float* w, int64_t dim, bool& has, DevBuf<double>& d, std::vector<double>& h) -> int {
+    if (!w) {
+      has = false;
+      h.clear();
+      return AC_OK;
+    }
+    h.assign(w, w + dim);
+    for (double x : h)
+      if (!(x >= 0.0) || !std::isfinite(x)) return ctx->fail(AC_ERR_ARG, "ac_set_weights: weights must be finite and >= 0");
+    HIPCHK(ctx, d.upload(h.data(), h.size(), ctx->stream));
+    has = true;
+    return AC_OK;
+  };
+  int rc = set(wv, ctx->dV, ctx->has_wv, ctx->wv, ctx->h_wv);
+  if (rc == AC_OK) rc = set(we, ctx->dE, ctx->has_we, ctx->we, ctx->h_we);
+  ctx->have_result = false;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // Options.
 
 extern "C" int ac_set_option(ac_ctx* ctx, int option, int value) {
@@ -845,6 +948,11 @@ extern "C" int ac_set_option(ac_ctx* ctx, int option, int value) {
       if (value < 4 || value > 256) return ctx->fail(AC_ERR_ARG, "ac_set_option: slab edges must be in [4, 256]");
       ctx->slab_edges = value;
       return AC_OK;
+    case AC_OPT_TAPE:
+      if (value < 0 || value > 1024) return ctx->fail(AC_ERR_ARG, "ac_set_option: checkpoint interval in [0, 1024]");
+      ctx->tape_K = value;
+      ctx->tape_valid = false;
+      return AC_OK;
     default:
       return ctx->fail(AC_ERR_ARG, "ac_set_option: unknown option " + std::to_string(option));
   }
@@ -861,6 +969,9 @@ extern "C" int ac_get_option(const ac_ctx* ctx, int option, int* value) {
       return AC_OK;
     case AC_OPT_SLAB_EDGES:
       *value = ctx->slab_edges;
+      return AC_OK;
+    case AC_OPT_TAPE:
+      *value = ctx->tape_K;
       return AC_OK;
     default:
       return AC_ERR_ARG;
@@ -1115,5 +1226,180 @@ extern "C" int ac_batch_autotune(ac_ctx* ctx, int probe_iters, int32_t* choice_o
     choice_out[1] = best_B;
   }
   if (est_ms_out) *est_ms_out = best;
+  return AC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Adjoint.
+
+// One iteration of the recompute: scalars in qs_in -> qs_out, the walk recording its argmax rows into FA.
+static void recompute_iteration(ac_ctx* ctx, int it, const QStat* qs_in, QStat* qs_out, int* FA) {
+  hipStream_t st = ctx->stream;
+  const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
+  ac::VertexArgs va = vertex_args(ctx);
+  if (Q > 0) {
+    va.first = it == 0;
+    va.qs = qs_in;
+    ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
+    ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_out, st);
+    ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
+                        ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_out, ctx->FMaxN.p, ctx->carry_in.p,
+                        walk_batch_of(ctx), st, FA, ctx->carg_in.p);
+    ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_out, ctx->FMaxN.p, st,
+                              ctx->carg_in.p, FA);
+  }
+  va.first = 0;
+  va.qs = Q > 0 ? qs_out : qs_in;
+  ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex, st);
+  ac::launch_reduce_lse(ctx->nb_vertex, P, ctx->ppart.p, ctx->pstat.p, st);
+}
+
+// Host side of the weight gradient from the outer-product sums A, B (K x dim) and C (K) and the pattern features
+// pf (K x dim): dL/dw_k = sum_q A[q][k] - 2 pf[q][k] B[q][k] + pf[q][k]^2 C[q].
+static void combine_outer(const std::vector<double>& s, const std::vector<float>& pf, int K, int dim, float* out) {
+  for (int k = 0; k < dim; ++k) {
+    double g = 0.0;
+    for (int q = 0; q < K; ++q) {
+      const double p = pf[(size_t)q * dim + k];
+      g += s[(size_t)q * dim + k] - 2.0 * p * s[(size_t)K * dim + (size_t)q * dim + k] + p * p * s[2 * (size_t)K * dim + q];
+    }
+    out[k] = (float)g;
+  }
+}
+
+extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, float* grad_we) {
+  if (!ctx) {
+    ac::set_global_error("ac_backward: null context");
+    return AC_ERR_ARG;
+  }
+  if (!grad_mu) return ctx->fail(AC_ERR_ARG, "ac_backward: null grad_mu");
+  if (!ctx->tape_valid || !ctx->have_result)
+    return ctx->fail(AC_ERR_STATE, "ac_backward: no taped run; set AC_OPT_TAPE > 0 and call ac_run first");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t n = ctx->dn;
+  const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs, K = ctx->tape_K, n_iter = ctx->tape_iters;
+  const int Q1 = std::max(1, Q);
+  const size_t nq = (size_t)n * Qs, np = (size_t)n * P;
+  const int nparts = ctx->nb_col;
+  const int64_t brows = ctx->bs->brows;
+  HIPCHK(ctx, ctx->gMU.upload(grad_mu, np, st));
+  HIPCHK(ctx, ctx->gCV.alloc(np));
+  for (DevBuf<float>* b : {&ctx->gFX, &ctx->gRX, &ctx->gEF, &ctx->gER, &ctx->gVR, &ctx->gVF}) HIPCHK(ctx, b->alloc(nq));
+  HIPCHK(ctx, ctx->gCE.alloc((size_t)std::max<int64_t>(1, brows) * Qs));
+  HIPCHK(ctx, ctx->cD.alloc(nq));
+  HIPCHK(ctx, ctx->colsum.alloc((size_t)P));
+  HIPCHK(ctx, ctx->bpart.alloc((size_t)nparts * std::max(4 * Q1, P)));
+  HIPCHK(ctx, ctx->red4.alloc((size_t)4 * Q1));
+  HIPCHK(ctx, ctx->cq.alloc((size_t)Q1));
+  HIPCHK(ctx, ctx->sgF.alloc((size_t)(K + 1) * nq));
+  HIPCHK(ctx, ctx->sgV.alloc((size_t)(K + 1) * nq));
+  HIPCHK(ctx, ctx->sgQ.alloc((size_t)(K + 1) * Q1));
+  HIPCHK(ctx, ctx->sgP.alloc((size_t)(K + 1) * P));
+  HIPCHK(ctx, ctx->tapeFA.alloc((size_t)K * nq));
+  HIPCHK(ctx, hipMemsetAsync(ctx->gCV.p, 0, np * sizeof(float), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->gFX.p, 0, nq * sizeof(float), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->gRX.p, 0, nq * sizeof(float), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->gCE.p, 0, ctx->gCE.n * sizeof(float), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->cD.p, 0, nq * sizeof(double), st));
+
+  ac::BwdGraph G{n, P, Q, Qs, ctx->CV.p, ctx->minCEo.p, ctx->lsO.p, ctx->d_ps.p, ctx->d_pd.p, ctx->d_inc_off.p,
+                 ctx->d_inc.p};
+  auto cpy = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
+  };
+  const int nck = n_iter == 0 ? 0 : (n_iter + K - 1) / K;
+  for (int c = nck - 1; c >= 0; --c) {
+    const int i0 = c * K, i1 = std::min(i0 + K, n_iter);
+    // Recompute the segment from its checkpoint, keeping every state and the walk's argmax rows.
+    HIPCHK(ctx, cpy(ctx->sgF.p, ctx->ckF.p + c * nq, nq * sizeof(float)));
+    HIPCHK(ctx, cpy(ctx->sgV.p, ctx->ckV.p + c * nq, nq * sizeof(float)));
+    HIPCHK(ctx, cpy(ctx->sgQ.p, ctx->ckQ.p + (size_t)c * Q1, Q1 * sizeof(QStat)));
+    HIPCHK(ctx, cpy(ctx->sgP.p, ctx->ckP.p + (size_t)c * P, P * sizeof(Lse2)));
+    HIPCHK(ctx, cpy(ctx->FMaxN.p, ctx->sgF.p, nq * sizeof(float)));
+    HIPCHK(ctx, cpy(ctx->VFm.p, ctx->sgV.p, nq * sizeof(float)));
+    HIPCHK(ctx, cpy(ctx->pstat.p, ctx->sgP.p, P * sizeof(Lse2)));
+    for (int i = i0; i < i1; ++i) {
+      const size_t s = (size_t)(i - i0);
+      recompute_iteration(ctx, i, ctx->sgQ.p + s * Q1, ctx->sgQ.p + (s + 1) * Q1, ctx->tapeFA.p + s * nq);
+      HIPCHK(ctx, cpy(ctx->sgF.p + (s + 1) * nq, ctx->FMaxN.p, nq * sizeof(float)));
+      HIPCHK(ctx, cpy(ctx->sgV.p + (s + 1) * nq, ctx->VFm.p, nq * sizeof(float)));
+      HIPCHK(ctx, cpy(ctx->sgP.p + (s + 1) * P, ctx->pstat.p, P * sizeof(Lse2)));
+      if (Q == 0) HIPCHK(ctx, cpy(ctx->sgQ.p + (s + 1) * Q1, ctx->sgQ.p + s * Q1, Q1 * sizeof(QStat)));
+    }
+    // Reverse through the segment.
+    for (int i = i1 - 1; i >= i0; --i) {
+      const size_t s = (size_t)(i - i0);
+      const ac::BwdState S0{ctx->sgF.p + s * nq, ctx->sgV.p + s * nq, ctx->sgQ.p + s * Q1, ctx->sgP.p + s * P, i == 0};
+      const ac::BwdState S1{ctx->sgF.p + (s + 1) * nq, ctx->sgV.p + (s + 1) * nq, ctx->sgQ.p + (s + 1) * Q1,
+                            ctx->sgP.p + (s + 1) * P, 0};
+      ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
+      ac::launch_bwd_out(G, S1, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
+      ac::launch_bwd_route(G, S1, ctx->gMU.p, 1, ctx->gFX.p, ctx->gRX.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
+                           ctx->red4.p, st);
+      if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVR.p, 0, nq * sizeof(float), st));
+      if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
+      ac::launch_bwd_in(G, S0, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, st);
+      ac::launch_bwd_scatter(G, ctx->gEF.p, ctx->gER.p, ctx->tapeFA.p + s * nq, ctx->AO.p, ctx->bs->bsrc.p,
+                             ctx->gVR.p, ctx->gCE.p, st);
+      ac::launch_bwd_mu(G, ctx->gVR.p, ctx->gVF.p, ctx->gMU.p, ctx->gFX.p, ctx->gRX.p, st);
+    }
+  }
+  // Prep (src/main.cu:107-148): FMax0 / RMax0 from -CE normalised by LSE0 = log sum_e exp(-CE), filled from
+  // MU0 = LSM(-CV).
+  const ac::BwdState S0{ctx->ckF.p, ctx->ckV.p, ctx->ckQ.p, ctx->ckP.p, 1};
+  std::vector<double> cq(Q1, 0.0);
+  if (Q > 0) {
+    ac::launch_bwd_route(G, S0, nullptr, 0, ctx->gFX.p, ctx->gRX.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
+                         ctx->red4.p, st);
+    ac::launch_bwd_scatter(G, ctx->gEF.p, ctx->gER.p, ctx->FA0.p, ctx->AO.p, ctx->bs->bsrc.p, nullptr, ctx->gCE.p, st);
+    ac::launch_bwd_prep_fill(G, ctx->red4.p, ctx->am0.p, ctx->gMU.p, st);
+    std::vector<double> red4((size_t)4 * Q);
+    std::vector<QStat> qs0(Q);
+    HIPCHK(ctx, hipMemcpyAsync(red4.data(), ctx->red4.p, red4.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipMemcpyAsync(qs0.data(), ctx->ckQ.p, Q * sizeof(QStat), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    for (int q = 0; q < Q; ++q) {
+      const double lse0 = qs0[q].mFE + qs0[q].lFE;
+      cq[q] = std::isfinite(lse0) ? (red4[q] + red4[Q + q]) * std::exp(-lse0) : 0.0;
+    }
+  }
+  HIPCHK(ctx, hipMemcpyAsync(ctx->cq.p, cq.data(), Q1 * sizeof(double), hipMemcpyHostToDevice, st));
+  ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
+  ac::launch_bwd_out(G, S0, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
+
+  // dL/dw: gC / (2 C) against the squared feature differences, as outer-product sums over rows.
+  ac::launch_bwd_vertex_coef((int64_t)np, ctx->CV.p, ctx->gCV.p, st);
+  const int dV = (int)ctx->dV, dE = (int)ctx->dE;
+  const int onp = 256;
+  std::vector<float> gwv(dV, 0.f), gwe(dE, 0.f);
+  if (dV > 0) {
+    const size_t w = (size_t)2 * P * dV + P;
+    HIPCHK(ctx, ctx->opart.alloc((size_t)onp * w));
+    HIPCHK(ctx, ctx->osum.alloc(w));
+    ac::launch_outer_acc(n, nullptr, ctx->vfeat.p, dV, ctx->gCV.p, P, P, ctx->opart.p, onp, st);
+    ac::launch_sum_parts(onp, (int)w, ctx->opart.p, ctx->osum.p, st);
+    std::vector<double> sv(w);
+    HIPCHK(ctx, hipMemcpyAsync(sv.data(), ctx->osum.p, w * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    combine_outer(sv, ctx->pvfeat, P, dV, gwv.data());
+  }
+  if (dE > 0 && Q > 0 && brows > 0) {
+    ac::launch_bwd_edge_coef(brows, Q, Qs, ctx->bs->beid.p, ctx->bs->bsrc.p, ctx->CE.p, ctx->cD.p, ctx->cq.p,
+                             ctx->gCE.p, st);
+    const size_t w = (size_t)2 * Q * dE + Q;
+    HIPCHK(ctx, ctx->opart.alloc((size_t)onp * w));
+    HIPCHK(ctx, ctx->osum.alloc(w));
+    ac::launch_outer_acc(brows, ctx->bs->beid.p, ctx->efeat.p, dE, ctx->gCE.p, Qs, Q, ctx->opart.p, onp, st);
+    ac::launch_sum_parts(onp, (int)w, ctx->opart.p, ctx->osum.p, st);
+    std::vector<double> se(w);
+    HIPCHK(ctx, hipMemcpyAsync(se.data(), ctx->osum.p, w * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    combine_outer(se, ctx->pefeat, Q, dE, gwe.data());
+  }
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  if (grad_wv) std::copy(gwv.begin(), gwv.end(), grad_wv);
+  if (grad_we) std::copy(gwe.begin(), gwe.end(), grad_we);
   return AC_OK;
 }

@@ -33,11 +33,13 @@ struct QStat {
   double fillF, fillR;
   double mRE, lRE;  // log sum_e exp(RE[q][e])
   double mFE, lFE;  // log sum_e exp(FE[q][e])
+  int vF, vR;       // the data vertex at which fillF / fillR is attained (lowest on ties; -1 in the prep state)
 };
 // Per-block partial of QStat.
 struct QPart {
   double vfmax, vrmax;
   Lse re, fe;
+  int avf, avr;  // vertex of vfmax / vrmax
 };
 
 // A contiguous run of edges in CSC or CSR order covering whole vertices, or one piece of a vertex whose edge list
@@ -85,7 +87,7 @@ enum VertexMode { kModeA = 0, kModeC = 1, kModeOut = 2 };
 
 // Launchers (ac_kernels.hip).  All asynchronous on `st`.
 void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, const double* pat, int K, int Kp,
-                  int Kstore, float* out, int out_stride, hipStream_t st);
+                  int Kstore, float* out, int out_stride, const double* w, hipStream_t st);
 void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks, hipStream_t st);
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st);
 void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st);
@@ -101,9 +103,9 @@ void launch_build_bundles(int64_t nbundles, const Bundle* bundles, const int* br
 void launch_remap_ks(int64_t m, const int2* ks, const int* rowmap, int2* out, hipStream_t st);
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
-                     double* carry, int B, hipStream_t st);
+                     double* carry, int B, hipStream_t st, int* FA = nullptr, int* carg = nullptr);
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
-                           float* FMaxN, hipStream_t st);
+                           float* FMaxN, hipStream_t st, const int* carg = nullptr, int* FA = nullptr);
 void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st);
 void launch_copy_cols(int64_t n, int P, int c0, int ncol, const float* src, float* dst, hipStream_t st);
 int walk_blocks(int64_t nruns, int Qs);

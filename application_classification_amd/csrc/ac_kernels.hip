@@ -68,6 +68,21 @@ __device__ __forceinline__ Lse lse_merge(Lse a, Lse b) {
 
 __device__ __forceinline__ Lse lse_empty() { return Lse{-kInf, 0.0}; }
 
+// max with the position of the maximum: the larger value wins, equal values keep the lower vertex (-1 = none).
+__device__ __forceinline__ void argmax_merge(double& m, int& a, double x, int b) {
+  if (x > m || (x == m && b >= 0 && (a < 0 || b < a))) {
+    m = x;
+    a = b;
+  }
+}
+
+__device__ __forceinline__ void qpart_merge(QPart& a, const QPart& b) {
+  argmax_merge(a.vfmax, a.avf, b.vfmax, b.avf);
+  argmax_merge(a.vrmax, a.avr, b.vrmax, b.avr);
+  a.re = lse_merge(a.re, b.re);
+  a.fe = lse_merge(a.fe, b.fe);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -75,16 +90,19 @@ __device__ __forceinline__ Lse lse_empty() { return Lse{-kInf, 0.0}; }
 // FP32.  Pattern features are [Kp / 8][dim][8] doubles (Kp = K rounded up to 8, zero padded: each group of eight
 // patterns is one contiguous block, eight values per feature); row r of the output reads
 // feature row idx[r] (idx == nullptr: row r).  Columns K <= c < Kstore of the output are written as 0, and rows with
-// idx[r] < 0 (padding of the bundle layout) as +inf.
+// idx[r] < 0 (padding of the bundle layout) as +inf.  With w != nullptr the distance is weighted per dimension,
+// sqrt(sum_k w_k (a_k - b_k)^2) (the learnable feature metric of the adjoint).
 
 constexpr int kCdTile = 128;
 
 // A tile of kCdTile feature rows is staged in LDS with row-contiguous (coalesced) loads; each thread then computes
 // one row's distances to eight patterns at a time, the pattern values being wave-uniform (scalar loads).
+template <bool kW>
 __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int* __restrict__ idx,
                                                         const float* __restrict__ feats, int dim,
                                                         const double* __restrict__ pat, int K, int Kp, int Kstore,
-                                                        float* __restrict__ out, int out_stride, int tile) {
+                                                        float* __restrict__ out, int out_stride, int tile,
+                                                        const double* __restrict__ w) {
   extern __shared__ float rows[];  // [tile][dim + 1], tile <= kCdTile
   const int ld = dim + 1;
   for (int64_t base = blockIdx.x * (int64_t)tile; base < nrows; base += (int64_t)gridDim.x * tile) {
@@ -107,10 +125,11 @@ __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int
         for (int i = 0; i < dim; ++i) {
           const double xi = (double)x[i];
           const double* pv = pat + ((int64_t)(k0 / 8) * dim + i) * 8;
+          const double wi = kW ? w[i] : 1.0;
 #pragma unroll
           for (int c = 0; c < 8; ++c) {
             const double d = xi - pv[c];
-            a[c] = fma(d, d, a[c]);
+            a[c] = fma(kW ? wi * d : d, d, a[c]);
           }
         }
 #pragma unroll
@@ -126,7 +145,7 @@ __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int
 // Fallback for feature widths too large to stage: one thread per output element, rows read from global memory.
 __global__ void k_cdist_direct(int64_t nrows, const int* __restrict__ idx, const float* __restrict__ feats, int dim,
                                const double* __restrict__ pat, int K, int Kp, int Kstore, float* __restrict__ out,
-                               int out_stride) {
+                               int out_stride, const double* __restrict__ w) {
   const int64_t total = nrows * Kstore;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = e / Kstore;
@@ -137,26 +156,30 @@ __global__ void k_cdist_direct(int64_t nrows, const int* __restrict__ idx, const
     if (k < K && src >= 0)
       for (int i = 0; i < dim; ++i) {
         const double d = (double)x[i] - pat[((int64_t)(k / 8) * dim + i) * 8 + k % 8];
-        a = fma(d, d, a);
+        a = fma(w ? w[i] * d : d, d, a);
       }
     out[r * out_stride + k] = src < 0 ? INFINITY : k < K ? (float)sqrt(a) : 0.f;
   }
 }
 
 void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, const double* pat, int K, int Kp,
-                  int Kstore, float* out, int out_stride, hipStream_t st) {
+                  int Kstore, float* out, int out_stride, const double* w, hipStream_t st) {
   if (nrows == 0 || Kstore == 0) return;
   if (dim <= 255) {
     // At most 64 KB of LDS: 128 rows up to 127 features, fewer rows beyond.
     const int tile = std::min(kCdTile, 16384 / (dim + 1));
     const size_t lds = (size_t)tile * (dim + 1) * sizeof(float);
     const int blocks = (int)std::min<int64_t>((nrows + tile - 1) / tile, 1 << 16);
-    hipLaunchKernelGGL(k_cdist_rows, dim3(blocks), dim3(kCdTile), lds, st, nrows, idx, feats, dim, pat, K, Kp,
-                       Kstore, out, out_stride, tile);
+    if (w)
+      hipLaunchKernelGGL(k_cdist_rows<true>, dim3(blocks), dim3(kCdTile), lds, st, nrows, idx, feats, dim, pat, K,
+                         Kp, Kstore, out, out_stride, tile, w);
+    else
+      hipLaunchKernelGGL(k_cdist_rows<false>, dim3(blocks), dim3(kCdTile), lds, st, nrows, idx, feats, dim, pat, K,
+                         Kp, Kstore, out, out_stride, tile, w);
   } else {
     const int blocks = (int)std::min<int64_t>((nrows * Kstore + 255) / 256, 1 << 16);
     hipLaunchKernelGGL(k_cdist_direct, dim3(blocks), dim3(256), 0, st, nrows, idx, feats, dim, pat, K, Kp, Kstore,
-                       out, out_stride);
+                       out, out_stride, w);
   }
 }
 
@@ -212,30 +235,21 @@ __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart
                                                   QStat* __restrict__ out) {
   __shared__ QPart red[256];
   const int q = blockIdx.x, t = threadIdx.x;
-  QPart s{-kInf, -kInf, lse_empty(), lse_empty()};
-  for (int b = t; b < nparts; b += 256) {
-    const QPart x = part[(int64_t)b * Q + q];
-    s.vfmax = fmax(s.vfmax, x.vfmax);
-    s.vrmax = fmax(s.vrmax, x.vrmax);
-    s.re = lse_merge(s.re, x.re);
-    s.fe = lse_merge(s.fe, x.fe);
-  }
+  QPart s{-kInf, -kInf, lse_empty(), lse_empty(), -1, -1};
+  for (int b = t; b < nparts; b += 256) qpart_merge(s, part[(int64_t)b * Q + q]);
   red[t] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (t < w) {
-      QPart a = red[t], b = red[t + w];
-      a.vfmax = fmax(a.vfmax, b.vfmax);
-      a.vrmax = fmax(a.vrmax, b.vrmax);
-      a.re = lse_merge(a.re, b.re);
-      a.fe = lse_merge(a.fe, b.fe);
+      QPart a = red[t];
+      qpart_merge(a, red[t + w]);
       red[t] = a;
     }
     __syncthreads();
   }
   if (t == 0) {
     const QPart& r = red[0];
-    out[q] = QStat{r.vrmax, r.vfmax, r.re.m, log1p(r.re.t), r.fe.m, log1p(r.fe.t)};
+    out[q] = QStat{r.vrmax, r.vfmax, r.re.m, log1p(r.re.t), r.fe.m, log1p(r.fe.t), r.avr, r.avf};
   }
 }
 
@@ -250,7 +264,7 @@ __global__ void k_prep_scalars(int Q, const int* __restrict__ ps, const int* __r
                                const Lse2* __restrict__ cv, const Lse2* __restrict__ ce, QStat* __restrict__ qs) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= Q) return;
-  qs[q] = QStat{-cv[ps[q]].l, -cv[pd[q]].l, ce[q].m, ce[q].l, ce[q].m, ce[q].l};
+  qs[q] = QStat{-cv[ps[q]].l, -cv[pd[q]].l, ce[q].m, ce[q].l, ce[q].m, ce[q].l, -1, -1};
 }
 
 void launch_prep_scalars(int P, int Q, const int* ps, const int* pd, const Lse2* cv, const Lse2* ce, QStat* qs,
@@ -425,13 +439,16 @@ int walk_batch() {
   return 2;
 }
 
-template <bool kWithVR, int B>
+// kArg: also records, per (v, q), the row of the in-edge attaining the maximum (the first one in walk order on
+// ties; -1 if none) into FA[v][Qs], and per carry slot into carg, for the adjoint's max-subgradient routing.
+template <bool kWithVR, int B, bool kArg>
 __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const Bundle* __restrict__ bundles,
                                                          const int* __restrict__ gslot, const int* __restrict__ bsrc,
                                                          const int* __restrict__ bdst, const float* __restrict__ CE,
                                                          const float* __restrict__ VR, int VRs, int Q, int Qs,
                                                          const QStat* __restrict__ qs, float* __restrict__ FMaxN,
-                                                         double* __restrict__ carry) {
+                                                         double* __restrict__ carry, int* __restrict__ FA,
+                                                         int* __restrict__ carg) {
   const int L4 = Qs / 4, G = 64 / L4, VR4s = VRs / 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane / L4, j = lane - g * L4;
@@ -447,6 +464,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
     sl[c] = q < Q ? qs[q].mFE + qs[q].lFE : 0.0;
   }
   double mx0 = -kInf, mx1 = -kInf, mx2 = -kInf, mx3 = -kInf;
+  int a0 = -1, a1 = -1, a2 = -1, a3 = -1;
   int cur = -1;
   auto emit = [&](int v) {
     float4 o;
@@ -455,6 +473,10 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
     o.z = (float)(mx2 - sl[2]);
     o.w = (float)(mx3 - sl[3]);
     reinterpret_cast<float4*>(FMaxN)[(int64_t)v * L4 + j] = o;
+    if (kArg) {
+      reinterpret_cast<int4*>(FA)[(int64_t)v * L4 + j] = make_int4(a0, a1, a2, a3);
+      a0 = a1 = a2 = a3 = -1;
+    }
     mx0 = mx1 = mx2 = mx3 = -kInf;
   };
   const int GL4 = G * L4;
@@ -477,16 +499,28 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
     _Pragma("unroll") for (int u = 0; u < B; ++u) W[u] =                                                 \
         kWithVR ? VRv[(int64_t)S[u] * VR4s + j] : f4v{0.f, 0.f, 0.f, 0.f};                               \
   }
-#define AC_WK_REDUCE(C, D, W)                                 \
-  _Pragma("unroll") for (int u = 0; u < B; ++u) {             \
-    if (D[u] != cur) {                                        \
-      if (cur >= 0) emit(cur);                                \
-      cur = D[u];                                             \
-    }                                                         \
-    mx0 = fmax(mx0, (double)W[u].x - (double)C[u].x);         \
-    mx1 = fmax(mx1, (double)W[u].y - (double)C[u].y);         \
-    mx2 = fmax(mx2, (double)W[u].z - (double)C[u].z);         \
-    mx3 = fmax(mx3, (double)W[u].w - (double)C[u].w);         \
+#define AC_WK_REDUCE(C, D, W, T)                                                    \
+  _Pragma("unroll") for (int u = 0; u < B; ++u) {                                   \
+    if (D[u] != cur) {                                                              \
+      if (cur >= 0) emit(cur);                                                      \
+      cur = D[u];                                                                   \
+    }                                                                               \
+    const double d0 = (double)W[u].x - (double)C[u].x;                              \
+    const double d1 = (double)W[u].y - (double)C[u].y;                              \
+    const double d2 = (double)W[u].z - (double)C[u].z;                              \
+    const double d3 = (double)W[u].w - (double)C[u].w;                              \
+    if (kArg) {                                                                     \
+      const int row_ = (int)(Bd.base + g) + ((T) * B + u) * G;                       \
+      if (d0 > mx0) mx0 = d0, a0 = row_;                                            \
+      if (d1 > mx1) mx1 = d1, a1 = row_;                                            \
+      if (d2 > mx2) mx2 = d2, a2 = row_;                                            \
+      if (d3 > mx3) mx3 = d3, a3 = row_;                                            \
+    } else {                                                                        \
+      mx0 = fmax(mx0, d0);                                                          \
+      mx1 = fmax(mx1, d1);                                                          \
+      mx2 = fmax(mx2, d2);                                                          \
+      mx3 = fmax(mx3, d3);                                                          \
+    }                                                                               \
   }
   int SA[B], SB[B], DA[B], DB[B];
   f4v CA[B], CB[B], WA[B], WB[B];
@@ -499,11 +533,11 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   for (int t = 0;; t += 2) {
     AC_WK_BATCH(CB, DB, WB, SB, t + 1);
     AC_WK_SRC(SB, t + 3);
-    AC_WK_REDUCE(CA, DA, WA);
+    AC_WK_REDUCE(CA, DA, WA, t);
     if (t + 1 > last) break;
     AC_WK_BATCH(CA, DA, WA, SA, t + 2);
     AC_WK_SRC(SA, t + 4);
-    AC_WK_REDUCE(CB, DB, WB);
+    AC_WK_REDUCE(CB, DB, WB, t + 1);
     if (t + 2 > last) break;
   }
 #undef AC_WK_SRC
@@ -515,40 +549,50 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
     o[1] = mx1;
     o[2] = mx2;
     o[3] = mx3;
+    if (kArg) reinterpret_cast<int4*>(carg)[(int64_t)slot * L4 + j] = make_int4(a0, a1, a2, a3);
   } else if (cur >= 0) {
     emit(cur);
   }
 }
 
-template <bool kWithVR>
+template <bool kWithVR, bool kArg>
 static void launch_walk_max_t(int B, int nb, hipStream_t st, int64_t nbundles, const Bundle* bundles,
                               const int* gslot, const int* bsrc, const int* bdst, const float* CE, const float* VR,
-                              int VRs, int Q, int Qs, const QStat* qs, float* FMaxN, double* carry) {
+                              int VRs, int Q, int Qs, const QStat* qs, float* FMaxN, double* carry, int* FA,
+                              int* carg) {
   switch (B) {
     case 1:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 1>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot, bsrc,
-                         bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 1, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
+                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg);
       break;
     case 4:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 4>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot, bsrc,
-                         bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 4, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
+                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg);
       break;
     default:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 2>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot, bsrc,
-                         bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 2, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
+                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg);
       break;
   }
 }
 
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
-                     double* carry, int B, hipStream_t st) {
+                     double* carry, int B, hipStream_t st, int* FA, int* carg) {
   if (nbundles == 0) return;
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
-  if (VR)
-    launch_walk_max_t<true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+  if (VR && FA)
+    launch_walk_max_t<true, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
+                                  carry, FA, carg);
+  else if (VR)
+    launch_walk_max_t<true, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
+                                   carry, FA, carg);
+  else if (FA)
+    launch_walk_max_t<false, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
+                                   carry, FA, carg);
   else
-    launch_walk_max_t<false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry);
+    launch_walk_max_t<false, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
+                                    carry, FA, carg);
 }
 
 // Bundle layout from the CSC arrays: one block per bundle writes its rows (step-major, run-minor).  rowmap[k] is
@@ -605,49 +649,70 @@ void launch_remap_ks(int64_t m, const int2* ks, const int* rowmap, int2* out, hi
   hipLaunchKernelGGL(k_remap_ks, dim3(blocks), dim3(256), 0, st, m, ks, rowmap, out);
 }
 
-// One wave per split vertex, as k_walk_src_fixup.
+// One wave per split vertex, as k_walk_src_fixup.  With carg: the argmax row of the best piece (the first in merge
+// order on ties) into FA.
 __global__ __launch_bounds__(256) void k_walk_max_fixup(int nsplit, const Split* __restrict__ splits,
                                                         const double* __restrict__ carry, int Q, int Qs,
-                                                        const QStat* __restrict__ qs, float* __restrict__ FMaxN) {
+                                                        const QStat* __restrict__ qs, float* __restrict__ FMaxN,
+                                                        const int* __restrict__ carg, int* __restrict__ FA) {
   __shared__ double red[4][64][4];
+  __shared__ int rarg[4][64][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int L4 = Qs / 4, G = 64 / L4;
   const int g = lane / L4, j = lane - g * L4;
   const int sidx = blockIdx.x * 4 + wave;
   const bool active = sidx < nsplit && g < G;
   double mx[4] = {-kInf, -kInf, -kInf, -kInf};
+  int ag[4] = {-1, -1, -1, -1};
   Split S{0, 0, 0, 0};
   if (sidx < nsplit) S = splits[sidx];
   if (active)
     for (int c = g; c < S.count; c += G) {
       const double4 o = *reinterpret_cast<const double4*>(carry + (int64_t)(S.first + c) * Qs + 4 * j);
-      mx[0] = fmax(mx[0], o.x);
-      mx[1] = fmax(mx[1], o.y);
-      mx[2] = fmax(mx[2], o.z);
-      mx[3] = fmax(mx[3], o.w);
+      const double ov[4] = {o.x, o.y, o.z, o.w};
+      int4 oa = make_int4(-1, -1, -1, -1);
+      if (carg) oa = reinterpret_cast<const int4*>(carg)[(int64_t)(S.first + c) * L4 + j];
+      const int oav[4] = {oa.x, oa.y, oa.z, oa.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (ov[u] > mx[u]) {
+          mx[u] = ov[u];
+          ag[u] = oav[u];
+        }
     }
 #pragma unroll
-  for (int u = 0; u < 4; ++u) red[wave][lane][u] = mx[u];
+  for (int u = 0; u < 4; ++u) {
+    red[wave][lane][u] = mx[u];
+    rarg[wave][lane][u] = ag[u];
+  }
   __syncthreads();
   if (sidx < nsplit && g == 0) {
     float4 out;
     float* po = reinterpret_cast<float*>(&out);
+    int oarg[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       double acc = -kInf;
-      for (int gg = 0; gg < G; ++gg) acc = fmax(acc, red[wave][gg * L4 + j][u]);
+      int aa = -1;
+      for (int gg = 0; gg < G; ++gg)
+        if (red[wave][gg * L4 + j][u] > acc) {
+          acc = red[wave][gg * L4 + j][u];
+          aa = rarg[wave][gg * L4 + j][u];
+        }
       const int q = 4 * j + u;
       po[u] = q < Q ? (float)((acc - qs[q].mFE) - qs[q].lFE) : 0.f;
+      oarg[u] = aa;
     }
     reinterpret_cast<float4*>(FMaxN)[(int64_t)S.v * L4 + j] = out;
+    if (carg) reinterpret_cast<int4*>(FA)[(int64_t)S.v * L4 + j] = make_int4(oarg[0], oarg[1], oarg[2], oarg[3]);
   }
 }
 
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
-                           float* FMaxN, hipStream_t st) {
+                           float* FMaxN, hipStream_t st, const int* carg, int* FA) {
   if (nsplit == 0) return;
   hipLaunchKernelGGL(k_walk_max_fixup, dim3((unsigned)((nsplit + 3) / 4)), dim3(256), 0, st, nsplit, splits, carry,
-                     Q, Qs, qs, FMaxN);
+                     Q, Qs, qs, FMaxN, carg, FA);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -755,6 +820,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
   const bool qthread = t < RQ * Q;
   const int q = qthread ? t % Q : 0, rq = qthread ? t / Q : 0;
   double vfmax = -kInf, vrmax = -kInf;
+  int avf = -1, avr = -1;
   Lse lre = lse_empty(), lfe = lse_empty();
 
   const f4v* F4 = reinterpret_cast<const f4v*>(a.FMaxN);
@@ -887,8 +953,14 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
           const double vf = MUs[r * ldm + q_pd] - fm;
           const double vr = MUs[r * ldm + q_ps] - rm;
           a.VR[(v0 + r) * a.VRs + q] = (float)vr;
-          vfmax = fmax(vfmax, vf);
-          vrmax = fmax(vrmax, vr);
+          if (vf > vfmax) {  // rows ascend within a thread, so the first maximum is the lowest vertex
+            vfmax = vf;
+            avf = (int)(v0 + r);
+          }
+          if (vr > vrmax) {
+            vrmax = vr;
+            avr = (int)(v0 + r);
+          }
           const bool out_edges = mo != INFINITY;
           const double xx = vf - (double)mo;
           a.VFm[i] = out_edges ? (float)xx : -INFINITY;
@@ -904,17 +976,11 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
   // Block partials, merged in a fixed order.
   if (kMode == kModeA) {
     QPart* red = reinterpret_cast<QPart*>(lds);
-    if (qthread) red[rq * Q + q] = QPart{vfmax, vrmax, lre, lfe};
+    if (qthread) red[rq * Q + q] = QPart{vfmax, vrmax, lre, lfe, avf, avr};
     __syncthreads();
     if (t < Q) {
-      QPart acc{-kInf, -kInf, lse_empty(), lse_empty()};
-      for (int r = 0; r < RQ; ++r) {
-        const QPart x = red[r * Q + t];
-        acc.vfmax = fmax(acc.vfmax, x.vfmax);
-        acc.vrmax = fmax(acc.vrmax, x.vrmax);
-        acc.re = lse_merge(acc.re, x.re);
-        acc.fe = lse_merge(acc.fe, x.fe);
-      }
+      QPart acc{-kInf, -kInf, lse_empty(), lse_empty(), -1, -1};
+      for (int r = 0; r < RQ; ++r) qpart_merge(acc, red[r * Q + t]);
       a.qpart[(int64_t)blockIdx.x * Q + t] = acc;
     }
   } else if (kMode == kModeC) {

@@ -14,7 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import AC_DATA, AC_OPT_GRAPH, AC_OPT_SLAB_EDGES, AC_OPT_WALK_BATCH, AC_PATTERN, ACError, HostGraph  # noqa: F401
+from ._lib import (AC_DATA, AC_OPT_GRAPH, AC_OPT_SLAB_EDGES, AC_OPT_TAPE, AC_OPT_WALK_BATCH, AC_PATTERN,  # noqa: F401
+                   ACError, HostGraph)
 
 
 @dataclass
@@ -142,7 +143,9 @@ class Matcher:
         return g
 
     def set_data(self, g) -> None:
-        self._dn = self._set(AC_DATA, g).num_nodes
+        g = self._set(AC_DATA, g)
+        self._dn = g.num_nodes
+        self._feat_dims = (g.vfeat.shape[1], g.efeat.shape[1] if g.efeat.ndim == 2 else 0)
 
     def set_pattern(self, g) -> None:
         self._pn = self._set(AC_PATTERN, g).num_nodes
@@ -188,6 +191,26 @@ class Matcher:
     @property
     def stream(self) -> int:
         return self._L.ac_stream(self._h) or 0
+
+    # -- adjoint (ac_set_weights / ac_backward) -----------------------------------------------------------------
+    def set_weights(self, wv=None, we=None) -> None:
+        """Per-dimension cdist weights (None: unweighted); set after the data graph."""
+        a = None if wv is None else np.ascontiguousarray(wv, np.float32)
+        b = None if we is None else np.ascontiguousarray(we, np.float32)
+        self._check(self._L.ac_set_weights(self._h, None if a is None else a.ctypes.data,
+                                           None if b is None else b.ctypes.data))
+        self._dims = (0 if a is None else a.size, 0 if b is None else b.size)
+
+    def backward(self, grad_mu):
+        """dL/dMU (dn x pn) of the last taped run (AC_OPT_TAPE > 0) -> (dL/dwv, dL/dwe)."""
+        g = np.ascontiguousarray(grad_mu, np.float32)
+        if g.shape != (self._dn, self._pn):
+            raise ValueError(f"grad_mu must be {self._dn} x {self._pn}")
+        dv, de = self._feat_dims
+        gwv = np.zeros(dv, np.float32)
+        gwe = np.zeros(de, np.float32)
+        self._check(self._L.ac_backward(self._h, g.ctypes.data, gwv.ctypes.data, gwe.ctypes.data))
+        return gwv, gwe
 
     # -- options (ac_set_option) --------------------------------------------------------------------------------
     def set_option(self, option: int, value: int) -> None:

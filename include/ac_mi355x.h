@@ -148,8 +148,22 @@ int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms);
 #define AC_OPT_WALK_BATCH 1  /* edge-walk steps per software-pipelined batch: 1, 2 or 4 (0 = default, 2)         */
 #define AC_OPT_GRAPH 2       /* 1: replay iteration pairs from a captured HIP graph (fewer launches); 0: launch each */
 #define AC_OPT_SLAB_EDGES 3  /* batched runs: most pattern edges packed into one slab, 4..256 (default 128)         */
+#define AC_OPT_TAPE 4        /* > 0: runs record what ac_backward needs, checkpointing every this many iterations  */
 int ac_set_option(ac_ctx* ctx, int option, int value);
 int ac_get_option(const ac_ctx* ctx, int option, int* value);
+
+/* -------------------------------------------------------------------------------------------------------------
+ * Adjoint (no counterpart in the reference): per-dimension cdist weights, d = sqrt(sum_k w_k (a_k - b_k)^2), and
+ * the gradient of a loss on the output MU with respect to them, by reverse mode through the prep and every
+ * iteration (max subgradients routed through the argmax rows / vertices recorded in the forward pass; the
+ * log-softmax Jacobians in FP32), recomputing each segment of AC_OPT_TAPE iterations from its checkpoint.
+ * ------------------------------------------------------------------------------------------------------------- */
+/* Weights of the vertex (dV) and edge (dE) feature distances, host arrays, >= 0; NULL = unweighted (the
+ * reference's cdist, src/ac.hxx:33-56).  Set after the data graph. */
+int ac_set_weights(ac_ctx* ctx, const float* wv, const float* we);
+/* After ac_run with AC_OPT_TAPE > 0: grad_mu = dL/dMU (dn x pn, the ac_get_mu layout, host) -> grad_wv = dL/dwv
+ * (dV), grad_we = dL/dwe (dE), host arrays (either may be NULL). */
+int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, float* grad_we);
 
 /* -------------------------------------------------------------------------------------------------------------
  * Batched scoring: many patterns against the context's resident data graph.  No counterpart in the reference,

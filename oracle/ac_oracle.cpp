@@ -45,17 +45,20 @@ namespace {
 
 constexpr int64_t kBlock = 4096;  // fixed reduction block, independent of the thread count
 
-// ac::cdist (src/ac.hxx:33-56): out[j * n_b + i] = sqrt(sum_k (a_j[k] - b_i[k])^2), a = pattern, b = data.
-// Pinned by: scipy.spatial.distance.cdist on random inputs (tests/test_oracle_pins.py::test_cdist_*).
+// ac::cdist (src/ac.hxx:33-56): out[j * n_b + i] = sqrt(sum_k w_k (a_j[k] - b_i[k])^2), a = pattern, b = data;
+// w == nullptr is the reference (all w_k = 1).  The per-dimension weights are the learnable feature metric of the
+// adjoint (no counterpart in the reference).
+// Pinned by: scipy.spatial.distance.cdist on random inputs, with and without w (tests/test_oracle_pins.py).
 template <typename R>
-void cdist(int64_t n_a, int64_t n_b, int64_t dim, const double* a, const double* b, R* out) {
+void cdist(int64_t n_a, int64_t n_b, int64_t dim, const double* a, const double* b, R* out,
+           const double* w = nullptr) {
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n_b; ++i)
     for (int64_t j = 0; j < n_a; ++j) {
       R acc = 0;
       for (int64_t k = 0; k < dim; ++k) {
         R d = (R)a[j * dim + k] - (R)b[i * dim + k];
-        acc += d * d;
+        acc += w ? (R)w[k] * d * d : d * d;
       }
       out[j * n_b + i] = std::sqrt(acc);
     }
@@ -156,7 +159,7 @@ void row_max_all(int64_t n_row, int64_t n_col, const R* x, R* out) {
 // The driver's prep + loop + transpose (src/main.cu:80-247) on pattern-major buffers.
 template <typename R>
 int run(const or_graph* D, const or_graph* P, int n_iter, unsigned flags, double* mu_out, double* fmax_out,
-        double* rmax_out, double* elapsed_s) {
+        double* rmax_out, double* elapsed_s, const double* wv = nullptr, const double* we = nullptr) {
   const int64_t dn = D->nV, de = D->nE, pn = P->nV, pe = P->nE;
   if (D->dV != P->dV || D->dE != P->dE || dn < 1 || pn < 1) return 1;
   const bool literal = flags & OR_LITERAL_LSM;
@@ -168,8 +171,8 @@ int run(const or_graph* D, const or_graph* P, int n_iter, unsigned flags, double
   auto t0 = std::chrono::steady_clock::now();
 
   // prep (src/main.cu:107-148)
-  cdist<R>(pn, dn, D->dV, P->vfeat, D->vfeat, CV.data());
-  cdist<R>(pe, de, D->dE, P->efeat, D->efeat, CE.data());
+  cdist<R>(pn, dn, D->dV, P->vfeat, D->vfeat, CV.data(), wv);
+  cdist<R>(pe, de, D->dE, P->efeat, D->efeat, CE.data(), we);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < pn * dn; ++i) MU[i] = -CV[i];
   const int64_t neg_len = (flags & OR_COMPAT_ZERO_TAIL) ? std::min(pn * dn, pe * de) : pe * de;
@@ -259,9 +262,20 @@ int or_run_f32(const or_graph* D, const or_graph* P, int n_iter, unsigned flags,
   return run<float>(D, P, n_iter, flags, mu_out, fmax_out, rmax_out, elapsed_s);
 }
 
+// The whole matcher with per-dimension cdist weights wv (dV) and we (dE); either may be NULL (unweighted).
+int or_run_w_f64(const or_graph* D, const or_graph* P, int n_iter, unsigned flags, int threads, const double* wv,
+                 const double* we, double* mu_out) {
+  if (threads > 0) omp_set_num_threads(threads);
+  return run<double>(D, P, n_iter, flags, mu_out, nullptr, nullptr, nullptr, wv, we);
+}
+
 // Stage functions, FP64, for the pinning tests.
 void or_cdist(int64_t n_a, int64_t n_b, int64_t dim, const double* a, const double* b, double* out) {
   cdist<double>(n_a, n_b, dim, a, b, out);
+}
+void or_cdist_w(int64_t n_a, int64_t n_b, int64_t dim, const double* a, const double* b, const double* w,
+                double* out) {
+  cdist<double>(n_a, n_b, dim, a, b, out, w);
 }
 void or_row_log_softmax(int64_t n_row, int64_t n_col, double* x, int literal) {
   row_log_softmax<double>(n_row, n_col, x, literal != 0);

@@ -63,7 +63,10 @@ def lib():
             f = getattr(L, name)
             f.restype = ctypes.c_int
             f.argtypes = [P, P, ctypes.c_int, ctypes.c_uint, ctypes.c_int, P, P, P, P]
+        L.or_run_w_f64.restype = ctypes.c_int
+        L.or_run_w_f64.argtypes = [P, P, ctypes.c_int, ctypes.c_uint, ctypes.c_int, P, P, P]
         L.or_cdist.argtypes = [i64, i64, i64, P, P, P]
+        L.or_cdist_w.argtypes = [i64, i64, i64, P, P, P, P]
         L.or_row_log_softmax.argtypes = [i64, i64, P, ctypes.c_int]
         L.or_row_max.argtypes = [i64, i64, P, P]
         L.or_edge_max_reduce.argtypes = [i64, i64, i64, P, P, P, P]
@@ -127,6 +130,20 @@ def run(data: Graph64, patt: Graph64, n_iter: int = -1, flags: int = 0, threads:
     return mu
 
 
+def run_weighted(data: Graph64, patt: Graph64, wv=None, we=None, n_iter: int = -1, threads: int = 0):
+    """Whole matcher (FP64) with per-dimension cdist weights wv (vertex features) and we (edge features);
+    None = unweighted.  Returns MU, dn x pn."""
+    dn, pn = data.vfeat.shape[0], patt.vfeat.shape[0]
+    mu = np.empty((dn, pn), dtype=np.float64)
+    wv = None if wv is None else np.ascontiguousarray(wv, np.float64)
+    we = None if we is None else np.ascontiguousarray(we, np.float64)
+    D, Pg = data.c(), patt.c()
+    rc = lib().or_run_w_f64(ctypes.byref(D), ctypes.byref(Pg), n_iter, 0, threads, _ptr(wv), _ptr(we), mu.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle rejected the inputs (feature dims differ or empty graph)")
+    return mu
+
+
 def timed_run(data: Graph64, patt: Graph64, n_iter: int = -1, threads: int = 0):
     """Runs the restatement as the timed CPU baseline; returns (prep_seconds, loop_seconds)."""
     dn, pn = data.vfeat.shape[0], patt.vfeat.shape[0]
@@ -139,10 +156,15 @@ def timed_run(data: Graph64, patt: Graph64, n_iter: int = -1, threads: int = 0):
     return el[0], el[1], mu
 
 
-def cdist(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+def cdist(a: np.ndarray, b: np.ndarray, w=None) -> np.ndarray:
     a = np.ascontiguousarray(a, np.float64)
     b = np.ascontiguousarray(b, np.float64)
     out = np.empty((a.shape[0], b.shape[0]), np.float64)
+    if w is not None:
+        w = np.ascontiguousarray(w, np.float64)
+        lib().or_cdist_w(a.shape[0], b.shape[0], a.shape[1], a.ctypes.data, b.ctypes.data, w.ctypes.data,
+                         out.ctypes.data)
+        return out
     lib().or_cdist(a.shape[0], b.shape[0], a.shape[1], a.ctypes.data, b.ctypes.data, out.ctypes.data)
     return out
 

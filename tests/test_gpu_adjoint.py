@@ -1,0 +1,174 @@
+"""Adjoint (ac_set_weights / ac_backward) through the C ABI, checked against the FP64 oracle.
+
+Bar: the gradient of the planted cross-entropy  L = -mean_p MU[vmap[p]][p]  with respect to the per-dimension cdist
+weights agrees with FP64 central finite differences (h = 1e-4) of the oracle's loss to 1e-3 of the gradient's
+largest component; the weighted forward agrees with the weighted oracle to the plain path's bar
+(|a - b| <= 1e-8 + 1e-5 |b|); every checkpoint interval gives the same gradient.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from application_classification_amd import workloads as W
+from application_classification_amd.matcher import AC_OPT_TAPE, Matcher
+from application_classification_amd._lib import ACError, AC_ERR_STATE
+from oracle import oracle as O
+from tests.conftest import allclose_report
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-4
+GRAD_TOL = 1e-3
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def g64(g):
+    return O.Graph64.of(g.vfeat, g.efeat, g.src, g.dst)
+
+
+def loss_grad_mu(dn, vmap):
+    pn = len(vmap)
+    g = np.zeros((dn, pn), np.float32)
+    g[np.asarray(vmap), np.arange(pn)] = -1.0 / pn
+    return g
+
+
+def oracle_loss(d64, p64, vmap, wv, we, n_iter):
+    mu = O.run_weighted(d64, p64, wv, we, n_iter=n_iter)
+    return -float(np.mean(mu[np.asarray(vmap), np.arange(len(vmap))]))
+
+
+def fd_grad(d64, p64, vmap, wv, we, n_iter):
+    gv = np.zeros_like(wv)
+    ge = np.zeros_like(we)
+    for w, g in ((wv, gv), (we, ge)):
+        for k in range(w.size):
+            w[k] += H
+            lp = oracle_loss(d64, p64, vmap, wv, we, n_iter)
+            w[k] -= 2 * H
+            lm = oracle_loss(d64, p64, vmap, wv, we, n_iter)
+            w[k] += H
+            g[k] = (lp - lm) / (2 * H)
+    return gv, ge
+
+
+def gpu_grad(data, patt, wv, we, n_iter, K=4):
+    with Matcher(0) as m:
+        m.set_option(AC_OPT_TAPE, K)
+        m.set_data(data)
+        m.set_weights(wv, we)
+        m.set_pattern(patt)
+        mu = m.run(n_iter)
+        gv, ge = m.backward(loss_grad_mu(data.num_nodes, patt.vmap))
+        return mu, gv.astype(np.float64), ge.astype(np.float64)
+
+
+def check_grad(g, fd, what):
+    scale = max(np.abs(fd).max(), 1e-12)
+    err = np.abs(g - fd).max() / scale
+    say(f"{what}: |grad - fd|_inf / |fd|_inf = {err:.2e} (|fd|_inf = {scale:.3e})")
+    assert err <= GRAD_TOL, f"{what}: gradient off by {err:.2e} of its scale"
+
+
+@pytest.fixture(scope="module")
+def small():
+    return W.small(0)
+
+
+def weights(seed, dv, de):
+    r = np.random.default_rng(seed)
+    return r.uniform(0.5, 1.5, dv), r.uniform(0.5, 1.5, de)
+
+
+def test_weighted_forward_matches_oracle(small):
+    d, p = small.data, small.pattern
+    wv, we = weights(1, 13, 13)
+    with Matcher(0) as m:
+        m.set_data(d)
+        m.set_weights(wv, we)
+        m.set_pattern(p)
+        mu = m.run()
+        bad, ratio = allclose_report(mu, O.run_weighted(g64(d), g64(p), wv, we), 1e-5, 1e-8)
+        say(f"weighted forward: max err/tol {ratio:.4f}")
+        assert bad == 0
+        m.set_weights(None, None)  # back to the reference's cdist
+        np.testing.assert_array_equal(m.run(), Matcher_plain(d, p))
+
+
+def Matcher_plain(d, p):
+    with Matcher(0) as m:
+        m.set_data(d)
+        m.set_pattern(p)
+        return m.run()
+
+
+@pytest.mark.parametrize("n_iter", [0, 3, -1])
+def test_gradient_matches_finite_differences_small(small, n_iter):
+    d, p = small.data, small.pattern
+    wv, we = weights(2, 13, 13)
+    iters = p.num_nodes if n_iter < 0 else n_iter
+    _, gv, ge = gpu_grad(d, p, wv, we, iters)
+    fv, fe = fd_grad(g64(d), g64(p), p.vmap, wv.copy(), we.copy(), iters)
+    check_grad(np.concatenate([gv, ge]), np.concatenate([fv, fe]), f"small, {iters} iterations")
+
+
+def test_gradient_matches_finite_differences_rmat_subgraph():
+    """R-MAT scale 11 (2,048 vertices: hubs, split in-edge lists, vertices without edges), 32-dim features,
+    a planted 12-vertex / 16-edge pattern; all 64 weights."""
+    w = W.rmat(11, 32, 12, 16, seed=4)
+    d, p = w.data, w.pattern
+    wv, we = weights(3, 32, 32)
+    _, gv, ge = gpu_grad(d, p, wv, we, p.num_nodes)
+    fv, fe = fd_grad(g64(d), g64(p), p.vmap, wv.copy(), we.copy(), p.num_nodes)
+    check_grad(np.concatenate([gv, ge]), np.concatenate([fv, fe]), "rmat11 d32")
+
+
+@pytest.mark.parametrize("K", [1, 3, 100])
+def test_checkpoint_intervals_agree(small, K):
+    d, p = small.data, small.pattern
+    wv, we = weights(5, 13, 13)
+    mu4, gv4, ge4 = gpu_grad(d, p, wv, we, -1, K=4)
+    muk, gvk, gek = gpu_grad(d, p, wv, we, -1, K=K)
+    np.testing.assert_array_equal(mu4, muk)  # taping does not change the forward
+    g4, gk = np.concatenate([gv4, ge4]), np.concatenate([gvk, gek])
+    assert np.abs(g4 - gk).max() <= 1e-5 * np.abs(g4).max()
+
+
+def test_taped_forward_equals_plain_forward(small):
+    d, p = small.data, small.pattern
+    mu, _, _ = gpu_grad(d, p, np.ones(13), np.ones(13), -1)
+    np.testing.assert_array_equal(mu, Matcher_plain(d, p))
+
+
+def test_backward_requires_a_taped_run(small):
+    with Matcher(0) as m:
+        m.set_data(small.data)
+        m.set_pattern(small.pattern)
+        m.run()
+        with pytest.raises(ACError) as e:
+            m.backward(np.zeros((small.data.num_nodes, small.pattern.num_nodes), np.float32))
+        assert e.value.code == AC_ERR_STATE
+
+
+def test_autograd_function(small):
+    torch = pytest.importorskip("torch")
+    from application_classification_amd.autograd import match_log_probs, planted_cross_entropy
+
+    d, p = small.data, small.pattern
+    wv, we = weights(6, 13, 13)
+    with Matcher(0) as m:
+        m.set_option(AC_OPT_TAPE, 4)
+        m.set_data(d)
+        m.set_pattern(p)
+        tv = torch.tensor(wv, dtype=torch.float32, requires_grad=True)
+        te = torch.tensor(we, dtype=torch.float32, requires_grad=True)
+        loss = planted_cross_entropy(match_log_probs(m, tv, te), p.vmap)
+        loss.backward()
+    _, gv, ge = gpu_grad(d, p, wv.astype(np.float32), we.astype(np.float32), -1)
+    np.testing.assert_allclose(tv.grad.numpy(), gv, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(te.grad.numpy(), ge, rtol=1e-5, atol=1e-7)

@@ -28,6 +28,31 @@ def test_cdist_matches_scipy_and_orientation():
     assert out.shape == (7, 11)
 
 
+def test_weighted_cdist_matches_scipy():
+    """Per-dimension weights (the adjoint's feature metric): sqrt(sum_k w_k (a_k - b_k)^2) = scipy's weighted
+    euclidean; unit weights give the reference's cdist."""
+    a = rng.normal(size=(6, 9))
+    b = rng.normal(size=(13, 9))
+    w = rng.uniform(0.2, 3.0, size=9)
+    np.testing.assert_allclose(O.cdist(a, b, w), scipy_cdist(a, b, "euclidean", w=w), rtol=1e-14, atol=0)
+    np.testing.assert_array_equal(O.cdist(a, b, np.ones(9)), O.cdist(a, b))
+
+
+def test_weighted_run_reduces_to_plain_and_to_scaled_features():
+    """Unit weights give the plain matcher exactly; weights w give the matcher on features scaled by sqrt(w)."""
+    from application_classification_amd import workloads as W
+
+    w = W.small(3)
+    d, p = _g64(w.data), _g64(w.pattern)
+    np.testing.assert_array_equal(O.run_weighted(d, p, np.ones(13), np.ones(13), n_iter=4), O.run(d, p, n_iter=4))
+    wv, we = rng.uniform(0.5, 2.0, 13), rng.uniform(0.5, 2.0, 13)
+    sv, se = np.sqrt(wv), np.sqrt(we)
+    d2 = O.Graph64.of(d.vfeat * sv, d.efeat * se, d.src, d.dst)
+    p2 = O.Graph64.of(p.vfeat * sv, p.efeat * se, p.src, p.dst)
+    np.testing.assert_allclose(O.run_weighted(d, p, wv, we, n_iter=4), O.run(d2, p2, n_iter=4), rtol=1e-9,
+                               atol=1e-12)
+
+
 def test_row_log_softmax_matches_scipy_and_invariants():
     x = rng.normal(scale=30, size=(4, 257))
     y = O.row_log_softmax(x)
