@@ -79,24 +79,40 @@ __global__ __launch_bounds__(256) void k_colsum(int64_t n, int C, const float* _
   }
 }
 
-// out[c] = sum over parts of part[b][c], in block order.
-__global__ void k_sum_parts(int nparts, int C, const double* __restrict__ part, double* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
+// out[c] = sum over parts of part[b][c]: a block takes CB = min(C, 32) columns and 256 / CB lanes per column, lane l
+// summing parts l, l + L, ... in order, the lanes then summed in order (deterministic).
+__global__ __launch_bounds__(256) void k_sum_parts(int nparts, int C, const double* __restrict__ part,
+                                                   double* __restrict__ out) {
+  __shared__ double red[256];
+  const int CB = C < 32 ? C : 32, L = 256 / CB, t = threadIdx.x;
+  const int c = blockIdx.x * CB + t % CB, l = t / CB;
   double a = 0.0;
-  for (int b = 0; b < nparts; ++b) a += part[(int64_t)b * C + c];
-  out[c] = a;
+  if (l < L && c < C)
+    for (int b = l; b < nparts; b += L) a += part[(int64_t)b * C + c];
+  red[t] = a;
+  __syncthreads();
+  if (t < CB && c < C) {
+    double s = 0.0;
+    for (int k = 0; k < L; ++k) s += red[k * CB + t];
+    out[c] = s;
+  }
 }
 
-// Log-softmax Jacobian of MU = LSM_v(Z) at state S: gMU <- gZ = gMU - exp(MU) colsum(gMU); gCV -= gZ.
-__global__ __launch_bounds__(256) void k_bwd_out(BwdGraph G, BwdState S, const double* __restrict__ colsum,
-                                                 float* __restrict__ gMU, float* __restrict__ gCV) {
+// MU[v][p] of a state, for every (v, p), into MU[n][P] (FP64).
+__global__ __launch_bounds__(256) void k_mu_state(BwdGraph G, BwdState S, double* __restrict__ MU) {
   const int64_t total = G.n * G.P;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = i / G.P;
-    const int p = (int)(i - v * G.P);
-    const double mu = mu_of(G, S, v, p);
-    const float gz = gMU[i] - __expf((float)mu) * (float)colsum[p];
+    MU[i] = mu_of(G, S, v, (int)(i - v * G.P));
+  }
+}
+
+// Log-softmax Jacobian of MU = LSM_v(Z): gMU <- gZ = gMU - exp(MU) colsum(gMU); gCV -= gZ.
+__global__ __launch_bounds__(256) void k_bwd_out(int64_t total, int P, const double* __restrict__ MU,
+                                                 const double* __restrict__ colsum, float* __restrict__ gMU,
+                                                 float* __restrict__ gCV) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gz = gMU[i] - __expf((float)MU[i]) * (float)colsum[i % P];
     gMU[i] = gz;
     gCV[i] -= gz;
   }
@@ -149,18 +165,18 @@ __global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const
 //   gVR = -S_F exp(VR - LSE_FE - minCEo + lsO) + [v = argmax VR] gfillF
 //   gVF = gE_R - S_R exp(VF - LSE_RE - minCEo + lsO) + [v = argmax VF] gfillR
 //   cD += S_F exp(VR - LSE_FE) + S_R exp(VF - LSE_RE)            (vertices with out-edges; FP64)
-__global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const QStat* __restrict__ qs1,
-                                                const double* __restrict__ red4, const float* __restrict__ gER,
-                                                float* __restrict__ gVR, float* __restrict__ gVF,
-                                                double* __restrict__ cD) {
+__global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const double* __restrict__ MU,
+                                                const QStat* __restrict__ qs1, const double* __restrict__ red4,
+                                                const float* __restrict__ gER, float* __restrict__ gVR,
+                                                float* __restrict__ gVF, double* __restrict__ cD) {
   const int Q = G.Q;
   const int64_t total = G.n * Q;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = i / Q;
     const int q = (int)(i - v * Q);
     const int64_t o = v * G.Qs + q;
-    const double VF = mu_of(G, S, v, G.pd[q]) - fmax_of(G, S, v, q);
-    const double VR = mu_of(G, S, v, G.ps[q]) - fmax(S.qs[q].fillR, rn_of(G, S, v, q));
+    const double VF = MU[v * G.P + G.pd[q]] - fmax_of(G, S, v, q);
+    const double VR = MU[v * G.P + G.ps[q]] - fmax(S.qs[q].fillR, rn_of(G, S, v, q));
     const QStat s1 = qs1[q];
     const double SF = red4[q], SR = red4[Q + q];
     const float mo = G.minCEo[o];
@@ -337,9 +353,10 @@ __global__ void k_bwd_vertex_coef(int64_t total, const float* __restrict__ CV, f
 }
 
 // Outer-product sums over rows: for q < K, k < dim,  A[q][k] = sum_r G[r][q] x_r[k]^2,  B[q][k] = sum_r G[r][q] x_r[k],
-// C[q] = sum_r G[r][q], x_r = feats[idx ? idx[r] : r] (rows with idx < 0 skipped).  A tile of 64 rows is staged in
-// LDS; each thread accumulates a 4 x 4 block of (q, k) in FP32 over the tile and in FP64 across tiles.  Per-block
-// partials part[b][2 K dim + K].
+// C[q] = sum_r G[r][q], x_r = feats[idx ? idx[r] : r] (rows with idx < 0 skipped).  Tiles of 64 rows are staged in
+// LDS, one 64 x 64 block of (q, k) at a time; the block's (q, k) pairs are covered by NT micro-tiles of 4 x 4 and the
+// 256 threads form 256 / NT row groups, each accumulating its own rows of the tile (FP32 within a tile, FP64 across
+// tiles), summed over groups in order at the end.  Per-block partials part[b][2 K dim + K].
 constexpr int kOuterRows = 64;
 
 __global__ __launch_bounds__(256) void k_outer_acc(int64_t nrows, const int* __restrict__ idx,
@@ -347,26 +364,43 @@ __global__ __launch_bounds__(256) void k_outer_acc(int64_t nrows, const int* __r
                                                    int Gs, int K, double* __restrict__ part) {
   __shared__ float gs[kOuterRows][65];
   __shared__ float xs[kOuterRows][65];
-  const int t = threadIdx.x, tq = t / 16, tk = t % 16;
+  __shared__ double acc[256][4];
+  const int t = threadIdx.x;
+  for (int e = t; e < kOuterRows * 65; e += 256) {  // columns past the valid ones stay finite
+    gs[e / 65][e % 65] = 0.f;
+    xs[e / 65][e % 65] = 0.f;
+  }
   const int nqb = (K + 63) / 64, nkb = (dim + 63) / 64;
   double* out = part + (int64_t)blockIdx.x * (2 * K * dim + K);
   for (int qb = 0; qb < nqb; ++qb)
     for (int kb = 0; kb < nkb; ++kb) {
+      const int kq = min(64, K - qb * 64), kk = min(64, dim - kb * 64);
+      const int nq4 = (kq + 3) / 4, nk4 = (kk + 3) / 4;
+      const int NT = nq4 * nk4, NG = 256 / NT;
+      const bool active = t < NG * NT;
+      const int mt = active ? t % NT : 0, rg = active ? t / NT : 0;
+      const int tq = mt / nk4, tk = mt % nk4;
       double A[4][4] = {}, B[4][4] = {}, C[4] = {};
       for (int64_t r0 = (int64_t)blockIdx.x * kOuterRows; r0 < nrows; r0 += (int64_t)gridDim.x * kOuterRows) {
         __syncthreads();
-        for (int e = t; e < kOuterRows * 64; e += 256) {
-          const int r = e / 64, c = e % 64;
+        // Only the block's valid columns are loaded; columns past them feed (q, k) pairs that are discarded.
+        for (int e = t; e < kOuterRows * kq; e += 256) {
+          const int r = e / kq, c = e - r * kq;
+          const int64_t row = r0 + r;
+          const bool live = row < nrows && (!idx || idx[row] >= 0);
+          gs[r][c] = live ? G[row * Gs + qb * 64 + c] : 0.f;
+        }
+        for (int e = t; e < kOuterRows * kk; e += 256) {
+          const int r = e / kk, c = e - r * kk;
           const int64_t row = r0 + r;
           int64_t src = -1;
           if (row < nrows) src = idx ? (int64_t)idx[row] : row;
-          const int q = qb * 64 + c, k = kb * 64 + c;
-          gs[r][c] = (src >= 0 && q < K) ? G[row * Gs + q] : 0.f;
-          xs[r][c] = (src >= 0 && k < dim) ? feats[src * dim + k] : 0.f;
+          xs[r][c] = src >= 0 ? feats[src * dim + kb * 64 + c] : 0.f;
         }
         __syncthreads();
+        if (!active) continue;
         float a[4][4] = {}, b[4][4] = {}, cc[4] = {};
-        for (int r = 0; r < kOuterRows; ++r) {
+        for (int r = rg; r < kOuterRows; r += NG) {
           float gv[4], xv[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
@@ -394,18 +428,34 @@ __global__ __launch_bounds__(256) void k_outer_acc(int64_t nrows, const int* __r
           }
         }
       }
-#pragma unroll
+      // Sum the row groups in order, four values of the micro-tile at a time.
       for (int u = 0; u < 4; ++u) {
-        const int q = qb * 64 + tq * 4 + u;
-        if (q >= K) continue;
+        for (int which = 0; which < 2; ++which) {
+          __syncthreads();
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
-          const int k = kb * 64 + tk * 4 + w;
-          if (k >= dim) continue;
-          out[(int64_t)q * dim + k] = A[u][w];
-          out[(int64_t)K * dim + (int64_t)q * dim + k] = B[u][w];
+          for (int w = 0; w < 4; ++w) acc[t][w] = which ? B[u][w] : A[u][w];
+          __syncthreads();
+          if (active && rg == 0) {
+            const int q = qb * 64 + tq * 4 + u;
+            for (int w = 0; w < 4; ++w) {
+              double s2 = 0.0;
+              for (int g2 = 0; g2 < NG; ++g2) s2 += acc[g2 * NT + mt][w];
+              const int k = kb * 64 + tk * 4 + w;
+              if (q < K && k < dim) out[(which ? (int64_t)K * dim : 0) + (int64_t)q * dim + k] = s2;
+            }
+          }
         }
-        if (kb == 0 && tk == 0) out[2 * (int64_t)K * dim + q] = C[u];
+        if (kb == 0) {
+          __syncthreads();
+          acc[t][0] = C[u];
+          __syncthreads();
+          if (active && rg == 0 && tk == 0) {
+            const int q = qb * 64 + tq * 4 + u;
+            double s2 = 0.0;
+            for (int g2 = 0; g2 < NG; ++g2) s2 += acc[g2 * NT + mt][0];
+            if (q < K) out[2 * (int64_t)K * dim + q] = s2;
+          }
+        }
       }
     }
 }
@@ -425,32 +475,41 @@ void launch_fill_int(int* p, int64_t count, int value, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_int, dim3(grid_for(count)), dim3(256), 0, st, p, count, value);
 }
 
+static int sum_blocks(int C) {
+  const int CB = C < 32 ? C : 32;
+  return (C + CB - 1) / CB;
+}
+
 void launch_sum_parts(int nparts, int C, const double* part, double* out, hipStream_t st) {
   if (C == 0) return;
-  hipLaunchKernelGGL(k_sum_parts, dim3((C + 255) / 256), dim3(256), 0, st, nparts, C, part, out);
+  hipLaunchKernelGGL(k_sum_parts, dim3(sum_blocks(C)), dim3(256), 0, st, nparts, C, part, out);
 }
 
 void launch_colsum(int64_t n, int C, const float* X, double* part, int nparts, double* out, hipStream_t st) {
   if (C == 0) return;
   hipLaunchKernelGGL(k_colsum, dim3(nparts), dim3(256), 0, st, n, C, X, part);
-  hipLaunchKernelGGL(k_sum_parts, dim3((C + 255) / 256), dim3(256), 0, st, nparts, C, part, out);
+  hipLaunchKernelGGL(k_sum_parts, dim3(sum_blocks(C)), dim3(256), 0, st, nparts, C, part, out);
 }
 
-void launch_bwd_out(const BwdGraph& G, const BwdState& S, const double* colsum, float* gMU, float* gCV, hipStream_t st) {
-  hipLaunchKernelGGL(k_bwd_out, dim3(grid_for(G.n * G.P)), dim3(256), 0, st, G, S, colsum, gMU, gCV);
+void launch_mu_state(const BwdGraph& G, const BwdState& S, double* MU, hipStream_t st) {
+  hipLaunchKernelGGL(k_mu_state, dim3(grid_for(G.n * G.P)), dim3(256), 0, st, G, S, MU);
+}
+
+void launch_bwd_out(const BwdGraph& G, const double* MU, const double* colsum, float* gMU, float* gCV, hipStream_t st) {
+  hipLaunchKernelGGL(k_bwd_out, dim3(grid_for(G.n * G.P)), dim3(256), 0, st, G.n * G.P, G.P, MU, colsum, gMU, gCV);
 }
 
 void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gFX,
                       const float* gRX, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st) {
   if (G.Q == 0) return;
   hipLaunchKernelGGL(k_bwd_route, dim3(nparts), dim3(256), 0, st, G, S, gZ, with_gz, gFX, gRX, gEF, gER, part);
-  hipLaunchKernelGGL(k_sum_parts, dim3((4 * G.Q + 255) / 256), dim3(256), 0, st, nparts, 4 * G.Q, part, red4);
+  hipLaunchKernelGGL(k_sum_parts, dim3(sum_blocks(4 * G.Q)), dim3(256), 0, st, nparts, 4 * G.Q, part, red4);
 }
 
-void launch_bwd_in(const BwdGraph& G, const BwdState& S, const QStat* qs1, const double* red4, const float* gER,
-                   float* gVR, float* gVF, double* cD, hipStream_t st) {
+void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const QStat* qs1, const double* red4,
+                   const float* gER, float* gVR, float* gVF, double* cD, hipStream_t st) {
   if (G.Q == 0) return;
-  hipLaunchKernelGGL(k_bwd_in, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, S, qs1, red4, gER, gVR, gVF, cD);
+  hipLaunchKernelGGL(k_bwd_in, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, S, MU, qs1, red4, gER, gVR, gVF, cD);
 }
 
 void launch_bwd_scatter(const BwdGraph& G, const float* gEF, const float* gER, const int* FA, const int* AO,

@@ -34,11 +34,12 @@ struct BwdState {
 void launch_fill_int(int* p, int64_t count, int value, hipStream_t st);
 void launch_sum_parts(int nparts, int C, const double* part, double* out, hipStream_t st);
 void launch_colsum(int64_t n, int C, const float* X, double* part, int nparts, double* out, hipStream_t st);
-void launch_bwd_out(const BwdGraph& G, const BwdState& S, const double* colsum, float* gMU, float* gCV, hipStream_t st);
+void launch_mu_state(const BwdGraph& G, const BwdState& S, double* MU, hipStream_t st);
+void launch_bwd_out(const BwdGraph& G, const double* MU, const double* colsum, float* gMU, float* gCV, hipStream_t st);
 void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gFX,
                       const float* gRX, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st);
-void launch_bwd_in(const BwdGraph& G, const BwdState& S, const QStat* qs1, const double* red4, const float* gER,
-                   float* gVR, float* gVF, double* cD, hipStream_t st);
+void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const QStat* qs1, const double* red4,
+                   const float* gER, float* gVR, float* gVF, double* cD, hipStream_t st);
 void launch_bwd_scatter(const BwdGraph& G, const float* gEF, const float* gER, const int* FA, const int* AO,
                         const int* bsrc, float* gVR, float* gCE, hipStream_t st);
 void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, float* gFX, float* gRX,

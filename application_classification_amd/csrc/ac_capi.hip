@@ -193,7 +193,7 @@ struct ac_ctx {
   DevBuf<Lse2> ckP, sgP;
   // Adjoint work buffers.
   DevBuf<float> gMU, gCV, gFX, gRX, gEF, gER, gVR, gVF, gCE, apv;
-  DevBuf<double> cD, colsum, bpart, red4, cq, opart, osum;
+  DevBuf<double> cD, colsum, bpart, red4, cq, opart, osum, MUd;
   DevBuf<int> api;
 
   // Batched scoring: the patterns (host copies, concatenated), where each one's result lives in `bout`, and the
@@ -646,7 +646,7 @@ static int enqueue_prep(ac_ctx* ctx) {
     }
   }
   if (ctx->tape_K > 0) {
-    ac::launch_col_argmin(n, P, ctx->CV.p, ctx->apv.p, ctx->api.p, ctx->nb_col, ctx->am0.p, st);
+    ac::launch_col_argmin(n, P, ctx->CV.p, ctx->apv.p, ctx->api.p, std::min(ctx->nb_col, 128), ctx->am0.p, st);
     int rc = checkpoint(ctx, 0, qs_prep);
     if (rc != AC_OK) return rc;
   }
@@ -1281,13 +1281,14 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs, K = ctx->tape_K, n_iter = ctx->tape_iters;
   const int Q1 = std::max(1, Q);
   const size_t nq = (size_t)n * Qs, np = (size_t)n * P;
-  const int nparts = ctx->nb_col;
+  const int nparts = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 512));
   const int64_t brows = ctx->bs->brows;
   HIPCHK(ctx, ctx->gMU.upload(grad_mu, np, st));
   HIPCHK(ctx, ctx->gCV.alloc(np));
   for (DevBuf<float>* b : {&ctx->gFX, &ctx->gRX, &ctx->gEF, &ctx->gER, &ctx->gVR, &ctx->gVF}) HIPCHK(ctx, b->alloc(nq));
   HIPCHK(ctx, ctx->gCE.alloc((size_t)std::max<int64_t>(1, brows) * Qs));
   HIPCHK(ctx, ctx->cD.alloc(nq));
+  HIPCHK(ctx, ctx->MUd.alloc(np));
   HIPCHK(ctx, ctx->colsum.alloc((size_t)P));
   HIPCHK(ctx, ctx->bpart.alloc((size_t)nparts * std::max(4 * Q1, P)));
   HIPCHK(ctx, ctx->red4.alloc((size_t)4 * Q1));
@@ -1309,6 +1310,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
   };
   const int nck = n_iter == 0 ? 0 : (n_iter + K - 1) / K;
+  bool have_mu = false;  // MUd holds MU of the output state of the next iteration to reverse
   for (int c = nck - 1; c >= 0; --c) {
     const int i0 = c * K, i1 = std::min(i0 + K, n_iter);
     // Recompute the segment from its checkpoint, keeping every state and the walk's argmax rows.
@@ -1333,13 +1335,16 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
       const ac::BwdState S0{ctx->sgF.p + s * nq, ctx->sgV.p + s * nq, ctx->sgQ.p + s * Q1, ctx->sgP.p + s * P, i == 0};
       const ac::BwdState S1{ctx->sgF.p + (s + 1) * nq, ctx->sgV.p + (s + 1) * nq, ctx->sgQ.p + (s + 1) * Q1,
                             ctx->sgP.p + (s + 1) * P, 0};
+      if (!have_mu) ac::launch_mu_state(G, S1, ctx->MUd.p, st);
       ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
-      ac::launch_bwd_out(G, S1, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
+      ac::launch_bwd_out(G, ctx->MUd.p, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
       ac::launch_bwd_route(G, S1, ctx->gMU.p, 1, ctx->gFX.p, ctx->gRX.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
                            ctx->red4.p, st);
+      ac::launch_mu_state(G, S0, ctx->MUd.p, st);  // MU of the input state, reused by the next step back
+      have_mu = true;
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVR.p, 0, nq * sizeof(float), st));
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
-      ac::launch_bwd_in(G, S0, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, st);
+      ac::launch_bwd_in(G, S0, ctx->MUd.p, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, st);
       ac::launch_bwd_scatter(G, ctx->gEF.p, ctx->gER.p, ctx->tapeFA.p + s * nq, ctx->AO.p, ctx->bs->bsrc.p,
                              ctx->gVR.p, ctx->gCE.p, st);
       ac::launch_bwd_mu(G, ctx->gVR.p, ctx->gVF.p, ctx->gMU.p, ctx->gFX.p, ctx->gRX.p, st);
@@ -1365,13 +1370,14 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
     }
   }
   HIPCHK(ctx, hipMemcpyAsync(ctx->cq.p, cq.data(), Q1 * sizeof(double), hipMemcpyHostToDevice, st));
+  if (!have_mu) ac::launch_mu_state(G, S0, ctx->MUd.p, st);
   ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
-  ac::launch_bwd_out(G, S0, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
+  ac::launch_bwd_out(G, ctx->MUd.p, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
 
   // dL/dw: gC / (2 C) against the squared feature differences, as outer-product sums over rows.
   ac::launch_bwd_vertex_coef((int64_t)np, ctx->CV.p, ctx->gCV.p, st);
   const int dV = (int)ctx->dV, dE = (int)ctx->dE;
-  const int onp = 256;
+  const int onp = 512;
   std::vector<float> gwv(dV, 0.f), gwe(dE, 0.f);
   if (dV > 0) {
     const size_t w = (size_t)2 * P * dV + P;

@@ -151,6 +151,74 @@ def measured_traffic(workload, stage):
     return None
 
 
+def bench_diff(args):
+    """Differentiation: R-MAT scale 18 (32-dim) with 64 planted 8-16-vertex patterns; per pattern the forward
+    (prep + iterations + output) alone, then forward with the tape (checkpoint every 4 iterations) + ac_backward of
+    the planted cross-entropy.  A step is one pass over the 64 patterns."""
+    from application_classification_amd import workloads as W
+    from application_classification_amd.matcher import AC_OPT_TAPE, Matcher
+
+    t = time.time()
+    w = W.rmat(18, 32, 12, 16, seed=0)
+    g = w.data
+    pats = W.uneven_patterns(g, 64, seed=7, v_range=(8, 16))
+    upd = sum(g.num_edges * p.num_edges * p.num_nodes for p in pats)
+    log(f"diff workload: dn={g.num_nodes} de={g.num_edges}, {len(pats)} patterns, {upd:.3e} edge-pair updates, "
+        f"generated in {time.time() - t:.1f} s")
+    rng = np.random.default_rng(0)
+    wv = rng.uniform(0.5, 1.5, 32).astype(np.float32)
+    we = rng.uniform(0.5, 1.5, 32).astype(np.float32)
+    grads = [np.zeros((len(pats), 32)), np.zeros((len(pats), 32))]
+
+    def fwd_pass(m, tape):
+        t0 = time.perf_counter()
+        for i, p in enumerate(pats):
+            m.set_pattern(p)
+            mu = m.run()
+            if tape:
+                gm = np.zeros_like(mu)
+                gm[p.vmap, np.arange(p.num_nodes)] = -1.0 / p.num_nodes
+                gv, ge = m.backward(gm)
+                grads[0][i], grads[1][i] = gv, ge
+        return (time.perf_counter() - t0) * 1e3
+
+    times = {}
+    for tape in (False, True):
+        m = Matcher(0)
+        m.set_option(AC_OPT_TAPE, 4 if tape else 0)
+        m.set_data(g)
+        m.set_weights(wv, we)
+        for _ in range(args.warmup):
+            fwd_pass(m, tape)
+        times[tape] = min(fwd_pass(m, tape) for _ in range(args.steps))
+        m.close()
+        log(f"{'forward+backward' if tape else 'forward'}: {times[tape]:.1f} ms per pass over {len(pats)} patterns")
+    out = {
+        "metric": "forward+backward time / forward time (64 patterns, R-MAT 18, 32-dim, checkpoint every 4)",
+        "value": times[True] / times[False],
+        "unit": "ratio",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": times[True],
+        "forward_ms_per_step": times[False],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "storage_dtype": "f32",
+        "gradient_dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "rmat18-d32-64-patterns-8to16", "data_vertices": g.num_nodes,
+                   "data_edges": g.num_edges, "patterns": len(pats), "checkpoint_every": 4,
+                   "updates_per_pass": upd, "parallelism": "single-gpu"},
+        "checks": {"grad_wv_norm": float(np.linalg.norm(grads[0].sum(0))),
+                   "grad_we_norm": float(np.linalg.norm(grads[1].sum(0))),
+                   "all_finite": bool(np.isfinite(grads[0]).all() and np.isfinite(grads[1]).all())},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_batched(args):
     """256 uneven planted patterns (V in [4, 32], E in [V, 2V]) against one resident R-MAT scale-20 graph (32-dim):
     the whole job (prep + iterations + output of every pattern) as 256 sequential plain runs, then as batched runs
@@ -254,7 +322,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="s22", choices=["s22", "s18", "small", "batched"])
+    ap.add_argument("--config", default="s22", choices=["s22", "s18", "small", "batched", "diff"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--summarize-profile", metavar="DIR",
@@ -288,6 +356,9 @@ def main():
 
     if args.config == "batched":
         bench_batched(args)
+        return
+    if args.config == "diff":
+        bench_diff(args)
         return
 
     from application_classification_amd import workloads as W
