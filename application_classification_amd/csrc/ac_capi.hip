@@ -171,6 +171,8 @@ struct ac_ctx {
   BundleSet* bs = nullptr;
   int walk_B = 0;          // steps per batch of the edge walk (0: default)
   bool use_graph = false;  // replay iteration pairs from a captured graph
+  bool prune = true;       // skip bundles whose edges cannot raise FMax above the fill (exact)
+  DevBuf<float> bmin;      // [bundles][Qs] least cost per bundle and pattern edge
   DevBuf<double> carry_in, carry_out;
   DevBuf<QPart> qpart;
   DevBuf<Lse> ppart, lpart;
@@ -633,6 +635,8 @@ static int enqueue_prep(ac_ctx* ctx) {
       HIPCHK(ctx, hipStreamSynchronize(st));
     }
     ac::launch_prep_scalars(P, Q, ctx->d_ps.p, ctx->d_pd.p, ctx->lse_cv.p, ctx->lse_ce.p, qs_prep, st);
+    HIPCHK(ctx, ctx->bmin.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles) * Qs));
+    ac::launch_bundle_min(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Qs, ctx->bmin.p, st);
     const bool tape = ctx->tape_K > 0;
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, nullptr, ctx->VRs, Q, Qs, qs_prep, ctx->FMaxN.p, ctx->carry_in.p,
@@ -696,7 +700,7 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
     MARK(AC_STAGE_REDUCE_A);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p,
-                        walk_batch_of(ctx), st);
+                        walk_batch_of(ctx), st, nullptr, nullptr, ctx->prune ? ctx->bmin.p : nullptr);
     MARK(AC_STAGE_EDGE_WALK);
     ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_new, ctx->FMaxN.p, st);
     MARK(AC_STAGE_WALK_FIXUP);
@@ -953,6 +957,9 @@ extern "C" int ac_set_option(ac_ctx* ctx, int option, int value) {
       ctx->tape_K = value;
       ctx->tape_valid = false;
       return AC_OK;
+    case AC_OPT_PRUNE:
+      ctx->prune = value != 0;
+      return AC_OK;
     default:
       return ctx->fail(AC_ERR_ARG, "ac_set_option: unknown option " + std::to_string(option));
   }
@@ -972,6 +979,9 @@ extern "C" int ac_get_option(const ac_ctx* ctx, int option, int* value) {
       return AC_OK;
     case AC_OPT_TAPE:
       *value = ctx->tape_K;
+      return AC_OK;
+    case AC_OPT_PRUNE:
+      *value = ctx->prune ? 1 : 0;
       return AC_OK;
     default:
       return AC_ERR_ARG;
@@ -1244,7 +1254,7 @@ static void recompute_iteration(ac_ctx* ctx, int it, const QStat* qs_in, QStat* 
     ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_out, st);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_out, ctx->FMaxN.p, ctx->carry_in.p,
-                        walk_batch_of(ctx), st, FA, ctx->carg_in.p);
+                        walk_batch_of(ctx), st, FA, ctx->carg_in.p, ctx->prune ? ctx->bmin.p : nullptr);
     ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_out, ctx->FMaxN.p, st,
                               ctx->carg_in.p, FA);
   }

@@ -34,6 +34,7 @@ struct QStat {
   double mRE, lRE;  // log sum_e exp(RE[q][e])
   double mFE, lFE;  // log sum_e exp(FE[q][e])
   int vF, vR;       // the data vertex at which fillF / fillR is attained (lowest on ties; -1 in the prep state)
+  double cthr;      // an edge whose cost is >= cthr cannot raise FMax above fillF (+inf: no bound)
 };
 // Per-block partial of QStat.
 struct QPart {
@@ -91,6 +92,7 @@ void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, co
 void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks, hipStream_t st);
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st);
 void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st);
+void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE, int Qs, float* bmin, hipStream_t st);
 void launch_prep_scalars(int P, int Q, const int* ps, const int* pd, const Lse2* cv, const Lse2* ce, QStat* qs,
                          hipStream_t st);
 void launch_fill(float* p, int64_t count, float value, hipStream_t st);
@@ -103,7 +105,8 @@ void launch_build_bundles(int64_t nbundles, const Bundle* bundles, const int* br
 void launch_remap_ks(int64_t m, const int2* ks, const int* rowmap, int2* out, hipStream_t st);
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
-                     double* carry, int B, hipStream_t st, int* FA = nullptr, int* carg = nullptr);
+                     double* carry, int B, hipStream_t st, int* FA = nullptr, int* carg = nullptr,
+                     const float* bmin = nullptr);
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
                            float* FMaxN, hipStream_t st, const int* carg = nullptr, int* FA = nullptr);
 void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st);

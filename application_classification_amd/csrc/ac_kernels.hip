@@ -249,7 +249,14 @@ __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart
   }
   if (t == 0) {
     const QPart& r = red[0];
-    out[q] = QStat{r.vrmax, r.vfmax, r.re.m, log1p(r.re.t), r.fe.m, log1p(r.fe.t), r.avr, r.avf};
+    // Edge pruning bound.  An edge term is (VR[src] - CE) - LSE_FE with VR[src] <= fillF (rounded to FP32 once),
+    // so it cannot exceed fillF when CE + LSE_FE >= 2^-22 (|fillF| + |CE| + |LSE_FE|); cthr = -LSE_FE plus four
+    // times that margin (a cost near the bound has |CE| ~ |LSE_FE|; a larger one clears it by itself).
+    const double fF = r.vrmax, sl = r.fe.m + log1p(r.fe.t);
+    double cthr = kInf;
+    if (fF > -kInf && fF < kInf && sl > -kInf && sl < kInf)
+      cthr = -sl + 0x1p-20 * (fabs(fF) + 2.0 * fabs(sl) + 1.0);
+    out[q] = QStat{r.vrmax, r.vfmax, r.re.m, log1p(r.re.t), r.fe.m, log1p(r.fe.t), r.avr, r.avf, cthr};
   }
 }
 
@@ -264,7 +271,7 @@ __global__ void k_prep_scalars(int Q, const int* __restrict__ ps, const int* __r
                                const Lse2* __restrict__ cv, const Lse2* __restrict__ ce, QStat* __restrict__ qs) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= Q) return;
-  qs[q] = QStat{-cv[ps[q]].l, -cv[pd[q]].l, ce[q].m, ce[q].l, ce[q].m, ce[q].l, -1, -1};
+  qs[q] = QStat{-cv[ps[q]].l, -cv[pd[q]].l, ce[q].m, ce[q].l, ce[q].m, ce[q].l, -1, -1, kInf};
 }
 
 void launch_prep_scalars(int P, int Q, const int* ps, const int* pd, const Lse2* cv, const Lse2* ce, QStat* qs,
@@ -448,7 +455,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
                                                          const float* __restrict__ VR, int VRs, int Q, int Qs,
                                                          const QStat* __restrict__ qs, float* __restrict__ FMaxN,
                                                          double* __restrict__ carry, int* __restrict__ FA,
-                                                         int* __restrict__ carg) {
+                                                         int* __restrict__ carg, const float* __restrict__ bmin) {
   const int L4 = Qs / 4, G = 64 / L4, VR4s = VRs / 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane / L4, j = lane - g * L4;
@@ -456,6 +463,28 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   if (g >= G || b >= nbundles) return;
   const Bundle Bd = bundles[b];
   const int slot = gslot[b * G + g];
+  if (bmin) {
+    // Exact pruning: when no edge of the bundle has a cost below its pattern edge's bound, none of its terms can
+    // raise FMax above the fill; the bundle's vertices get -inf (the fill then wins, as it would have).
+    bool live = false;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = 4 * j + u;
+      if (q < Q && (double)bmin[b * Qs + q] < qs[q].cthr) live = true;
+    }
+    if (!__any(live)) {
+      if (slot >= 0) {
+        double* o = carry + (int64_t)slot * Qs + 4 * j;
+        o[0] = o[1] = o[2] = o[3] = -kInf;
+      } else {
+        const int v0 = bdst[Bd.base + g], v1 = bdst[Bd.base + (int64_t)(Bd.len - 1) * G + g];
+        const float4 ninf = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
+        if (v0 >= 0)
+          for (int v = v0; v <= v1; ++v) reinterpret_cast<float4*>(FMaxN)[(int64_t)v * L4 + j] = ninf;
+      }
+      return;
+    }
+  }
   // log sum_e exp(FE[q][e]) as one FP64 value (its FP64 rounding is far below the FP32 rounding of the result).
   double sl[4];
 #pragma unroll
@@ -559,40 +588,77 @@ template <bool kWithVR, bool kArg>
 static void launch_walk_max_t(int B, int nb, hipStream_t st, int64_t nbundles, const Bundle* bundles,
                               const int* gslot, const int* bsrc, const int* bdst, const float* CE, const float* VR,
                               int VRs, int Q, int Qs, const QStat* qs, float* FMaxN, double* carry, int* FA,
-                              int* carg) {
+                              int* carg, const float* bmin) {
   switch (B) {
     case 1:
       hipLaunchKernelGGL((k_walk_max<kWithVR, 1, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
-                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg);
+                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin);
       break;
     case 4:
       hipLaunchKernelGGL((k_walk_max<kWithVR, 4, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
-                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg);
+                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin);
       break;
     default:
       hipLaunchKernelGGL((k_walk_max<kWithVR, 2, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
-                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg);
+                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin);
       break;
   }
 }
 
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
-                     double* carry, int B, hipStream_t st, int* FA, int* carg) {
+                     double* carry, int B, hipStream_t st, int* FA, int* carg, const float* bmin) {
   if (nbundles == 0) return;
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
   if (VR && FA)
     launch_walk_max_t<true, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                  carry, FA, carg);
+                                  carry, FA, carg, bmin);
   else if (VR)
     launch_walk_max_t<true, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                   carry, FA, carg);
+                                   carry, FA, carg, bmin);
   else if (FA)
     launch_walk_max_t<false, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                   carry, FA, carg);
+                                   carry, FA, carg, bmin);
   else
     launch_walk_max_t<false, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                    carry, FA, carg);
+                                    carry, FA, carg, bmin);
+}
+
+// Per bundle and pattern edge, the least cost among the bundle's rows (padding rows are +inf): bmin[b][Qs].  One
+// wave per bundle, as the walk; the groups' minima are merged through LDS.
+__global__ __launch_bounds__(kWalkBlock) void k_bundle_min(int64_t nbundles, const Bundle* __restrict__ bundles,
+                                                           const float* __restrict__ CE, int Qs,
+                                                           float* __restrict__ bmin) {
+  __shared__ f4v red[kWalkBlock / 64][64];
+  const int L4 = Qs / 4, G = 64 / L4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane / L4, j = lane - g * L4;
+  const int64_t b = (int64_t)blockIdx.x * (kWalkBlock / 64) + wave;
+  f4v m = f4v{INFINITY, INFINITY, INFINITY, INFINITY};
+  if (g < G && b < nbundles) {
+    const Bundle Bd = bundles[b];
+    const f4v* ce = reinterpret_cast<const f4v*>(CE) + (Bd.base + g) * L4 + j;
+    for (int s = 0; s < Bd.len; ++s) {
+      const f4v c = __builtin_nontemporal_load(ce + (int64_t)s * G * L4);
+      m = f4v{fminf(m[0], c[0]), fminf(m[1], c[1]), fminf(m[2], c[2]), fminf(m[3], c[3])};
+    }
+  }
+  red[wave][lane] = m;
+  __syncthreads();
+  if (b < nbundles && lane < L4) {
+    f4v a = red[wave][lane];
+    for (int gg = 1; gg < G; ++gg) {
+      const f4v c = red[wave][gg * L4 + lane];
+      a = f4v{fminf(a[0], c[0]), fminf(a[1], c[1]), fminf(a[2], c[2]), fminf(a[3], c[3])};
+    }
+    reinterpret_cast<f4v*>(bmin)[b * L4 + lane] = a;
+  }
+}
+
+void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE, int Qs, float* bmin, hipStream_t st) {
+  if (nbundles == 0) return;
+  const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
+  hipLaunchKernelGGL(k_bundle_min, dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, CE, Qs, bmin);
 }
 
 // Bundle layout from the CSC arrays: one block per bundle writes its rows (step-major, run-minor).  rowmap[k] is

@@ -14,7 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (AC_DATA, AC_OPT_GRAPH, AC_OPT_SLAB_EDGES, AC_OPT_TAPE, AC_OPT_WALK_BATCH, AC_PATTERN,  # noqa: F401
+from ._lib import (AC_DATA, AC_OPT_GRAPH, AC_OPT_PRUNE, AC_OPT_SLAB_EDGES, AC_OPT_TAPE, AC_OPT_WALK_BATCH,  # noqa: F401
+                   AC_PATTERN,
                    ACError, HostGraph)
 
 

@@ -370,7 +370,10 @@ def main():
     dn, de, pn, pe = d.num_nodes, d.num_edges, p.num_nodes, p.num_edges
     log(f"workload {w.name}: dn={dn} de={de} pn={pn} pe={pe} generated in {time.time() - t:.1f} s")
 
+    from application_classification_amd.matcher import AC_OPT_PRUNE
+
     m = Matcher(0)
+    m.set_option(AC_OPT_PRUNE, 0)  # the headline walks every edge; the exact pruning is measured separately below
     t = time.time()
     m.set_data(d)
     m.set_pattern(p)
@@ -428,6 +431,25 @@ def main():
                     "alg_bytes_per_launch": B[dom], "avg_launch_ms": avg_ms}
         log(f"stages (ms, summed over {pn} iterations): {json.dumps({k: round(v, 4) for k, v in stages.items()})}")
 
+    # Exact pruning (AC_OPT_PRUNE, the library default): the same run skipping edge bundles that provably cannot
+    # raise FMax above the fill; the output must be bit-identical.
+    m.set_option(AC_OPT_PRUNE, 1)
+    m.run_async()
+    m.sync()
+    pr_loop = pr_prep = 0.0
+    for i in range(args.steps):
+        m.run_async()
+        m.sync()
+        a, b = m.timing()
+        pr_prep += a
+        pr_loop += b
+    pr_loop /= args.steps
+    pr_prep /= args.steps
+    pruned = {"value": updates / (pr_loop * 1e-3), "loop_ms_per_step": pr_loop, "prep_ms_per_step": pr_prep,
+              "speedup_loop": loop_ms / pr_loop, "bit_identical": bool(np.array_equal(m.mu(), mu))}
+    log(f"pruned: loop {pr_loop:.3f} ms ({pruned['speedup_loop']:.2f}x), identical: {pruned['bit_identical']}")
+    m.set_option(AC_OPT_PRUNE, 0)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -458,6 +480,7 @@ def main():
         "hbm_roofline_frac_iteration": iter_frac,
         "alg_bytes_per_iteration": B["iteration"],
         "roofline": roofline,
+        "pruned": pruned,
         "cpu_baseline": cpu,
         "stage_ms": stages,
         "checks": checks,

@@ -149,6 +149,9 @@ int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms);
 #define AC_OPT_GRAPH 2       /* 1: replay iteration pairs from a captured HIP graph (fewer launches); 0: launch each */
 #define AC_OPT_SLAB_EDGES 3  /* batched runs: most pattern edges packed into one slab, 4..256 (default 128)         */
 #define AC_OPT_TAPE 4        /* > 0: runs record what ac_backward needs, checkpointing every this many iterations  */
+#define AC_OPT_PRUNE 5       /* 1 (default): the edge walk skips bundles of edges whose costs are all at or above a
+                              * per-pattern-edge bound past which no edge term can exceed the fill value, so the
+                              * result is bit-identical; 0: every edge is walked                                  */
 int ac_set_option(ac_ctx* ctx, int option, int value);
 int ac_get_option(const ac_ctx* ctx, int option, int* value);
 

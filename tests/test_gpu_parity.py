@@ -249,3 +249,29 @@ def test_stage_compute_mu_and_transpose():
     got = mu.cpu().numpy().reshape(pn, dn)
     np.testing.assert_array_equal(got, ref.astype(np.float32))  # FP64 sum in the same order, rounded once
     np.testing.assert_array_equal(out.cpu().numpy().reshape(dn, pn), got.T)
+
+
+@pytest.mark.parametrize("case", ["small", "rmat10", "rmat14", "s18", "uneven"])
+def test_pruning_is_exact(small, case):
+    """Skipping bundles whose costs all lie past the per-pattern-edge bound leaves every output bit-identical."""
+    from application_classification_amd.matcher import AC_OPT_PRUNE
+
+    if case == "small":
+        runs = [(small.data, small.pattern)]
+    elif case == "s18":
+        w = W.s18(0)
+        runs = [(w.data, w.pattern)]
+    elif case == "uneven":
+        runs = [(small.data, p) for p in W.uneven_patterns(small.data, 4, seed=9, v_range=(3, 20))]
+    else:
+        c = cases.rmat_case(int(case[4:]), 16, 8, 12)
+        runs = [(c.data, c.pattern)]
+    for d, p in runs:
+        out = []
+        for prune in (0, 1):
+            with Matcher(0) as m:
+                m.set_option(AC_OPT_PRUNE, prune)
+                m.set_data(d)
+                m.set_pattern(p)
+                out.append(m.run())
+        np.testing.assert_array_equal(out[0], out[1])
