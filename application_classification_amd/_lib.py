@@ -25,6 +25,7 @@ AC_OPT_GRAPH = 2
 AC_OPT_SLAB_EDGES = 3
 AC_OPT_TAPE = 4
 AC_OPT_PRUNE = 5
+AC_OPT_CDIST_MFMA = 6
 STAGES = ["prep", "vertex_a", "reduce_a", "edge_walk", "walk_fixup", "vertex_c", "reduce_c", "output"]
 
 ERROR_NAMES = {

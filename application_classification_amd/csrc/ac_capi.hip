@@ -172,6 +172,8 @@ struct ac_ctx {
   int walk_B = 0;          // steps per batch of the edge walk (0: default)
   bool use_graph = false;  // replay iteration pairs from a captured graph
   bool prune = true;       // skip bundles whose edges cannot raise FMax above the fill (exact)
+  bool use_mfma_cdist = true;  // feature distances on FP64 matrix cores (GEMM form, cancellation-guarded)
+  DevBuf<double> cd_scratch;   // the matrix-core operands of the pattern features
   DevBuf<float> bmin;      // [bundles][Qs] least cost per bundle and pattern edge
   DevBuf<double> carry_in, carry_out;
   DevBuf<QPart> qpart;
@@ -611,14 +613,17 @@ static int enqueue_prep(ac_ctx* ctx) {
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
   MARK(-1);
   // ---- prep (src/main.cu:107-148)
+  HIPCHK(ctx, ctx->cd_scratch.alloc(std::max<size_t>(1, std::max(ac::cdist_scratch_doubles(P, (int)ctx->dV),
+                                                                  ac::cdist_scratch_doubles(Qs, (int)ctx->dE)))));
+  double* cds = ctx->use_mfma_cdist ? ctx->cd_scratch.p : nullptr;
   ac::launch_cdist(n, nullptr, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P,
-                   ctx->has_wv ? ctx->wv.p : nullptr, st);
+                   ctx->has_wv ? ctx->wv.p : nullptr, cds, st);
   ac::launch_col_lse_neg(n, P, ctx->CV.p, ctx->lpart.p, ctx->nb_col, st);
   ac::launch_reduce_lse(ctx->nb_col, P, ctx->lpart.p, ctx->lse_cv.p, st);
   HIPCHK(ctx, hipMemcpyAsync(ctx->pstat.p, ctx->lse_cv.p, sizeof(Lse2) * P, hipMemcpyDeviceToDevice, st));
   if (Q > 0) {
     ac::launch_cdist(ctx->bs->brows, ctx->bs->beid.p, ctx->efeat.p, (int)ctx->dE, ctx->pf_e.p, Q, round8(Q), Qs, ctx->CE.p,
-                     Qs, ctx->has_we ? ctx->we.p : nullptr, st);
+                     Qs, ctx->has_we ? ctx->we.p : nullptr, cds, st);
     ac::launch_fill(ctx->minCEo.p, n * Qs, INFINITY, st);
     ac::launch_fill(ctx->lsO.p, n * Qs, 0.f, st);
     ac::launch_fill(ctx->VFm.p, n * Qs, -INFINITY, st);
@@ -960,6 +965,9 @@ extern "C" int ac_set_option(ac_ctx* ctx, int option, int value) {
     case AC_OPT_PRUNE:
       ctx->prune = value != 0;
       return AC_OK;
+    case AC_OPT_CDIST_MFMA:
+      ctx->use_mfma_cdist = value != 0;
+      return AC_OK;
     default:
       return ctx->fail(AC_ERR_ARG, "ac_set_option: unknown option " + std::to_string(option));
   }
@@ -982,6 +990,9 @@ extern "C" int ac_get_option(const ac_ctx* ctx, int option, int* value) {
       return AC_OK;
     case AC_OPT_PRUNE:
       *value = ctx->prune ? 1 : 0;
+      return AC_OK;
+    case AC_OPT_CDIST_MFMA:
+      *value = ctx->use_mfma_cdist ? 1 : 0;
       return AC_OK;
     default:
       return AC_ERR_ARG;

@@ -87,8 +87,10 @@ struct VertexArgs {
 enum VertexMode { kModeA = 0, kModeC = 1, kModeOut = 2 };
 
 // Launchers (ac_kernels.hip).  All asynchronous on `st`.
+// scratch (may be null: no matrix-core path): cdist_scratch_doubles(Kstore, dim) doubles.
 void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, const double* pat, int K, int Kp,
-                  int Kstore, float* out, int out_stride, const double* w, hipStream_t st);
+                  int Kstore, float* out, int out_stride, const double* w, double* scratch, hipStream_t st);
+size_t cdist_scratch_doubles(int Kstore, int dim);
 void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks, hipStream_t st);
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st);
 void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st);

@@ -142,6 +142,150 @@ __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int
   }
 }
 
+// FP64 matrix cores (v_mfma_f64_16x16x4f64), GEMM form: d^2 = |x|^2 + |p|^2 - 2 x.p with the weights folded in
+// (x.p -> sum w x p, |x|^2 -> sum w x^2, |p|^2 -> sum w p^2).  One wave computes a tile of 16 rows against 16
+// patterns per instruction sequence; the operand k index is permuted so that each lane's A fragment is one contiguous
+// chunk of its row: lane l = i + 16 kk holds row i's features [kk D4, (kk + 1) D4), and step s of the product takes
+// feature kk D4 + s from both operands (the sum over k is order-free in exact arithmetic and the accumulation is FP64).
+// D[4 r + l / 16][l % 16] is register r of lane l.  Pairs with d^2 < 2^-10 (|x|^2 + |p|^2) lose digits to
+// cancellation and are recomputed directly as sum w (x - p)^2 in FP64, so the result keeps the direct form's
+// accuracy (relative error of d^2 below 1e-11 elsewhere).
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+template <int D4>
+__global__ __launch_bounds__(256) void k_cdist_mfma(int64_t nrows, const int* __restrict__ idx,
+                                                    const float* __restrict__ feats, int dim,
+                                                    const double* __restrict__ patm, const double* __restrict__ pp,
+                                                    const double* __restrict__ w, const double* __restrict__ pat,
+                                                    int K, int Kstore, float* __restrict__ out, int out_stride) {
+  extern __shared__ double bsh[];  // [npt * 16][4 D4] pattern operands (the block's copy of patm)
+  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int npt = (Kstore + 15) / 16;
+  for (int e = threadIdx.x; e < npt * 16 * 4 * D4; e += 256) bsh[e] = patm[e];
+  __syncthreads();
+  // Whole 16-float chunks load as float4s when the row width is exactly 4 D4 (e.g. 32 or 64 features).
+  const bool vec = dim == 4 * D4 && (D4 % 4) == 0;
+  auto load_a = [&](int64_t r0, double* a, int64_t& src) {
+    const int64_t row = r0 + i;
+    src = row < nrows ? (idx ? (int64_t)idx[row] : row) : -1;
+    if (vec) {
+      const f4v* x4 = reinterpret_cast<const f4v*>(feats + (src >= 0 ? src : 0) * dim + kk * D4);
+#pragma unroll
+      for (int u = 0; u < D4 / 4; ++u) {
+        const f4v v = src >= 0 ? x4[u] : f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) a[4 * u + c] = (double)v[c];
+      }
+    } else {
+      const float* x = feats + (src >= 0 ? src : 0) * dim;
+#pragma unroll
+      for (int s2 = 0; s2 < D4; ++s2) {
+        const int d = kk * D4 + s2;
+        a[s2] = (src >= 0 && d < dim) ? (double)x[d] : 0.0;
+      }
+    }
+  };
+  double a[D4], an[D4];
+  int64_t src = -1, srcn = -1;
+  int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  if (r0 < nrows) load_a(r0, a, src);
+  for (; r0 < nrows; r0 += nw * 16) {
+    if (r0 + nw * 16 < nrows) load_a(r0 + nw * 16, an, srcn);  // the next tile's rows, in flight meanwhile
+    double xx = 0.0;
+#pragma unroll
+    for (int s2 = 0; s2 < D4; ++s2) {
+      const int d = kk * D4 + s2;
+      xx = fma(w ? w[d < dim ? d : 0] * a[s2] : a[s2], a[s2], xx);
+    }
+    xx += __shfl_xor(xx, 16, 64);
+    xx += __shfl_xor(xx, 32, 64);  // lanes i, i + 16, ...: |x|^2 of row i
+    double xxr[4];
+    int64_t srcr[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      xxr[r] = __shfl(xx, 4 * r + kk, 64);
+      srcr[r] = __shfl(src, 4 * r + kk, 64);
+    }
+    // Pattern tiles in pairs: two independent accumulator chains interleave on the matrix core.
+    for (int pt = 0; pt < npt; pt += 2) {
+      const bool two = pt + 1 < npt;
+      const double* bp0 = bsh + (pt * 16 + i) * (4 * D4) + kk * D4;
+      const double* bp1 = bsh + ((two ? pt + 1 : pt) * 16 + i) * (4 * D4) + kk * D4;
+      d4v acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s2 = 0; s2 < D4; ++s2) {
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], bp0[s2], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], bp1[s2], acc1, 0, 0, 0);
+      }
+      for (int h = 0; h < (two ? 2 : 1); ++h) {
+        const int c = (pt + h) * 16 + i;
+        const d4v acc = h ? acc1 : acc0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t rr = r0 + 4 * r + kk;
+          if (rr >= nrows || c >= Kstore) continue;
+          float v;
+          if (srcr[r] < 0) {
+            v = INFINITY;
+          } else if (c >= K) {
+            v = 0.f;
+          } else {
+            const double ref = xxr[r] + pp[c];
+            double d2 = ref - 2.0 * acc[r];
+            if (d2 < 0x1p-10 * ref) {  // cancellation: recompute directly
+              const float* xr = feats + srcr[r] * dim;
+              d2 = 0.0;
+              for (int d = 0; d < dim; ++d) {
+                const double e = (double)xr[d] - pat[((int64_t)(c / 8) * dim + d) * 8 + c % 8];
+                d2 = fma(w ? w[d] * e : e, e, d2);
+              }
+            }
+            v = (float)sqrt(d2);
+          }
+          out[rr * out_stride + c] = v;
+        }
+      }
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < D4; ++s2) a[s2] = an[s2];
+    src = srcn;
+  }
+}
+
+// MFMA operands of the patterns: patm[c][4 D4] = w_d p_c[d] (zero past dim or past K), pp[c] = sum_d w_d p_c[d]^2.
+__global__ void k_build_patm(int K, int Kp16, int dim, int D4, const double* __restrict__ pat,
+                             const double* __restrict__ w, double* __restrict__ patm, double* __restrict__ pp) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Kp16) return;
+  double s2 = 0.0;
+  for (int d = 0; d < 4 * D4; ++d) {
+    double v = 0.0;
+    if (c < K && d < dim) {
+      const double pv = pat[((int64_t)(c / 8) * dim + d) * 8 + c % 8];
+      const double wd = w ? w[d] : 1.0;
+      v = wd * pv;
+      s2 += v * pv;
+    }
+    patm[(int64_t)c * (4 * D4) + d] = v;
+  }
+  pp[c] = s2;
+}
+
+int cdist_mfma_d4(int dim) {
+  if (dim <= 16) return 4;
+  if (dim <= 32) return 8;
+  if (dim <= 64) return 16;
+  if (dim <= 128) return 32;
+  return 0;
+}
+
+size_t cdist_scratch_doubles(int K, int dim) {
+  const int D4 = cdist_mfma_d4(dim);
+  const int Kp16 = (K + 15) / 16 * 16;
+  return D4 ? (size_t)Kp16 * (4 * D4 + 1) : 0;
+}
+
 // Fallback for feature widths too large to stage: one thread per output element, rows read from global memory.
 __global__ void k_cdist_direct(int64_t nrows, const int* __restrict__ idx, const float* __restrict__ feats, int dim,
                                const double* __restrict__ pat, int K, int Kp, int Kstore, float* __restrict__ out,
@@ -163,8 +307,32 @@ __global__ void k_cdist_direct(int64_t nrows, const int* __restrict__ idx, const
 }
 
 void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, const double* pat, int K, int Kp,
-                  int Kstore, float* out, int out_stride, const double* w, hipStream_t st) {
+                  int Kstore, float* out, int out_stride, const double* w, double* scratch, hipStream_t st) {
   if (nrows == 0 || Kstore == 0) return;
+  const int D4 = cdist_mfma_d4(dim);
+  // The pattern operands are staged in LDS: up to 64 KB.
+  if (scratch && D4 && dim > 0 && (size_t)((Kstore + 15) / 16 * 16) * 4 * D4 * sizeof(double) <= 65536) {
+    const int Kp16 = (Kstore + 15) / 16 * 16;
+    double* patm = scratch;
+    double* pp = scratch + (size_t)Kp16 * 4 * D4;
+    hipLaunchKernelGGL(k_build_patm, dim3((Kp16 + 63) / 64), dim3(64), 0, st, K, Kp16, dim, D4, pat, w, patm, pp);
+    const int blocks = (int)std::min<int64_t>((nrows + 63) / 64, 8192);
+    const size_t lds = (size_t)Kp16 * 4 * D4 * sizeof(double);
+#define AC_CD_MFMA(N)                                                                                            \
+  case N:                                                                                                        \
+    hipLaunchKernelGGL(k_cdist_mfma<N>, dim3(blocks), dim3(256), lds, st, nrows, idx, feats, dim, patm, pp, w, pat, \
+                       K, Kstore, out, out_stride);                                                              \
+    return;
+    switch (D4) {
+      AC_CD_MFMA(4)
+      AC_CD_MFMA(8)
+      AC_CD_MFMA(16)
+      AC_CD_MFMA(32)
+      default:
+        break;
+    }
+#undef AC_CD_MFMA
+  }
   if (dim <= 255) {
     // At most 64 KB of LDS: 128 rows up to 127 features, fewer rows beyond.
     const int tile = std::min(kCdTile, 16384 / (dim + 1));

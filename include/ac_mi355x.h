@@ -152,6 +152,8 @@ int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms);
 #define AC_OPT_PRUNE 5       /* 1 (default): the edge walk skips bundles of edges whose costs are all at or above a
                               * per-pattern-edge bound past which no edge term can exceed the fill value, so the
                               * result is bit-identical; 0: every edge is walked                                  */
+#define AC_OPT_CDIST_MFMA 6  /* 1 (default): feature distances on FP64 matrix cores in GEMM form, pairs that lose
+                              * digits to cancellation recomputed directly; 0: direct form on the vector units     */
 int ac_set_option(ac_ctx* ctx, int option, int value);
 int ac_get_option(const ac_ctx* ctx, int option, int* value);
 

@@ -275,3 +275,24 @@ def test_pruning_is_exact(small, case):
                 m.set_pattern(p)
                 out.append(m.run())
         np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.01])
+def test_matrix_core_cdist(small, sigma):
+    """Feature distances on FP64 matrix cores (GEMM form) against the direct form and the oracle; sigma = 0 plants
+    exact copies (distance exactly 0), which the cancellation guard recomputes directly."""
+    from application_classification_amd._lib import AC_OPT_CDIST_MFMA
+
+    d = small.data
+    p = W.planted_pattern(d, 10, 14, seed=11, sigma=sigma)
+    out = []
+    for mm in (1, 0):
+        with Matcher(0) as m:
+            m.set_option(AC_OPT_CDIST_MFMA, mm)
+            m.set_data(d)
+            m.set_pattern(p)
+            out.append(m.run())
+    ref = O.run(g64(d), g64(p))
+    assert_parity(out[0], ref, f"matrix-core cdist, sigma={sigma}")
+    assert_parity(out[0], out[1], f"matrix-core vs direct cdist, sigma={sigma}")
+    np.testing.assert_array_equal(out[0].argmax(axis=0), p.vmap)
