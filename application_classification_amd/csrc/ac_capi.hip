@@ -20,6 +20,7 @@ using ac::Lse;
 using ac::Lse2;
 using ac::QPart;
 using ac::QStat;
+using ac::IsoCol;
 using ac::Bundle;
 using ac::Run;
 using ac::Split;
@@ -139,6 +140,11 @@ struct ac_ctx {
   DevBuf<Run> runs_in, runs_out;
   DevBuf<Split> splits_in, splits_out;
   std::vector<Run> h_runs_in;
+  // Internal vertex order: vertices with in- and out-edges, in-edges only, out-edges only, then isolated ones, each
+  // class in ascending caller order.  perm maps an internal vertex to the caller's; the vertex passes stream the
+  // first n_act = dn - n_iso rows only.
+  DevBuf<int> d_perm;
+  int64_t n_act = 0, n_iso = 0;
   int64_t nruns_in = 0, nruns_out = 0;
   int nsplit_in = 0, nsplit_out = 0, nslots_in = 0, nslots_out = 0;
 
@@ -180,6 +186,7 @@ struct ac_ctx {
   DevBuf<Lse> ppart, lpart;
   DevBuf<QStat> qstat;  // [3][Q]: prep state, then by iteration parity
   DevBuf<Lse2> pstat, lse_cv, lse_ce;
+  DevBuf<IsoCol> iso;   // [P] the isolated vertices' columns of CV (when n_iso > 0)
   int nb_vertex = 0, nb_col = 0, nb_src = 0, VT = 0, VRs = 0;
   int64_t work_key[5] = {-1, -1, -1, -1, -1};
 
@@ -196,7 +203,7 @@ struct ac_ctx {
   DevBuf<QStat> ckQ, sgQ;
   DevBuf<Lse2> ckP, sgP;
   // Adjoint work buffers.
-  DevBuf<float> gMU, gCV, gFX, gRX, gEF, gER, gVR, gVF, gCE, apv;
+  DevBuf<float> gMUin, gMU, gCV, gFX, gRX, gEF, gER, gVR, gVF, gCE, apv;
   DevBuf<double> cD, colsum, bpart, red4, cq, opart, osum, MUd;
   DevBuf<int> api;
 
@@ -287,6 +294,33 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
                     const int32_t* src, const int32_t* dst) {
   if (nE >= (int64_t)INT32_MAX - 4096)
     return ctx->fail(AC_ERR_ARG, "ac_set_graph: more than 2^31 edges is not supported");
+  // Internal order by degree class (see ac_ctx::d_perm); every structure below uses internal vertex ids.
+  std::vector<int> perm, inv(nV);
+  int64_t n_iso = 0;
+  std::vector<int32_t> isrc(nE), idst(nE);
+  {
+    std::vector<uint8_t> cls(nV, 3);  // bit 0 clear: has in-edges; bit 1 clear: has out-edges
+    for (int64_t e = 0; e < nE; ++e) {
+      cls[dst[e]] &= 2;
+      cls[src[e]] &= 1;
+    }
+    // Class key: 0 both, 1 in-edges only (cls 2), 2 out-edges only (cls 1), 3 isolated.
+    const uint8_t key[4] = {0, 2, 1, 3};
+    perm.reserve(nV);
+    for (int c = 0; c < 4; ++c)
+      for (int64_t v = 0; v < nV; ++v)
+        if (key[cls[v]] == c) {
+          inv[v] = (int)perm.size();
+          perm.push_back((int)v);
+          if (c == 3) ++n_iso;
+        }
+    for (int64_t e = 0; e < nE; ++e) {
+      isrc[e] = inv[src[e]];
+      idst[e] = inv[dst[e]];
+    }
+  }
+  src = isrc.data();
+  dst = idst.data();
   // CSC (by destination, stable) and CSR (by source, stable) views.
   std::vector<int64_t> in_off(nV + 1, 0), out_off(nV + 1, 0);
   for (int64_t e = 0; e < nE; ++e) {
@@ -321,6 +355,7 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   ctx->has_data = false;
   hipStream_t st = ctx->stream;
   HIPCHK(ctx, ctx->vfeat.upload(vf, (size_t)(nV * dV), st));
+  HIPCHK(ctx, ctx->d_perm.upload(perm.data(), perm.size(), st));
   HIPCHK(ctx, ctx->efeat.upload(ef, (size_t)(nE * dE), st));
   HIPCHK(ctx, ctx->csc_sd.upload(csc_sd.data(), csc_sd.size(), st));
   HIPCHK(ctx, ctx->csr_ks.upload(csr_ks.data(), csr_ks.size(), st));
@@ -330,6 +365,8 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   HIPCHK(ctx, ctx->splits_in.upload(sin.data(), sin.size(), st));
   HIPCHK(ctx, ctx->splits_out.upload(sout.data(), sout.size(), st));
   ctx->dn = nV;
+  ctx->n_iso = n_iso;
+  ctx->n_act = nV - n_iso;
   ctx->dV = dV;
   ctx->de = nE;
   ctx->dE = dE;
@@ -507,10 +544,10 @@ static int ensure_work(ac_ctx* ctx) {
   if (rc != AC_OK) return rc;
   const int64_t key[5] = {ctx->dn, ctx->de, ctx->P, ctx->Q, ctx->bs->brows};
   if (std::equal(key, key + 5, ctx->work_key)) return AC_OK;
-  const int64_t n = ctx->dn, m = ctx->de;
+  const int64_t n = ctx->dn;
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
   ctx->VT = ac::vertex_tile(P, Q);
-  const int64_t ntiles = (n + ctx->VT - 1) / ctx->VT;
+  const int64_t ntiles = (std::max<int64_t>(ctx->n_act, 1) + ctx->VT - 1) / ctx->VT;
   ctx->nb_vertex = (int)std::min<int64_t>(ntiles, 8192);
   ctx->nb_col = (int)std::min<int64_t>((n + 255) / 256, 1024);
   ctx->nb_src = std::max(1, ac::walk_blocks(ctx->nruns_out, Qs));
@@ -533,6 +570,9 @@ static int ensure_work(ac_ctx* ctx) {
   HIPCHK(ctx, ctx->pstat.alloc((size_t)P));
   HIPCHK(ctx, ctx->lse_cv.alloc((size_t)P));
   HIPCHK(ctx, ctx->lse_ce.alloc((size_t)std::max(1, Q)));
+  HIPCHK(ctx, ctx->iso.alloc((size_t)P));
+  HIPCHK(ctx, ctx->apv.alloc((size_t)ctx->nb_col * P));
+  HIPCHK(ctx, ctx->api.alloc((size_t)ctx->nb_col * P));
   // Padding columns of the row-major [n][Qs] arrays are never read as data; keep them zero.
   HIPCHK(ctx, hipMemsetAsync(ctx->VR.p, 0, ctx->VR.n * sizeof(float), ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->qstat.p, 0, ctx->qstat.n * sizeof(QStat), ctx->stream));
@@ -616,10 +656,13 @@ static int enqueue_prep(ac_ctx* ctx) {
   HIPCHK(ctx, ctx->cd_scratch.alloc(std::max<size_t>(1, std::max(ac::cdist_scratch_doubles(P, (int)ctx->dV),
                                                                   ac::cdist_scratch_doubles(Qs, (int)ctx->dE)))));
   double* cds = ctx->use_mfma_cdist ? ctx->cd_scratch.p : nullptr;
-  ac::launch_cdist(n, nullptr, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P,
+  ac::launch_cdist(n, ctx->d_perm.p, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P,
                    ctx->has_wv ? ctx->wv.p : nullptr, cds, st);
   ac::launch_col_lse_neg(n, P, ctx->CV.p, ctx->lpart.p, ctx->nb_col, st);
   ac::launch_reduce_lse(ctx->nb_col, P, ctx->lpart.p, ctx->lse_cv.p, st);
+  if (ctx->n_iso > 0)
+    ac::launch_iso_cols(ctx->n_iso, P, ctx->CV.p + ctx->n_act * P, (int)ctx->n_act, ctx->lpart.p, ctx->apv.p,
+                        ctx->api.p, std::min(ctx->nb_col, 256), ctx->iso.p, st);
   HIPCHK(ctx, hipMemcpyAsync(ctx->pstat.p, ctx->lse_cv.p, sizeof(Lse2) * P, hipMemcpyDeviceToDevice, st));
   if (Q > 0) {
     ac::launch_cdist(ctx->bs->brows, ctx->bs->beid.p, ctx->efeat.p, (int)ctx->dE, ctx->pf_e.p, Q, round8(Q), Qs, ctx->CE.p,
@@ -665,9 +708,25 @@ static int enqueue_prep(ac_ctx* ctx) {
 }
 
 
+// The closed-form share of the isolated vertices for the reductions after a vertex pass that read state `qs`.
+static ac::IsoArgs iso_args(ac_ctx* ctx, const QStat* qs, int first) {
+  ac::IsoArgs a{};
+  a.col = ctx->n_iso > 0 ? ctx->iso.p : nullptr;
+  a.inc_off = ctx->d_inc_off.p;
+  a.inc = ctx->d_inc.p;
+  a.ps = ctx->d_ps.p;
+  a.pd = ctx->d_pd.p;
+  a.qs = qs;
+  a.pst = ctx->pstat.p;
+  a.P = ctx->P;
+  a.first = first;
+  return a;
+}
+
 static ac::VertexArgs vertex_args(ac_ctx* ctx) {
   ac::VertexArgs va{};
-  va.n = ctx->dn;
+  va.n = ctx->n_act;
+  va.perm = ctx->d_perm.p;
   va.P = ctx->P;
   va.Q = ctx->Q;
   va.Qs = ctx->Qs;
@@ -701,7 +760,8 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
     va.qs = qs_state;
     ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
     MARK(AC_STAGE_VERTEX_A);
-    ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_new, st);
+    const ac::IsoArgs isoA = iso_args(ctx, qs_state, va.first);
+    ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_new, st, &isoA);
     MARK(AC_STAGE_REDUCE_A);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p,
@@ -715,16 +775,18 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
   va.qs = qs_state;
   ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex, st);
   MARK(AC_STAGE_VERTEX_C);
-  ac::launch_reduce_lse(ctx->nb_vertex, P, ctx->ppart.p, ctx->pstat.p, st);
+  ac::launch_reduce_lse_iso(ctx->nb_vertex, P, ctx->ppart.p, ctx->pstat.p, iso_args(ctx, qs_state, 0), st);
   MARK(AC_STAGE_REDUCE_C);
   return AC_OK;
 }
 
 // Output of the current state (src/main.cu:247): MU[v][p] written directly in dn x pn order into OUT.
 static int enqueue_output(ac_ctx* ctx, ac::VertexArgs& va, int done, const QStat* qs_state) {
-  va.first = done == 0;
-  va.qs = qs_state;
-  ac::launch_vertex(ac::kModeOut, va, ctx->nb_vertex, ctx->stream);
+  ac::VertexArgs vo = va;
+  vo.n = ctx->dn;  // every row, isolated ones included, written to the caller's row
+  vo.first = done == 0;
+  vo.qs = qs_state;
+  ac::launch_vertex(ac::kModeOut, vo, ctx->nb_vertex, ctx->stream);
   return AC_OK;
 }
 
@@ -1262,7 +1324,8 @@ static void recompute_iteration(ac_ctx* ctx, int it, const QStat* qs_in, QStat* 
     va.first = it == 0;
     va.qs = qs_in;
     ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
-    ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_out, st);
+    const ac::IsoArgs isoA = iso_args(ctx, qs_in, va.first);
+    ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_out, st, &isoA);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_out, ctx->FMaxN.p, ctx->carry_in.p,
                         walk_batch_of(ctx), st, FA, ctx->carg_in.p, ctx->prune ? ctx->bmin.p : nullptr);
@@ -1272,7 +1335,7 @@ static void recompute_iteration(ac_ctx* ctx, int it, const QStat* qs_in, QStat* 
   va.first = 0;
   va.qs = Q > 0 ? qs_out : qs_in;
   ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex, st);
-  ac::launch_reduce_lse(ctx->nb_vertex, P, ctx->ppart.p, ctx->pstat.p, st);
+  ac::launch_reduce_lse_iso(ctx->nb_vertex, P, ctx->ppart.p, ctx->pstat.p, iso_args(ctx, va.qs, 0), st);
 }
 
 // Host side of the weight gradient from the outer-product sums A, B (K x dim) and C (K) and the pattern features
@@ -1304,7 +1367,10 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   const size_t nq = (size_t)n * Qs, np = (size_t)n * P;
   const int nparts = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 512));
   const int64_t brows = ctx->bs->brows;
-  HIPCHK(ctx, ctx->gMU.upload(grad_mu, np, st));
+  // grad_mu is in the caller's vertex order; the adjoint works in the internal one.
+  HIPCHK(ctx, ctx->gMUin.upload(grad_mu, np, st));
+  HIPCHK(ctx, ctx->gMU.alloc(np));
+  ac::launch_gather_rows(ctx->dn, ctx->P, ctx->d_perm.p, ctx->gMUin.p, ctx->gMU.p, st);
   HIPCHK(ctx, ctx->gCV.alloc(np));
   for (DevBuf<float>* b : {&ctx->gFX, &ctx->gRX, &ctx->gEF, &ctx->gER, &ctx->gVR, &ctx->gVF}) HIPCHK(ctx, b->alloc(nq));
   HIPCHK(ctx, ctx->gCE.alloc((size_t)std::max<int64_t>(1, brows) * Qs));
@@ -1404,7 +1470,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
     const size_t w = (size_t)2 * P * dV + P;
     HIPCHK(ctx, ctx->opart.alloc((size_t)onp * w));
     HIPCHK(ctx, ctx->osum.alloc(w));
-    ac::launch_outer_acc(n, nullptr, ctx->vfeat.p, dV, ctx->gCV.p, P, P, ctx->opart.p, onp, st);
+    ac::launch_outer_acc(n, ctx->d_perm.p, ctx->vfeat.p, dV, ctx->gCV.p, P, P, ctx->opart.p, onp, st);
     ac::launch_sum_parts(onp, (int)w, ctx->opart.p, ctx->osum.p, st);
     std::vector<double> sv(w);
     HIPCHK(ctx, hipMemcpyAsync(sv.data(), ctx->osum.p, w * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -43,6 +43,28 @@ struct QPart {
   int avf, avr;  // vertex of vfmax / vrmax
 };
 
+// Isolated data vertices (no in- and no out-edge) hold constant rows (FMaxN = VFm = -inf), so their MU column is
+// -CV + (a per-p constant).  Per pattern vertex p: the log-sum-exp of -CV over them as (m, t), and the minimum of CV
+// with its lowest (internal) vertex, computed once per run.
+struct IsoCol {
+  double m, t;
+  float cmin;
+  int vmin;  // -1: no isolated vertex
+};
+
+// What the reductions need to add the isolated vertices' share in closed form (col == nullptr: none).
+struct IsoArgs {
+  const IsoCol* col;
+  const int* inc_off;
+  const int* inc;
+  const int* ps;
+  const int* pd;
+  const QStat* qs;   // scalars of the state the vertex pass read
+  const Lse2* pst;   // its MU column log-sum-exps (pass A only)
+  int P;
+  int first;         // the prep state: no FMax / RMax terms
+};
+
 // A contiguous run of edges in CSC or CSR order covering whole vertices, or one piece of a vertex whose edge list
 // is longer than a run (slot >= 0 then names its carry slot).
 struct Run {
@@ -73,6 +95,7 @@ struct VertexArgs {
   float* VR;            // [n][VRs]
   int VRs;              // row stride of VR: a power of two up to 32 floats, else a multiple of 32 (whole lines)
   float* out;           // [n][P]
+  const int* perm;      // mode OUT: internal row -> caller's row of `out` (nullptr: identity)
   const int* ps;
   const int* pd;
   const int* inc_off;  // [P + 1]
@@ -93,7 +116,12 @@ void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, co
 size_t cdist_scratch_doubles(int Kstore, int dim);
 void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks, hipStream_t st);
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st);
-void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st);
+void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st, const IsoArgs* iso = nullptr);
+void launch_reduce_lse_iso(int nparts, int P, const Lse* part, Lse2* out, const IsoArgs& iso, hipStream_t st);
+// Per column p of the n rows X[n][P]: IsoCol{LSE of -X, min X and its lowest row + base}; partial buffers of
+// nparts * P entries each.
+void launch_iso_cols(int64_t n, int P, const float* X, int base, Lse* lpart, float* pv, int* pi, int nparts,
+                     IsoCol* out, hipStream_t st);
 void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE, int Qs, float* bmin, hipStream_t st);
 void launch_prep_scalars(int P, int Q, const int* ps, const int* pd, const Lse2* cv, const Lse2* ce, QStat* qs,
                          hipStream_t st);
@@ -112,6 +140,8 @@ void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, 
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
                            float* FMaxN, hipStream_t st, const int* carg = nullptr, int* FA = nullptr);
 void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st);
+// out[i][c] = in[perm[i]][c] for the n rows of ncol floats.
+void launch_gather_rows(int64_t n, int ncol, const int* perm, const float* in, float* out, hipStream_t st);
 void launch_copy_cols(int64_t n, int P, int c0, int ncol, const float* src, float* dst, hipStream_t st);
 int walk_blocks(int64_t nruns, int Qs);
 int walk_groups(int Qs);  // runs per bundle: 64 / (Qs / 4)

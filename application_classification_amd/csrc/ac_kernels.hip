@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <climits>
 #include <cstdlib>
 
 #include "ac_internal.hpp"
@@ -378,9 +379,11 @@ void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks
   hipLaunchKernelGGL(k_col_lse_neg, dim3(nblocks), dim3(256), 0, st, n, P, X, part);
 }
 
-// Reduce per-block partials [nparts][ncol] column by column in a fixed order (deterministic).
+// Reduce per-block partials [nparts][ncol] column by column in a fixed order (deterministic).  With iso.col, column
+// p also receives the isolated vertices' share of the MU column log-sum-exp (merged last): their MU_pre is
+// -CV + K_p, K_p the sum of the fill values of p's incidences, so the share is LSE(-CV) + K_p.
 __global__ __launch_bounds__(256) void k_reduce_lse(int nparts, int ncol, const Lse* __restrict__ part,
-                                                    Lse2* __restrict__ out) {
+                                                    Lse2* __restrict__ out, IsoArgs iso) {
   __shared__ Lse red[256];
   const int c = blockIdx.x, t = threadIdx.x;
   Lse s = lse_empty();
@@ -391,16 +394,47 @@ __global__ __launch_bounds__(256) void k_reduce_lse(int nparts, int ncol, const 
     if (t < w) red[t] = lse_merge(red[t], red[t + w]);
     __syncthreads();
   }
-  if (t == 0) out[c] = Lse2{red[0].m, log1p(red[0].t)};
+  if (t == 0) {
+    Lse r = red[0];
+    if (iso.col && iso.col[c].vmin >= 0) {
+      double K = 0.0;
+      if (!iso.first)
+        for (int k = iso.inc_off[c]; k < iso.inc_off[c + 1]; ++k) {
+          const int code = iso.inc[k], qq = code >> 1;
+          K += (code & 1) ? fmax(iso.qs[qq].fillR, -kInf) : fmax(iso.qs[qq].fillF, -kInf);
+        }
+      const double m = iso.col[c].m + K;
+      if (m > -kInf) r = lse_merge(r, Lse{m, iso.col[c].t});
+    }
+    out[c] = Lse2{r.m, log1p(r.t)};
+  }
 }
 
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st) {
   if (ncol == 0) return;
-  hipLaunchKernelGGL(k_reduce_lse, dim3(ncol), dim3(256), 0, st, nparts, ncol, part, out);
+  IsoArgs none{};
+  hipLaunchKernelGGL(k_reduce_lse, dim3(ncol), dim3(256), 0, st, nparts, ncol, part, out, none);
+}
+
+void launch_reduce_lse_iso(int nparts, int P, const Lse* part, Lse2* out, const IsoArgs& iso, hipStream_t st) {
+  if (P == 0) return;
+  hipLaunchKernelGGL(k_reduce_lse, dim3(P), dim3(256), 0, st, nparts, P, part, out, iso);
+}
+
+// MU of pattern vertex p at the isolated vertex of least cost: the largest MU over the isolated vertices (MU is
+// monotone in -CV), evaluated with the vertex pass's own operation sequence (FMaxN = VFm = -inf rows).
+__device__ double iso_mu_max(const IsoArgs& iso, int p) {
+  double acc = -(double)iso.col[p].cmin;
+  if (!iso.first)
+    for (int k = iso.inc_off[p]; k < iso.inc_off[p + 1]; ++k) {
+      const int code = iso.inc[k], qq = code >> 1;
+      acc += (code & 1) ? fmax(iso.qs[qq].fillR, -kInf) : fmax(iso.qs[qq].fillF, (double)(-INFINITY));
+    }
+  return (acc - iso.pst[p].m) - iso.pst[p].l;
 }
 
 __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart* __restrict__ part,
-                                                  QStat* __restrict__ out) {
+                                                  QStat* __restrict__ out, IsoArgs iso) {
   __shared__ QPart red[256];
   const int q = blockIdx.x, t = threadIdx.x;
   QPart s{-kInf, -kInf, lse_empty(), lse_empty(), -1, -1};
@@ -416,7 +450,19 @@ __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart
     __syncthreads();
   }
   if (t == 0) {
-    const QPart& r = red[0];
+    QPart r = red[0];
+    if (iso.col) {
+      // The isolated vertices' VF = MU[pd] - fillF and VR = MU[ps] - fillR peak at their least-cost vertex; they
+      // have no out-edges, so they add nothing to the edge log-sum-exps.
+      const int pd = iso.pd[q], ps = iso.ps[q];
+      const double fm = fmax(iso.qs[q].fillF, (double)(-INFINITY)), rm = fmax(iso.qs[q].fillR, -kInf);
+      const double vf = iso.col[pd].vmin >= 0 ? iso_mu_max(iso, pd) - fm : -kInf;
+      const double vr = iso.col[ps].vmin >= 0 ? iso_mu_max(iso, ps) - rm : -kInf;
+      // As in the vertex pass, a -inf value records no vertex.
+      const QPart w{vf, vr, lse_empty(), lse_empty(), vf > -kInf ? iso.col[pd].vmin : -1,
+                    vr > -kInf ? iso.col[ps].vmin : -1};
+      qpart_merge(r, w);
+    }
     // Edge pruning bound.  An edge term is (VR[src] - CE) - LSE_FE with VR[src] <= fillF (rounded to FP32 once),
     // so it cannot exceed fillF when CE + LSE_FE >= 2^-22 (|fillF| + |CE| + |LSE_FE|); cthr = -LSE_FE plus four
     // times that margin (a cost near the bound has |CE| ~ |LSE_FE|; a larger one clears it by itself).
@@ -428,9 +474,79 @@ __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart
   }
 }
 
-void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st) {
+void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st, const IsoArgs* iso) {
   if (Q == 0) return;
-  hipLaunchKernelGGL(k_reduce_q, dim3(Q), dim3(256), 0, st, nparts, Q, part, out);
+  IsoArgs none{};
+  hipLaunchKernelGGL(k_reduce_q, dim3(Q), dim3(256), 0, st, nparts, Q, part, out, iso ? *iso : none);
+}
+
+// Isolated-vertex columns: one pass over X[n][P] giving per-block partials of LSE(-X) and of (min X, lowest row).
+__global__ __launch_bounds__(256) void k_iso_cols(int64_t n, int P, const float* __restrict__ X, Lse* __restrict__ lpart,
+                                                  float* __restrict__ pv, int* __restrict__ pi) {
+  __shared__ Lse red[256];
+  __shared__ float rv[256];
+  __shared__ int ri[256];
+  const int t = threadIdx.x, RP = 256 / P;
+  const bool active = t < RP * P;
+  const int p = active ? t % P : 0;
+  Lse acc = lse_empty();
+  float best = INFINITY;
+  int bi = INT_MAX;
+  if (active)
+    for (int64_t v = (int64_t)blockIdx.x * RP + t / P; v < n; v += (int64_t)gridDim.x * RP) {
+      const float x = X[v * P + p];
+      lse_add(acc, -(double)x);
+      if (x < best) {  // rows ascend within a thread
+        best = x;
+        bi = (int)v;
+      }
+    }
+  red[t] = acc;
+  rv[t] = best;
+  ri[t] = bi;
+  __syncthreads();
+  if (t < P) {
+    Lse s = lse_empty();
+    for (int r = 0; r < RP; ++r) {
+      s = lse_merge(s, red[r * P + t]);
+      const float x = rv[r * P + t];
+      const int xi = ri[r * P + t];
+      if (x < best || (x == best && xi < bi)) {
+        best = x;
+        bi = xi;
+      }
+    }
+    lpart[(int64_t)blockIdx.x * P + t] = s;
+    pv[(int64_t)blockIdx.x * P + t] = best;
+    pi[(int64_t)blockIdx.x * P + t] = bi;
+  }
+}
+
+__global__ void k_iso_final(int nparts, int P, int base, const Lse* __restrict__ lpart, const float* __restrict__ pv,
+                            const int* __restrict__ pi, IsoCol* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  Lse s = lse_empty();
+  float best = INFINITY;
+  int bi = INT_MAX;
+  for (int b = 0; b < nparts; ++b) {
+    s = lse_merge(s, lpart[(int64_t)b * P + p]);
+    const float x = pv[(int64_t)b * P + p];
+    const int xi = pi[(int64_t)b * P + p];
+    if (x < best || (x == best && xi < bi)) {
+      best = x;
+      bi = xi;
+    }
+  }
+  // A column whose isolated costs are all +inf has no finite share: treat it as empty.
+  out[p] = bi == INT_MAX ? IsoCol{-kInf, 0.0, INFINITY, -1} : IsoCol{s.m, s.t, best, base + bi};
+}
+
+void launch_iso_cols(int64_t n, int P, const float* X, int base, Lse* lpart, float* pv, int* pi, int nparts,
+                     IsoCol* out, hipStream_t st) {
+  if (P == 0) return;
+  hipLaunchKernelGGL(k_iso_cols, dim3(nparts), dim3(256), 0, st, n, P, X, lpart, pv, pi);
+  hipLaunchKernelGGL(k_iso_final, dim3((P + 255) / 256), dim3(256), 0, st, nparts, P, base, lpart, pv, pi, out);
 }
 
 // Scalars of the prep state (src/main.cu:116-146): MU0 = LSM(-CV), RE0 = FE0 = LSM(-CE).  The column maximum of
@@ -1167,7 +1283,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
           if (kA)
             MUs[r * ldm + p] = mu;
           else
-            a.out[(v0 + r) * P + p] = (float)mu;
+            a.out[(int64_t)(a.perm ? a.perm[v0 + r] : v0 + r) * P + p] = (float)mu;
         }
       }
     }
@@ -1254,6 +1370,21 @@ __global__ void k_copy_cols(int64_t n, int P, int c0, int ncol, const float* __r
     const int64_t v = i / ncol;
     dst[i] = src[v * P + c0 + (int)(i - v * ncol)];
   }
+}
+
+__global__ void k_gather_rows(int64_t n, int ncol, const int* __restrict__ perm, const float* __restrict__ in,
+                              float* __restrict__ out) {
+  const int64_t total = n * ncol;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ncol, c = i - r * ncol;
+    out[i] = in[(int64_t)perm[r] * ncol + c];
+  }
+}
+
+void launch_gather_rows(int64_t n, int ncol, const int* perm, const float* in, float* out, hipStream_t st) {
+  if (n * ncol == 0) return;
+  const int blocks = (int)std::min<int64_t>((n * ncol + 255) / 256, 1 << 16);
+  hipLaunchKernelGGL(k_gather_rows, dim3(blocks), dim3(256), 0, st, n, ncol, perm, in, out);
 }
 
 void launch_copy_cols(int64_t n, int P, int c0, int ncol, const float* src, float* dst, hipStream_t st) {
