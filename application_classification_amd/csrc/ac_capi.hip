@@ -39,6 +39,17 @@ int run_edges() {
 
 int round8(int x) { return (x + 7) / 8 * 8; }
 
+// Most blocks of vertex pass A / C.  Each block walks its tiles grid-stride and leaves one partial per pattern edge
+// (A) or vertex (C), merged serially afterwards, so fewer blocks make a cheaper merge: a few resident rounds of
+// blocks on 256 CUs (measured at R-MAT scale 22).
+int vertex_blocks(bool pass_c) {
+  if (const char* e = std::getenv(pass_c ? "AC_VERTEX_BLOCKS_C" : "AC_VERTEX_BLOCKS")) {  // tuning override
+    const int v = std::atoi(e);
+    if (v >= 64 && v <= 65536) return v;
+  }
+  return pass_c ? 1024 : 2048;
+}
+
 // Row stride of the gathered VR rows: whole 128-byte lines (or an exact fraction of one), so that a gather of one
 // row never straddles two lines.
 int gather_stride(int Qs) {
@@ -139,6 +150,10 @@ struct ac_ctx {
   DevBuf<int> csc_eid;
   DevBuf<Run> runs_in, runs_out;
   DevBuf<Split> splits_in, splits_out;
+  // The plain walk's fixup work: chunks of at most kFixChunk pieces of a split vertex ({v, first piece, count, out
+  // partial or -1}) and the vertices of several chunks ({v, first partial, count}).
+  DevBuf<Split> fix_chunks, fix_multi;
+  int nfix_chunks = 0, nfix_multi = 0, nfix_parts = 0;
   std::vector<Run> h_runs_in;
   // Internal vertex order: vertices with in- and out-edges, in-edges only, out-edges only, then isolated ones, each
   // class in ascending caller order.  perm maps an internal vertex to the caller's; the vertex passes stream the
@@ -182,12 +197,13 @@ struct ac_ctx {
   DevBuf<double> cd_scratch;   // the matrix-core operands of the pattern features
   DevBuf<float> bmin;      // [bundles][Qs] least cost per bundle and pattern edge
   DevBuf<double> carry_in, carry_out;
+  DevBuf<float> fix_part;  // [nfix_parts][Qs]
   DevBuf<QPart> qpart;
   DevBuf<Lse> ppart, lpart;
   DevBuf<QStat> qstat;  // [3][Q]: prep state, then by iteration parity
   DevBuf<Lse2> pstat, lse_cv, lse_ce;
   DevBuf<IsoCol> iso;   // [P] the isolated vertices' columns of CV (when n_iso > 0)
-  int nb_vertex = 0, nb_col = 0, nb_src = 0, VT = 0, VRs = 0;
+  int nb_vertex = 0, nb_vertex_c = 0, nb_col = 0, nb_src = 0, VT = 0, VRs = 0;
   int64_t work_key[5] = {-1, -1, -1, -1, -1};
 
   bool have_result = false;
@@ -351,6 +367,20 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   int slin = 0, slout = 0;
   build_runs(in_off, nV, rin, sin, slin);
   build_runs(out_off, nV, rout, sout, slout);
+  std::vector<Split> fch, fms;
+  int nparts = 0;
+  for (const Split& S : sin) {
+    const int nch = (S.count + ac::kFixChunk - 1) / ac::kFixChunk;
+    if (nch == 1) {
+      fch.push_back(Split{S.v, S.first, S.count, -1});
+      continue;
+    }
+    fms.push_back(Split{S.v, nparts, nch, 0});
+    for (int c = 0; c < nch; ++c)
+      fch.push_back(Split{S.v, S.first + c * ac::kFixChunk, std::min(ac::kFixChunk, S.count - c * ac::kFixChunk),
+                          nparts + c});
+    nparts += nch;
+  }
 
   ctx->has_data = false;
   hipStream_t st = ctx->stream;
@@ -364,6 +394,11 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   HIPCHK(ctx, ctx->runs_out.upload(rout.data(), rout.size(), st));
   HIPCHK(ctx, ctx->splits_in.upload(sin.data(), sin.size(), st));
   HIPCHK(ctx, ctx->splits_out.upload(sout.data(), sout.size(), st));
+  HIPCHK(ctx, ctx->fix_chunks.upload(fch.data(), fch.size(), st));
+  HIPCHK(ctx, ctx->fix_multi.upload(fms.data(), fms.size(), st));
+  ctx->nfix_chunks = (int)fch.size();
+  ctx->nfix_multi = (int)fms.size();
+  ctx->nfix_parts = nparts;
   ctx->dn = nV;
   ctx->n_iso = n_iso;
   ctx->n_act = nV - n_iso;
@@ -548,7 +583,8 @@ static int ensure_work(ac_ctx* ctx) {
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
   ctx->VT = ac::vertex_tile(P, Q);
   const int64_t ntiles = (std::max<int64_t>(ctx->n_act, 1) + ctx->VT - 1) / ctx->VT;
-  ctx->nb_vertex = (int)std::min<int64_t>(ntiles, 8192);
+  ctx->nb_vertex = (int)std::min<int64_t>(ntiles, vertex_blocks(false));
+  ctx->nb_vertex_c = (int)std::min<int64_t>(ntiles, vertex_blocks(true));
   ctx->nb_col = (int)std::min<int64_t>((n + 255) / 256, 1024);
   ctx->nb_src = std::max(1, ac::walk_blocks(ctx->nruns_out, Qs));
   const int nb_l = std::max(ctx->nb_col, ctx->nb_src);
@@ -562,9 +598,10 @@ static int ensure_work(ac_ctx* ctx) {
   HIPCHK(ctx, ctx->VR.alloc((size_t)(n * ctx->VRs)));
   HIPCHK(ctx, ctx->OUT.alloc((size_t)(n * P)));
   HIPCHK(ctx, ctx->carry_in.alloc((size_t)std::max(1, ctx->nslots_in) * Qs));
+  HIPCHK(ctx, ctx->fix_part.alloc((size_t)std::max(1, ctx->nfix_parts) * Qs));
   HIPCHK(ctx, ctx->carry_out.alloc((size_t)std::max(1, ctx->nslots_out) * Qs * 2));
   HIPCHK(ctx, ctx->qpart.alloc((size_t)ctx->nb_vertex * std::max(1, Q)));
-  HIPCHK(ctx, ctx->ppart.alloc((size_t)ctx->nb_vertex * P));
+  HIPCHK(ctx, ctx->ppart.alloc((size_t)ctx->nb_vertex_c * P));
   HIPCHK(ctx, ctx->lpart.alloc((size_t)nb_l * std::max(P, std::max(Q, 1))));
   HIPCHK(ctx, ctx->qstat.alloc((size_t)3 * std::max(1, Q)));
   HIPCHK(ctx, ctx->pstat.alloc((size_t)P));
@@ -643,6 +680,12 @@ static int ensure_tape(ac_ctx* ctx, int n_iter) {
   return AC_OK;
 }
 
+// Combines the split vertices' pieces of a plain (non-recording) walk.
+static void launch_fixup32(ac_ctx* ctx, hipStream_t st) {
+  ac::launch_walk_max_fixup32(ctx->nfix_chunks, ctx->fix_chunks.p, ctx->nfix_multi, ctx->fix_multi.p, ctx->carry_in.p,
+                              ctx->Qs, ctx->fix_part.p, ctx->FMaxN.p, st);
+}
+
 // Prep (src/main.cu:107-148), between events ev[0] and ev[1].
 static int enqueue_prep(ac_ctx* ctx) {
   hipStream_t st = ctx->stream;
@@ -689,8 +732,11 @@ static int enqueue_prep(ac_ctx* ctx) {
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, nullptr, ctx->VRs, Q, Qs, qs_prep, ctx->FMaxN.p, ctx->carry_in.p,
                         walk_batch_of(ctx), st, tape ? ctx->FA0.p : nullptr, tape ? ctx->carg_in.p : nullptr);
-    ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_prep, ctx->FMaxN.p, st,
-                              tape ? ctx->carg_in.p : nullptr, tape ? ctx->FA0.p : nullptr);
+    if (tape)
+      ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_prep, ctx->FMaxN.p, st,
+                                ctx->carg_in.p, ctx->FA0.p);
+    else
+      launch_fixup32(ctx, st);
     if (tape) {
       // Rows of the minimum-cost out-edge of every (v, q) (the RMax argmax), for the adjoint.
       ac::launch_fill_int(ctx->AO.p, n * Qs, INT32_MAX, st);
@@ -767,15 +813,15 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p,
                         walk_batch_of(ctx), st, nullptr, nullptr, ctx->prune ? ctx->bmin.p : nullptr);
     MARK(AC_STAGE_EDGE_WALK);
-    ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_new, ctx->FMaxN.p, st);
+    launch_fixup32(ctx, st);
     MARK(AC_STAGE_WALK_FIXUP);
     qs_state = qs_new;
   }
   va.first = 0;
   va.qs = qs_state;
-  ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex, st);
+  ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex_c, st);
   MARK(AC_STAGE_VERTEX_C);
-  ac::launch_reduce_lse_iso(ctx->nb_vertex, P, ctx->ppart.p, ctx->pstat.p, iso_args(ctx, qs_state, 0), st);
+  ac::launch_reduce_lse_iso(ctx->nb_vertex_c, P, ctx->ppart.p, ctx->pstat.p, iso_args(ctx, qs_state, 0), st);
   MARK(AC_STAGE_REDUCE_C);
   return AC_OK;
 }
@@ -1334,8 +1380,8 @@ static void recompute_iteration(ac_ctx* ctx, int it, const QStat* qs_in, QStat* 
   }
   va.first = 0;
   va.qs = Q > 0 ? qs_out : qs_in;
-  ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex, st);
-  ac::launch_reduce_lse_iso(ctx->nb_vertex, P, ctx->ppart.p, ctx->pstat.p, iso_args(ctx, va.qs, 0), st);
+  ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex_c, st);
+  ac::launch_reduce_lse_iso(ctx->nb_vertex_c, P, ctx->ppart.p, ctx->pstat.p, iso_args(ctx, va.qs, 0), st);
 }
 
 // Host side of the weight gradient from the outer-product sums A, B (K x dim) and C (K) and the pattern features

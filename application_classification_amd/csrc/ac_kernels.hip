@@ -757,9 +757,11 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
       if (q < Q && (double)bmin[b * Qs + q] < qs[q].cthr) live = true;
     }
     if (!__any(live)) {
-      if (slot >= 0) {
+      if (slot >= 0 && kArg) {
         double* o = carry + (int64_t)slot * Qs + 4 * j;
         o[0] = o[1] = o[2] = o[3] = -kInf;
+      } else if (slot >= 0) {
+        reinterpret_cast<f4v*>(carry)[(int64_t)slot * L4 + j] = f4v{-INFINITY, -INFINITY, -INFINITY, -INFINITY};
       } else {
         const int v0 = bdst[Bd.base + g], v1 = bdst[Bd.base + (int64_t)(Bd.len - 1) * G + g];
         const float4 ninf = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
@@ -856,13 +858,18 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
 #undef AC_WK_SRC
 #undef AC_WK_BATCH
 #undef AC_WK_REDUCE
-  if (slot >= 0) {
+  if (slot >= 0 && kArg) {
     double* o = carry + (int64_t)slot * Qs + 4 * j;
     o[0] = mx0;
     o[1] = mx1;
     o[2] = mx2;
     o[3] = mx3;
-    if (kArg) reinterpret_cast<int4*>(carg)[(int64_t)slot * L4 + j] = make_int4(a0, a1, a2, a3);
+    reinterpret_cast<int4*>(carg)[(int64_t)slot * L4 + j] = make_int4(a0, a1, a2, a3);
+  } else if (slot >= 0) {
+    // A piece of a split vertex: its maximum, normalised and rounded like a finished row.  Rounding is monotone,
+    // so the maximum over the pieces of the rounded values is the rounded maximum (exact).
+    reinterpret_cast<f4v*>(carry)[(int64_t)slot * L4 + j] = f4v{(float)(mx0 - sl[0]), (float)(mx1 - sl[1]),
+                                                                (float)(mx2 - sl[2]), (float)(mx3 - sl[3])};
   } else if (cur >= 0) {
     emit(cur);
   }
@@ -1056,6 +1063,80 @@ __global__ __launch_bounds__(256) void k_walk_max_fixup(int nsplit, const Split*
     reinterpret_cast<float4*>(FMaxN)[(int64_t)S.v * L4 + j] = out;
     if (carg) reinterpret_cast<int4*>(FA)[(int64_t)S.v * L4 + j] = make_int4(oarg[0], oarg[1], oarg[2], oarg[3]);
   }
+}
+
+// Plain path: the pieces hold normalised FP32 maxima, and their maximum is exact in any order.  Pieces are grouped
+// into chunks of at most kFixChunk (Split{v, first piece, count, out}); one group of Qs/4 lanes takes a chunk with
+// all its loads in flight and writes the vertex's row (out < 0) or chunk partial `out`; vertices of several chunks
+// (MSplit{v, first partial, count}) are then finished from their partials, one wave each.
+__global__ __launch_bounds__(256) void k_fix_chunks(int nchunk, const Split* __restrict__ chunks,
+                                                    const f4v* __restrict__ carry, int Qs, f4v* __restrict__ part,
+                                                    f4v* __restrict__ FMaxN) {
+  const int lane = threadIdx.x & 63;
+  const int L4 = Qs / 4, G = 64 / L4;
+  const int g = lane / L4, j = lane - g * L4;
+  if (g >= G) return;
+  const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x / 64) * G;
+  for (int64_t ci = ((int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * G + g; ci < nchunk;
+       ci += ngroups) {
+    const Split C = chunks[ci];
+    const f4v* c = carry + (int64_t)C.first * L4 + j;
+    f4v x[kFixChunk];
+#pragma unroll
+    for (int k = 0; k < kFixChunk; ++k)
+      x[k] = k < C.count ? c[(int64_t)k * L4] : f4v{-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    f4v m = x[0];
+#pragma unroll
+    for (int k = 1; k < kFixChunk; ++k)
+      m = f4v{fmaxf(m.x, x[k].x), fmaxf(m.y, x[k].y), fmaxf(m.z, x[k].z), fmaxf(m.w, x[k].w)};
+    if (C.pad < 0)
+      FMaxN[(int64_t)C.v * L4 + j] = m;
+    else
+      part[(int64_t)C.pad * L4 + j] = m;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fix_multi(int nms, const Split* __restrict__ ms, const f4v* __restrict__ part,
+                                                   int Qs, f4v* __restrict__ FMaxN) {
+  const int lane = threadIdx.x & 63;
+  const int L4 = Qs / 4, G = 64 / L4;
+  const int g = lane / L4, j = lane - g * L4;
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (w >= nms) return;
+  const Split M = ms[w];
+  f4v m0 = {-INFINITY, -INFINITY, -INFINITY, -INFINITY}, m1 = m0;
+  if (g < G) {
+    int c = g;
+    for (; c + G < M.count; c += 2 * G) {
+      const f4v a = part[(int64_t)(M.first + c) * L4 + j], b = part[(int64_t)(M.first + c + G) * L4 + j];
+      m0 = f4v{fmaxf(m0.x, a.x), fmaxf(m0.y, a.y), fmaxf(m0.z, a.z), fmaxf(m0.w, a.w)};
+      m1 = f4v{fmaxf(m1.x, b.x), fmaxf(m1.y, b.y), fmaxf(m1.z, b.z), fmaxf(m1.w, b.w)};
+    }
+    if (c < M.count) {
+      const f4v a = part[(int64_t)(M.first + c) * L4 + j];
+      m0 = f4v{fmaxf(m0.x, a.x), fmaxf(m0.y, a.y), fmaxf(m0.z, a.z), fmaxf(m0.w, a.w)};
+    }
+  }
+  f4v m = {fmaxf(m0.x, m1.x), fmaxf(m0.y, m1.y), fmaxf(m0.z, m1.z), fmaxf(m0.w, m1.w)};
+  // Across the groups: lane (0, j) gathers lanes (gg, j).
+  for (int gg = 1; gg < G; ++gg) {
+    const int src = gg * L4 + j;
+    const f4v o = {__shfl(m.x, src, 64), __shfl(m.y, src, 64), __shfl(m.z, src, 64), __shfl(m.w, src, 64)};
+    m = f4v{fmaxf(m.x, o.x), fmaxf(m.y, o.y), fmaxf(m.z, o.z), fmaxf(m.w, o.w)};
+  }
+  if (g == 0) FMaxN[(int64_t)M.v * L4 + j] = m;
+}
+
+void launch_walk_max_fixup32(int nchunk, const Split* chunks, int nms, const Split* ms, const double* carry, int Qs,
+                             float* part, float* FMaxN, hipStream_t st) {
+  if (nchunk == 0) return;
+  const int G = 64 / (Qs / 4);
+  const int blocks = (int)std::min<int64_t>(((int64_t)nchunk + 4 * G - 1) / (4 * G), 8192);
+  hipLaunchKernelGGL(k_fix_chunks, dim3(blocks), dim3(256), 0, st, nchunk, chunks, reinterpret_cast<const f4v*>(carry),
+                     Qs, reinterpret_cast<f4v*>(part), reinterpret_cast<f4v*>(FMaxN));
+  if (nms > 0)
+    hipLaunchKernelGGL(k_fix_multi, dim3((nms + 3) / 4), dim3(256), 0, st, nms, ms, reinterpret_cast<const f4v*>(part),
+                       Qs, reinterpret_cast<f4v*>(FMaxN));
 }
 
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
