@@ -1206,6 +1206,11 @@ __device__ __forceinline__ Lse wave_lse_merge(Lse s) {
   return s;
 }
 
+// Read-only inputs seen through the constant address space: loads at wave-uniform indexes become scalar loads, and
+// branches on their values scalar branches.
+typedef const __attribute__((address_space(4))) int cint_t;
+typedef const __attribute__((address_space(4))) QStat cqstat_t;
+
 // kPW > 0 (mode C with P <= 4 * kPW): each lane accumulates its rows' log-sum-exp of its wave's kPW pattern vertices
 // in registers over all tiles, merged across the wave once at the end.  kPW == 0: one wave-level reduction per tile.
 template <int kMode, int kPW>
@@ -1226,8 +1231,11 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
   float* Cs = Ls + (kA ? VT * ldq : 0);                // [VT][ldp] CV
   int* incs = reinterpret_cast<int*>(Cs + VT * ldp);   // [2Q]
   int* incoff = incs + 2 * Q;                          // [P + 1]
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const bool sums = !a.first;
+  cint_t* const c_inc = (cint_t*)a.inc;
+  cint_t* const c_incoff = (cint_t*)a.inc_off;
+  cqstat_t* const c_qs = (cqstat_t*)a.qs;
 
   for (int i = t; i < 2 * Q; i += kVertexBlock) incs[i] = a.inc[i];
   for (int i = t; i <= P; i += kVertexBlock) incoff[i] = a.inc_off[i];
@@ -1325,19 +1333,21 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
       X = -kInf;                                                                                           \
       if ((R_) < nv) {                                                                                     \
         double acc_ = -(double)Cs[(R_) * ldp + (P_)];                                                      \
-        if (sums)                                                                                          \
-          for (int k_ = incoff[P_]; k_ < incoff[(P_) + 1]; ++k_) {                                        \
-            const int code_ = incs[k_];                                                                    \
+        if (sums) {                                                                                        \
+          const int k1_ = c_incoff[(P_) + 1];                                                              \
+          for (int k_ = c_incoff[P_]; k_ < k1_; ++k_) {                                                    \
+            const int code_ = c_inc[k_];                                                                   \
             const int qq_ = code_ >> 1;                                                                    \
             if (code_ & 1) {                                                                               \
               const float vfm_ = Vs[(R_) * ldq + qq_];                                                     \
               const double rn_ =                                                                           \
-                  vfm_ == -INFINITY ? -kInf : ((double)vfm_ - tab[2 * Q + qq_]) - tab[3 * Q + qq_];        \
-              acc_ += fmax(tab[Q + qq_], rn_);                                                             \
+                  vfm_ == -INFINITY ? -kInf : ((double)vfm_ - c_qs[qq_].mRE) - c_qs[qq_].lRE;              \
+              acc_ += fmax(c_qs[qq_].fillR, rn_);                                                          \
             } else {                                                                                       \
-              acc_ += fmax(tab[qq_], (double)Fs[(R_) * ldq + qq_]);                                        \
+              acc_ += fmax(c_qs[qq_].fillF, (double)Fs[(R_) * ldq + qq_]);                                 \
             }                                                                                              \
           }                                                                                                \
+        }                                                                                                  \
         X = acc_;                                                                                          \
       }                                                                                                    \
     }
