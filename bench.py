@@ -107,11 +107,11 @@ def summarize_profile(src, workload, out_dir):
     lines = [f"# Kernel profile: {workload}", "",
              "Commands (run from the repository root on one MI355X):", "",
              "    rocprofv3 --kernel-trace --stats --output-format csv -d <dir>/trace -o s22 -- "
-             "python -u bench.py --steps 2 --warmup 1 --no-cpu-baseline",
+             "python -u bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-pruned",
              "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/fetch -o s22 -- "
-             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile",
+             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile --no-pruned",
              "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir>/write -o s22 -- "
-             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile",
+             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile --no-pruned",
              "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/cfetch -o copy -- python -u bench.py --copy-calibration",
              "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir>/cwrite -o copy -- python -u bench.py --copy-calibration",
              f"    python bench.py --summarize-profile <dir>", "",
@@ -325,6 +325,8 @@ def main():
     ap.add_argument("--config", default="s22", choices=["s22", "s18", "small", "batched", "diff"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-pruned", action="store_true",
+                    help="skip the pruned run (profiled commands then hold the headline launches only)")
     ap.add_argument("--summarize-profile", metavar="DIR",
                     help="summarise rocprofv3 outputs under DIR into profiles/ and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -433,22 +435,24 @@ def main():
 
     # Exact pruning (AC_OPT_PRUNE, the library default): the same run skipping edge bundles that provably cannot
     # raise FMax above the fill; the output must be bit-identical.
-    m.set_option(AC_OPT_PRUNE, 1)
-    m.run_async()
-    m.sync()
-    pr_loop = pr_prep = 0.0
-    for i in range(args.steps):
+    pruned = None
+    if not args.no_pruned:
+        m.set_option(AC_OPT_PRUNE, 1)
         m.run_async()
         m.sync()
-        a, b = m.timing()
-        pr_prep += a
-        pr_loop += b
-    pr_loop /= args.steps
-    pr_prep /= args.steps
-    pruned = {"value": updates / (pr_loop * 1e-3), "loop_ms_per_step": pr_loop, "prep_ms_per_step": pr_prep,
-              "speedup_loop": loop_ms / pr_loop, "bit_identical": bool(np.array_equal(m.mu(), mu))}
-    log(f"pruned: loop {pr_loop:.3f} ms ({pruned['speedup_loop']:.2f}x), identical: {pruned['bit_identical']}")
-    m.set_option(AC_OPT_PRUNE, 0)
+        pr_loop = pr_prep = 0.0
+        for i in range(args.steps):
+            m.run_async()
+            m.sync()
+            a, b = m.timing()
+            pr_prep += a
+            pr_loop += b
+        pr_loop /= args.steps
+        pr_prep /= args.steps
+        pruned = {"value": updates / (pr_loop * 1e-3), "loop_ms_per_step": pr_loop, "prep_ms_per_step": pr_prep,
+                  "speedup_loop": loop_ms / pr_loop, "bit_identical": bool(np.array_equal(m.mu(), mu))}
+        log(f"pruned: loop {pr_loop:.3f} ms ({pruned['speedup_loop']:.2f}x), identical: {pruned['bit_identical']}")
+        m.set_option(AC_OPT_PRUNE, 0)
 
     cpu = None
     if not args.no_cpu_baseline:
