@@ -460,6 +460,117 @@ __global__ __launch_bounds__(256) void k_outer_acc(int64_t nrows, const int* __r
     }
 }
 
+// The same sums on FP64 matrix cores, for the (q, k) block [q0, q0 + 16 NQT) x [k0, k0 + 16 NKT):
+// A = G^T (x*x), B = G^T x as GEMMs over the rows with v_mfma_f64_16x16x4f64, four rows per step.  A-operand lane
+// i + 16k holds G[row k][q0 + i], B-operand lane j + 16k holds x[row k][k0 + j] (or its square), and D[4r +
+// lane/16][lane%16] sits in register r.  Products of floats are exact in FP64; the sums are FP64.  The block's four
+// waves are summed in order.
+typedef double d4v __attribute__((ext_vector_type(4)));
+template <int NQT, int NKT>
+__global__ __launch_bounds__(256) void k_outer_mfma(int64_t nrows, const int* __restrict__ idx,
+                                                    const float* __restrict__ feats, int dim,
+                                                    const float* __restrict__ G, int Gs, int K, int q0, int k0,
+                                                    double* __restrict__ part) {
+  __shared__ double red[3][64][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = lane & 15, kr = lane >> 4;
+  d4v accA[NQT][NKT], accB[NQT][NKT];
+#pragma unroll
+  for (int a = 0; a < NQT; ++a)
+#pragma unroll
+    for (int b = 0; b < NKT; ++b) accA[a][b] = accB[a][b] = d4v{0.0, 0.0, 0.0, 0.0};
+  double cs[NQT];
+#pragma unroll
+  for (int a = 0; a < NQT; ++a) cs[a] = 0.0;
+  // Steps of kOS x 4 rows, software-pipelined: the raw values of the next step are loaded while the current one
+  // runs on the matrix cores.
+  constexpr int kOS = 4;
+  const int64_t stride = (int64_t)gridDim.x * 4 * (4 * kOS);
+  float gf[kOS][NQT], xf[kOS][NKT];
+  auto load = [&](int64_t r0) {
+#pragma unroll
+    for (int h = 0; h < kOS; ++h) {
+      const int64_t row = r0 + 4 * h + kr;
+      int64_t src = -1;
+      if (row < nrows) src = idx ? (int64_t)idx[row] : row;
+#pragma unroll
+      for (int a = 0; a < NQT; ++a) {
+        const int q = q0 + 16 * a + i;
+        gf[h][a] = (src >= 0 && q < K) ? G[row * Gs + q] : 0.f;
+      }
+#pragma unroll
+      for (int b = 0; b < NKT; ++b) {
+        const int k = k0 + 16 * b + i;
+        xf[h][b] = (src >= 0 && k < dim) ? feats[src * dim + k] : 0.f;
+      }
+    }
+  };
+  int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * (4 * kOS);
+  if (r0 < nrows) load(r0);
+  for (; r0 < nrows; r0 += stride) {
+    double g[kOS][NQT], x[kOS][NKT];
+#pragma unroll
+    for (int h = 0; h < kOS; ++h) {
+#pragma unroll
+      for (int a = 0; a < NQT; ++a) g[h][a] = gf[h][a];
+#pragma unroll
+      for (int b = 0; b < NKT; ++b) x[h][b] = xf[h][b];
+    }
+    if (r0 + stride < nrows) load(r0 + stride);
+#pragma unroll
+    for (int h = 0; h < kOS; ++h)
+#pragma unroll
+      for (int a = 0; a < NQT; ++a) {
+        cs[a] += g[h][a];
+#pragma unroll
+        for (int b = 0; b < NKT; ++b) {
+          accB[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[h][a], x[h][b], accB[a][b], 0, 0, 0);
+          accA[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[h][a], x[h][b] * x[h][b], accA[a][b], 0, 0, 0);
+        }
+      }
+  }
+  // C: the four row slots of each column, then the waves.
+#pragma unroll
+  for (int a = 0; a < NQT; ++a) {
+    cs[a] += __shfl_xor(cs[a], 16, 64);
+    cs[a] += __shfl_xor(cs[a], 32, 64);
+  }
+  double* out = part + (int64_t)blockIdx.x * (2 * (int64_t)K * dim + K);
+  auto block_sum = [&](d4v v) {
+    __syncthreads();
+    if (wave > 0)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[wave - 1][lane][r] = v[r];
+    __syncthreads();
+    if (wave == 0)
+      for (int w = 0; w < 3; ++w)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] += red[w][lane][r];
+    return v;
+  };
+#pragma unroll
+  for (int a = 0; a < NQT; ++a)
+#pragma unroll
+    for (int b = 0; b < NKT; ++b)
+#pragma unroll
+      for (int which = 0; which < 2; ++which) {
+        const d4v v = block_sum(which ? accB[a][b] : accA[a][b]);
+        if (wave == 0)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int q = q0 + 16 * a + 4 * r + kr, k = k0 + 16 * b + i;
+            if (q < K && k < dim) out[(which ? (int64_t)K * dim : 0) + (int64_t)q * dim + k] = v[r];
+          }
+      }
+  if (k0 == 0)
+#pragma unroll
+    for (int a = 0; a < NQT; ++a) {
+      const d4v v = block_sum(d4v{cs[a], 0.0, 0.0, 0.0});
+      const int q = q0 + 16 * a + i;
+      if (wave == 0 && kr == 0 && q < K) out[2 * (int64_t)K * dim + q] = v[0];
+    }
+}
+
 __global__ void k_fill_int(int* __restrict__ p, int64_t count, int value) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = value;
@@ -553,7 +664,11 @@ void launch_bwd_vertex_coef(int64_t total, const float* CV, float* G, hipStream_
 
 void launch_outer_acc(int64_t nrows, const int* idx, const float* feats, int dim, const float* G, int Gs, int K,
                       double* part, int nparts, hipStream_t st) {
-  hipLaunchKernelGGL(k_outer_acc, dim3(nparts), dim3(256), 0, st, nrows, idx, feats, dim, G, Gs, K, part);
+  // Matrix-core path in (q, k) blocks of 32 x 32 (each block of the grid covers every row once per launch).
+  for (int q0 = 0; q0 < K; q0 += 32)
+    for (int k0 = 0; k0 < dim; k0 += 32)
+      hipLaunchKernelGGL((k_outer_mfma<2, 2>), dim3(nparts), dim3(256), 0, st, nrows, idx, feats, dim, G, Gs, K, q0,
+                         k0, part);
 }
 
 }  // namespace ac

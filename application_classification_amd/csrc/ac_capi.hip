@@ -583,8 +583,11 @@ static int ensure_work(ac_ctx* ctx) {
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
   ctx->VT = ac::vertex_tile(P, Q);
   const int64_t ntiles = (std::max<int64_t>(ctx->n_act, 1) + ctx->VT - 1) / ctx->VT;
-  ctx->nb_vertex = (int)std::min<int64_t>(ntiles, vertex_blocks(false));
-  ctx->nb_vertex_c = (int)std::min<int64_t>(ntiles, vertex_blocks(true));
+  // A few tiles per block at least, so that small graphs do not pay for merging many near-empty partials.
+  ctx->nb_vertex = (int)std::min<int64_t>(std::max<int64_t>(std::min<int64_t>(ntiles, 512), ntiles / 4),
+                                          vertex_blocks(false));
+  ctx->nb_vertex_c = (int)std::min<int64_t>(std::max<int64_t>(std::min<int64_t>(ntiles, 512), ntiles / 8),
+                                            vertex_blocks(true));
   ctx->nb_col = (int)std::min<int64_t>((n + 255) / 256, 1024);
   ctx->nb_src = std::max(1, ac::walk_blocks(ctx->nruns_out, Qs));
   const int nb_l = std::max(ctx->nb_col, ctx->nb_src);
@@ -600,8 +603,8 @@ static int ensure_work(ac_ctx* ctx) {
   HIPCHK(ctx, ctx->carry_in.alloc((size_t)std::max(1, ctx->nslots_in) * Qs));
   HIPCHK(ctx, ctx->fix_part.alloc((size_t)std::max(1, ctx->nfix_parts) * Qs));
   HIPCHK(ctx, ctx->carry_out.alloc((size_t)std::max(1, ctx->nslots_out) * Qs * 2));
-  HIPCHK(ctx, ctx->qpart.alloc((size_t)ctx->nb_vertex * std::max(1, Q)));
-  HIPCHK(ctx, ctx->ppart.alloc((size_t)ctx->nb_vertex_c * P));
+  HIPCHK(ctx, ctx->qpart.alloc((size_t)(ctx->nb_vertex + 1) * std::max(1, Q)));
+  HIPCHK(ctx, ctx->ppart.alloc((size_t)(ctx->nb_vertex_c + 1) * P));
   HIPCHK(ctx, ctx->lpart.alloc((size_t)nb_l * std::max(P, std::max(Q, 1))));
   HIPCHK(ctx, ctx->qstat.alloc((size_t)3 * std::max(1, Q)));
   HIPCHK(ctx, ctx->pstat.alloc((size_t)P));
@@ -754,25 +757,11 @@ static int enqueue_prep(ac_ctx* ctx) {
 }
 
 
-// The closed-form share of the isolated vertices for the reductions after a vertex pass that read state `qs`.
-static ac::IsoArgs iso_args(ac_ctx* ctx, const QStat* qs, int first) {
-  ac::IsoArgs a{};
-  a.col = ctx->n_iso > 0 ? ctx->iso.p : nullptr;
-  a.inc_off = ctx->d_inc_off.p;
-  a.inc = ctx->d_inc.p;
-  a.ps = ctx->d_ps.p;
-  a.pd = ctx->d_pd.p;
-  a.qs = qs;
-  a.pst = ctx->pstat.p;
-  a.P = ctx->P;
-  a.first = first;
-  return a;
-}
-
 static ac::VertexArgs vertex_args(ac_ctx* ctx) {
   ac::VertexArgs va{};
   va.n = ctx->n_act;
   va.perm = ctx->d_perm.p;
+  va.iso = ctx->n_iso > 0 ? ctx->iso.p : nullptr;
   va.P = ctx->P;
   va.Q = ctx->Q;
   va.Qs = ctx->Qs;
@@ -806,8 +795,7 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
     va.qs = qs_state;
     ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
     MARK(AC_STAGE_VERTEX_A);
-    const ac::IsoArgs isoA = iso_args(ctx, qs_state, va.first);
-    ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_new, st, &isoA);
+    ac::launch_reduce_q(ctx->nb_vertex + 1, Q, ctx->qpart.p, qs_new, st);
     MARK(AC_STAGE_REDUCE_A);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p,
@@ -821,7 +809,7 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
   va.qs = qs_state;
   ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex_c, st);
   MARK(AC_STAGE_VERTEX_C);
-  ac::launch_reduce_lse_iso(ctx->nb_vertex_c, P, ctx->ppart.p, ctx->pstat.p, iso_args(ctx, qs_state, 0), st);
+  ac::launch_reduce_lse(ctx->nb_vertex_c + 1, P, ctx->ppart.p, ctx->pstat.p, st);
   MARK(AC_STAGE_REDUCE_C);
   return AC_OK;
 }
@@ -1370,8 +1358,7 @@ static void recompute_iteration(ac_ctx* ctx, int it, const QStat* qs_in, QStat* 
     va.first = it == 0;
     va.qs = qs_in;
     ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
-    const ac::IsoArgs isoA = iso_args(ctx, qs_in, va.first);
-    ac::launch_reduce_q(ctx->nb_vertex, Q, ctx->qpart.p, qs_out, st, &isoA);
+    ac::launch_reduce_q(ctx->nb_vertex + 1, Q, ctx->qpart.p, qs_out, st);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_out, ctx->FMaxN.p, ctx->carry_in.p,
                         walk_batch_of(ctx), st, FA, ctx->carg_in.p, ctx->prune ? ctx->bmin.p : nullptr);
@@ -1381,7 +1368,7 @@ static void recompute_iteration(ac_ctx* ctx, int it, const QStat* qs_in, QStat* 
   va.first = 0;
   va.qs = Q > 0 ? qs_out : qs_in;
   ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex_c, st);
-  ac::launch_reduce_lse_iso(ctx->nb_vertex_c, P, ctx->ppart.p, ctx->pstat.p, iso_args(ctx, va.qs, 0), st);
+  ac::launch_reduce_lse(ctx->nb_vertex_c + 1, P, ctx->ppart.p, ctx->pstat.p, st);
 }
 
 // Host side of the weight gradient from the outer-product sums A, B (K x dim) and C (K) and the pattern features

@@ -52,19 +52,6 @@ struct IsoCol {
   int vmin;  // -1: no isolated vertex
 };
 
-// What the reductions need to add the isolated vertices' share in closed form (col == nullptr: none).
-struct IsoArgs {
-  const IsoCol* col;
-  const int* inc_off;
-  const int* inc;
-  const int* ps;
-  const int* pd;
-  const QStat* qs;   // scalars of the state the vertex pass read
-  const Lse2* pst;   // its MU column log-sum-exps (pass A only)
-  int P;
-  int first;         // the prep state: no FMax / RMax terms
-};
-
 // A contiguous run of edges in CSC or CSR order covering whole vertices, or one piece of a vertex whose edge list
 // is longer than a run (slot >= 0 then names its carry slot).
 struct Run {
@@ -96,14 +83,16 @@ struct VertexArgs {
   int VRs;              // row stride of VR: a power of two up to 32 floats, else a multiple of 32 (whole lines)
   float* out;           // [n][P]
   const int* perm;      // mode OUT: internal row -> caller's row of `out` (nullptr: identity)
+  const IsoCol* iso;    // [P] isolated vertices' columns (nullptr: none); modes A and C write their share as the
+                        // extra partial slot gridDim.x
   const int* ps;
   const int* pd;
   const int* inc_off;  // [P + 1]
   const int* inc;      // incidence of each pattern vertex: (q << 1) | (1 if RMax term else 0), ascending q
   const QStat* qs;     // scalars that finish FMax / RMax of the state being read
   const Lse2* pst;     // log-sum-exp of each MU column
-  QPart* qpart;        // [gridDim][Q]
-  Lse* ppart;          // [gridDim][P]
+  QPart* qpart;        // [gridDim + 1][Q]
+  Lse* ppart;          // [gridDim + 1][P]
   int first;           // MU = LSM(-CV): the prep state, no FMax / RMax terms
 };
 
@@ -116,8 +105,7 @@ void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, co
 size_t cdist_scratch_doubles(int Kstore, int dim);
 void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks, hipStream_t st);
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st);
-void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st, const IsoArgs* iso = nullptr);
-void launch_reduce_lse_iso(int nparts, int P, const Lse* part, Lse2* out, const IsoArgs& iso, hipStream_t st);
+void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st);
 // Per column p of the n rows X[n][P]: IsoCol{LSE of -X, min X and its lowest row + base}; partial buffers of
 // nparts * P entries each.
 void launch_iso_cols(int64_t n, int P, const float* X, int base, Lse* lpart, float* pv, int* pi, int nparts,

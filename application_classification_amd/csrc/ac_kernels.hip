@@ -379,11 +379,9 @@ void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks
   hipLaunchKernelGGL(k_col_lse_neg, dim3(nblocks), dim3(256), 0, st, n, P, X, part);
 }
 
-// Reduce per-block partials [nparts][ncol] column by column in a fixed order (deterministic).  With iso.col, column
-// p also receives the isolated vertices' share of the MU column log-sum-exp (merged last): their MU_pre is
-// -CV + K_p, K_p the sum of the fill values of p's incidences, so the share is LSE(-CV) + K_p.
+// Reduce per-block partials [nparts][ncol] column by column in a fixed order (deterministic).
 __global__ __launch_bounds__(256) void k_reduce_lse(int nparts, int ncol, const Lse* __restrict__ part,
-                                                    Lse2* __restrict__ out, IsoArgs iso) {
+                                                    Lse2* __restrict__ out) {
   __shared__ Lse red[256];
   const int c = blockIdx.x, t = threadIdx.x;
   Lse s = lse_empty();
@@ -394,47 +392,16 @@ __global__ __launch_bounds__(256) void k_reduce_lse(int nparts, int ncol, const 
     if (t < w) red[t] = lse_merge(red[t], red[t + w]);
     __syncthreads();
   }
-  if (t == 0) {
-    Lse r = red[0];
-    if (iso.col && iso.col[c].vmin >= 0) {
-      double K = 0.0;
-      if (!iso.first)
-        for (int k = iso.inc_off[c]; k < iso.inc_off[c + 1]; ++k) {
-          const int code = iso.inc[k], qq = code >> 1;
-          K += (code & 1) ? fmax(iso.qs[qq].fillR, -kInf) : fmax(iso.qs[qq].fillF, -kInf);
-        }
-      const double m = iso.col[c].m + K;
-      if (m > -kInf) r = lse_merge(r, Lse{m, iso.col[c].t});
-    }
-    out[c] = Lse2{r.m, log1p(r.t)};
-  }
+  if (t == 0) out[c] = Lse2{red[0].m, log1p(red[0].t)};
 }
 
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st) {
   if (ncol == 0) return;
-  IsoArgs none{};
-  hipLaunchKernelGGL(k_reduce_lse, dim3(ncol), dim3(256), 0, st, nparts, ncol, part, out, none);
-}
-
-void launch_reduce_lse_iso(int nparts, int P, const Lse* part, Lse2* out, const IsoArgs& iso, hipStream_t st) {
-  if (P == 0) return;
-  hipLaunchKernelGGL(k_reduce_lse, dim3(P), dim3(256), 0, st, nparts, P, part, out, iso);
-}
-
-// MU of pattern vertex p at the isolated vertex of least cost: the largest MU over the isolated vertices (MU is
-// monotone in -CV), evaluated with the vertex pass's own operation sequence (FMaxN = VFm = -inf rows).
-__device__ double iso_mu_max(const IsoArgs& iso, int p) {
-  double acc = -(double)iso.col[p].cmin;
-  if (!iso.first)
-    for (int k = iso.inc_off[p]; k < iso.inc_off[p + 1]; ++k) {
-      const int code = iso.inc[k], qq = code >> 1;
-      acc += (code & 1) ? fmax(iso.qs[qq].fillR, -kInf) : fmax(iso.qs[qq].fillF, (double)(-INFINITY));
-    }
-  return (acc - iso.pst[p].m) - iso.pst[p].l;
+  hipLaunchKernelGGL(k_reduce_lse, dim3(ncol), dim3(256), 0, st, nparts, ncol, part, out);
 }
 
 __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart* __restrict__ part,
-                                                  QStat* __restrict__ out, IsoArgs iso) {
+                                                  QStat* __restrict__ out) {
   __shared__ QPart red[256];
   const int q = blockIdx.x, t = threadIdx.x;
   QPart s{-kInf, -kInf, lse_empty(), lse_empty(), -1, -1};
@@ -450,19 +417,7 @@ __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart
     __syncthreads();
   }
   if (t == 0) {
-    QPart r = red[0];
-    if (iso.col) {
-      // The isolated vertices' VF = MU[pd] - fillF and VR = MU[ps] - fillR peak at their least-cost vertex; they
-      // have no out-edges, so they add nothing to the edge log-sum-exps.
-      const int pd = iso.pd[q], ps = iso.ps[q];
-      const double fm = fmax(iso.qs[q].fillF, (double)(-INFINITY)), rm = fmax(iso.qs[q].fillR, -kInf);
-      const double vf = iso.col[pd].vmin >= 0 ? iso_mu_max(iso, pd) - fm : -kInf;
-      const double vr = iso.col[ps].vmin >= 0 ? iso_mu_max(iso, ps) - rm : -kInf;
-      // As in the vertex pass, a -inf value records no vertex.
-      const QPart w{vf, vr, lse_empty(), lse_empty(), vf > -kInf ? iso.col[pd].vmin : -1,
-                    vr > -kInf ? iso.col[ps].vmin : -1};
-      qpart_merge(r, w);
-    }
+    const QPart r = red[0];
     // Edge pruning bound.  An edge term is (VR[src] - CE) - LSE_FE with VR[src] <= fillF (rounded to FP32 once),
     // so it cannot exceed fillF when CE + LSE_FE >= 2^-22 (|fillF| + |CE| + |LSE_FE|); cthr = -LSE_FE plus four
     // times that margin (a cost near the bound has |CE| ~ |LSE_FE|; a larger one clears it by itself).
@@ -474,10 +429,9 @@ __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart
   }
 }
 
-void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st, const IsoArgs* iso) {
+void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st) {
   if (Q == 0) return;
-  IsoArgs none{};
-  hipLaunchKernelGGL(k_reduce_q, dim3(Q), dim3(256), 0, st, nparts, Q, part, out, iso ? *iso : none);
+  hipLaunchKernelGGL(k_reduce_q, dim3(Q), dim3(256), 0, st, nparts, Q, part, out);
 }
 
 // Isolated-vertex columns: one pass over X[n][P] giving per-block partials of LSE(-X) and of (min X, lowest row).
@@ -522,14 +476,18 @@ __global__ __launch_bounds__(256) void k_iso_cols(int64_t n, int P, const float*
   }
 }
 
-__global__ void k_iso_final(int nparts, int P, int base, const Lse* __restrict__ lpart, const float* __restrict__ pv,
-                            const int* __restrict__ pi, IsoCol* __restrict__ out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
+// One block per column: partials merged by a fixed tree (the minimum's lowest row wins ties in any order).
+__global__ __launch_bounds__(256) void k_iso_final(int nparts, int P, int base, const Lse* __restrict__ lpart,
+                                                   const float* __restrict__ pv, const int* __restrict__ pi,
+                                                   IsoCol* __restrict__ out) {
+  __shared__ Lse rs[256];
+  __shared__ float rv[256];
+  __shared__ int ri[256];
+  const int p = blockIdx.x, t = threadIdx.x;
   Lse s = lse_empty();
   float best = INFINITY;
   int bi = INT_MAX;
-  for (int b = 0; b < nparts; ++b) {
+  for (int b = t; b < nparts; b += 256) {
     s = lse_merge(s, lpart[(int64_t)b * P + p]);
     const float x = pv[(int64_t)b * P + p];
     const int xi = pi[(int64_t)b * P + p];
@@ -538,15 +496,30 @@ __global__ void k_iso_final(int nparts, int P, int base, const Lse* __restrict__
       bi = xi;
     }
   }
+  rs[t] = s;
+  rv[t] = best;
+  ri[t] = bi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      rs[t] = lse_merge(rs[t], rs[t + w]);
+      if (rv[t + w] < rv[t] || (rv[t + w] == rv[t] && ri[t + w] < ri[t])) {
+        rv[t] = rv[t + w];
+        ri[t] = ri[t + w];
+      }
+    }
+    __syncthreads();
+  }
   // A column whose isolated costs are all +inf has no finite share: treat it as empty.
-  out[p] = bi == INT_MAX ? IsoCol{-kInf, 0.0, INFINITY, -1} : IsoCol{s.m, s.t, best, base + bi};
+  if (t == 0)
+    out[p] = ri[0] == INT_MAX ? IsoCol{-kInf, 0.0, INFINITY, -1} : IsoCol{rs[0].m, rs[0].t, rv[0], base + ri[0]};
 }
 
 void launch_iso_cols(int64_t n, int P, const float* X, int base, Lse* lpart, float* pv, int* pi, int nparts,
                      IsoCol* out, hipStream_t st) {
   if (P == 0) return;
   hipLaunchKernelGGL(k_iso_cols, dim3(nparts), dim3(256), 0, st, n, P, X, lpart, pv, pi);
-  hipLaunchKernelGGL(k_iso_final, dim3((P + 255) / 256), dim3(256), 0, st, nparts, P, base, lpart, pv, pi, out);
+  hipLaunchKernelGGL(k_iso_final, dim3(P), dim3(256), 0, st, nparts, P, base, lpart, pv, pi, out);
 }
 
 // Scalars of the prep state (src/main.cu:116-146): MU0 = LSM(-CV), RE0 = FE0 = LSM(-CE).  The column maximum of
@@ -1211,6 +1184,43 @@ __device__ __forceinline__ Lse wave_lse_merge(Lse s) {
 typedef const __attribute__((address_space(4))) int cint_t;
 typedef const __attribute__((address_space(4))) QStat cqstat_t;
 
+// The isolated vertices' share, in closed form (block 0 of a vertex pass writes it as one extra partial).  Their
+// FMaxN = VFm = -inf rows make MU_pre = -CV + K_p, K_p the sum of the fills of p's incidences (in the vertex pass's
+// own operation sequence), so MU is largest at the least-cost isolated vertex and the column log-sum-exp is
+// LSE(-CV) + K_p.  They have no out-edges: no edge terms.
+__device__ double iso_kp(const VertexArgs& a, int p, bool sums, double acc) {
+  if (sums)
+    for (int k = a.inc_off[p]; k < a.inc_off[p + 1]; ++k) {
+      const int code = a.inc[k], qq = code >> 1;
+      acc += (code & 1) ? fmax(a.qs[qq].fillR, -kInf) : fmax(a.qs[qq].fillF, (double)(-INFINITY));
+    }
+  return acc;
+}
+
+__device__ QPart iso_qpart(const VertexArgs& a, int q) {
+  QPart w{-kInf, -kInf, lse_empty(), lse_empty(), -1, -1};
+  if (!a.iso) return w;
+  const int pd = a.pd[q], ps = a.ps[q];
+  const double fm = fmax(a.qs[q].fillF, (double)(-INFINITY)), rm = fmax(a.qs[q].fillR, -kInf);
+  if (a.iso[pd].vmin >= 0) {
+    const double mu = (iso_kp(a, pd, !a.first, -(double)a.iso[pd].cmin) - a.pst[pd].m) - a.pst[pd].l;
+    w.vfmax = mu - fm;
+    if (w.vfmax > -kInf) w.avf = a.iso[pd].vmin;  // as in the vertex pass, -inf records no vertex
+  }
+  if (a.iso[ps].vmin >= 0) {
+    const double mu = (iso_kp(a, ps, !a.first, -(double)a.iso[ps].cmin) - a.pst[ps].m) - a.pst[ps].l;
+    w.vrmax = mu - rm;
+    if (w.vrmax > -kInf) w.avr = a.iso[ps].vmin;
+  }
+  return w;
+}
+
+__device__ Lse iso_lse(const VertexArgs& a, int p) {
+  if (!a.iso || a.iso[p].vmin < 0) return lse_empty();
+  const double m = iso_kp(a, p, !a.first, a.iso[p].m);
+  return m > -kInf ? Lse{m, a.iso[p].t} : lse_empty();
+}
+
 // kPW > 0 (mode C with P <= 4 * kPW): each lane accumulates its rows' log-sum-exp of its wave's kPW pattern vertices
 // in registers over all tiles, merged across the wave once at the end.  kPW == 0: one wave-level reduction per tile.
 template <int kMode, int kPW>
@@ -1253,6 +1263,14 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
     } else {
       lacc[i] = lse_empty();
     }
+  }
+
+  // Block 0 also writes the isolated vertices' partial, after the blocks' own ones (slot gridDim.x).
+  if (blockIdx.x == 0) {
+    if (kMode == kModeA)
+      for (int i = t; i < Q; i += kVertexBlock) a.qpart[(int64_t)gridDim.x * Q + i] = iso_qpart(a, i);
+    else if (kMode == kModeC)
+      for (int i = t; i < P; i += kVertexBlock) a.ppart[(int64_t)gridDim.x * P + i] = iso_lse(a, i);
   }
 
   const int RQ = Q > 0 ? kVertexBlock / Q : 0;
