@@ -98,15 +98,6 @@ __global__ __launch_bounds__(256) void k_sum_parts(int nparts, int C, const doub
   }
 }
 
-// MU[v][p] of a state, for every (v, p), into MU[n][P] (FP64).
-__global__ __launch_bounds__(256) void k_mu_state(BwdGraph G, BwdState S, double* __restrict__ MU) {
-  const int64_t total = G.n * G.P;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = i / G.P;
-    MU[i] = mu_of(G, S, v, (int)(i - v * G.P));
-  }
-}
-
 // Log-softmax Jacobian of MU = LSM_v(Z): gMU <- gZ = gMU - exp(MU) colsum(gMU); gCV -= gZ.
 __global__ __launch_bounds__(256) void k_bwd_out(int64_t total, int P, const double* __restrict__ MU,
                                                  const double* __restrict__ colsum, float* __restrict__ gMU,
@@ -352,116 +343,10 @@ __global__ void k_bwd_vertex_coef(int64_t total, const float* __restrict__ CV, f
   }
 }
 
-// Outer-product sums over rows: for q < K, k < dim,  A[q][k] = sum_r G[r][q] x_r[k]^2,  B[q][k] = sum_r G[r][q] x_r[k],
-// C[q] = sum_r G[r][q], x_r = feats[idx ? idx[r] : r] (rows with idx < 0 skipped).  Tiles of 64 rows are staged in
-// LDS, one 64 x 64 block of (q, k) at a time; the block's (q, k) pairs are covered by NT micro-tiles of 4 x 4 and the
-// 256 threads form 256 / NT row groups, each accumulating its own rows of the tile (FP32 within a tile, FP64 across
-// tiles), summed over groups in order at the end.  Per-block partials part[b][2 K dim + K].
-constexpr int kOuterRows = 64;
-
-__global__ __launch_bounds__(256) void k_outer_acc(int64_t nrows, const int* __restrict__ idx,
-                                                   const float* __restrict__ feats, int dim, const float* __restrict__ G,
-                                                   int Gs, int K, double* __restrict__ part) {
-  __shared__ float gs[kOuterRows][65];
-  __shared__ float xs[kOuterRows][65];
-  __shared__ double acc[256][4];
-  const int t = threadIdx.x;
-  for (int e = t; e < kOuterRows * 65; e += 256) {  // columns past the valid ones stay finite
-    gs[e / 65][e % 65] = 0.f;
-    xs[e / 65][e % 65] = 0.f;
-  }
-  const int nqb = (K + 63) / 64, nkb = (dim + 63) / 64;
-  double* out = part + (int64_t)blockIdx.x * (2 * K * dim + K);
-  for (int qb = 0; qb < nqb; ++qb)
-    for (int kb = 0; kb < nkb; ++kb) {
-      const int kq = min(64, K - qb * 64), kk = min(64, dim - kb * 64);
-      const int nq4 = (kq + 3) / 4, nk4 = (kk + 3) / 4;
-      const int NT = nq4 * nk4, NG = 256 / NT;
-      const bool active = t < NG * NT;
-      const int mt = active ? t % NT : 0, rg = active ? t / NT : 0;
-      const int tq = mt / nk4, tk = mt % nk4;
-      double A[4][4] = {}, B[4][4] = {}, C[4] = {};
-      for (int64_t r0 = (int64_t)blockIdx.x * kOuterRows; r0 < nrows; r0 += (int64_t)gridDim.x * kOuterRows) {
-        __syncthreads();
-        // Only the block's valid columns are loaded; columns past them feed (q, k) pairs that are discarded.
-        for (int e = t; e < kOuterRows * kq; e += 256) {
-          const int r = e / kq, c = e - r * kq;
-          const int64_t row = r0 + r;
-          const bool live = row < nrows && (!idx || idx[row] >= 0);
-          gs[r][c] = live ? G[row * Gs + qb * 64 + c] : 0.f;
-        }
-        for (int e = t; e < kOuterRows * kk; e += 256) {
-          const int r = e / kk, c = e - r * kk;
-          const int64_t row = r0 + r;
-          int64_t src = -1;
-          if (row < nrows) src = idx ? (int64_t)idx[row] : row;
-          xs[r][c] = src >= 0 ? feats[src * dim + kb * 64 + c] : 0.f;
-        }
-        __syncthreads();
-        if (!active) continue;
-        float a[4][4] = {}, b[4][4] = {}, cc[4] = {};
-        for (int r = rg; r < kOuterRows; r += NG) {
-          float gv[4], xv[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            gv[u] = gs[r][tq * 4 + u];
-            xv[u] = xs[r][tk * 4 + u];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            cc[u] += gv[u];
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-              const float gx = gv[u] * xv[w];
-              b[u][w] += gx;
-              a[u][w] = fmaf(gx, xv[w], a[u][w]);
-            }
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          C[u] += cc[u];
-#pragma unroll
-          for (int w = 0; w < 4; ++w) {
-            A[u][w] += a[u][w];
-            B[u][w] += b[u][w];
-          }
-        }
-      }
-      // Sum the row groups in order, four values of the micro-tile at a time.
-      for (int u = 0; u < 4; ++u) {
-        for (int which = 0; which < 2; ++which) {
-          __syncthreads();
-#pragma unroll
-          for (int w = 0; w < 4; ++w) acc[t][w] = which ? B[u][w] : A[u][w];
-          __syncthreads();
-          if (active && rg == 0) {
-            const int q = qb * 64 + tq * 4 + u;
-            for (int w = 0; w < 4; ++w) {
-              double s2 = 0.0;
-              for (int g2 = 0; g2 < NG; ++g2) s2 += acc[g2 * NT + mt][w];
-              const int k = kb * 64 + tk * 4 + w;
-              if (q < K && k < dim) out[(which ? (int64_t)K * dim : 0) + (int64_t)q * dim + k] = s2;
-            }
-          }
-        }
-        if (kb == 0) {
-          __syncthreads();
-          acc[t][0] = C[u];
-          __syncthreads();
-          if (active && rg == 0 && tk == 0) {
-            const int q = qb * 64 + tq * 4 + u;
-            double s2 = 0.0;
-            for (int g2 = 0; g2 < NG; ++g2) s2 += acc[g2 * NT + mt][0];
-            if (q < K) out[2 * (int64_t)K * dim + q] = s2;
-          }
-        }
-      }
-    }
-}
-
-// The same sums on FP64 matrix cores, for the (q, k) block [q0, q0 + 16 NQT) x [k0, k0 + 16 NKT):
-// A = G^T (x*x), B = G^T x as GEMMs over the rows with v_mfma_f64_16x16x4f64, four rows per step.  A-operand lane
+// Outer-product sums over rows, for q < K, k < dim:  A[q][k] = sum_r G[r][q] x_r[k]^2,  B[q][k] = sum_r G[r][q]
+// x_r[k],  C[q] = sum_r G[r][q],  x_r = feats[idx ? idx[r] : r] (rows with idx < 0 skipped), one block partial each.
+// For the (q, k) block [q0, q0 + 16 NQT) x [k0, k0 + 16 NKT) they are GEMMs over the rows on FP64 matrix cores
+// (v_mfma_f64_16x16x4f64, four rows per step).  A-operand lane
 // i + 16k holds G[row k][q0 + i], B-operand lane j + 16k holds x[row k][k0 + j] (or its square), and D[4r +
 // lane/16][lane%16] sits in register r.  Products of floats are exact in FP64; the sums are FP64.  The block's four
 // waves are summed in order.
@@ -600,10 +485,6 @@ void launch_colsum(int64_t n, int C, const float* X, double* part, int nparts, d
   if (C == 0) return;
   hipLaunchKernelGGL(k_colsum, dim3(nparts), dim3(256), 0, st, n, C, X, part);
   hipLaunchKernelGGL(k_sum_parts, dim3(sum_blocks(C)), dim3(256), 0, st, nparts, C, part, out);
-}
-
-void launch_mu_state(const BwdGraph& G, const BwdState& S, double* MU, hipStream_t st) {
-  hipLaunchKernelGGL(k_mu_state, dim3(grid_for(G.n * G.P)), dim3(256), 0, st, G, S, MU);
 }
 
 void launch_bwd_out(const BwdGraph& G, const double* MU, const double* colsum, float* gMU, float* gCV, hipStream_t st) {

@@ -34,7 +34,6 @@ struct BwdState {
 void launch_fill_int(int* p, int64_t count, int value, hipStream_t st);
 void launch_sum_parts(int nparts, int C, const double* part, double* out, hipStream_t st);
 void launch_colsum(int64_t n, int C, const float* X, double* part, int nparts, double* out, hipStream_t st);
-void launch_mu_state(const BwdGraph& G, const BwdState& S, double* MU, hipStream_t st);
 void launch_bwd_out(const BwdGraph& G, const double* MU, const double* colsum, float* gMU, float* gCV, hipStream_t st);
 void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gFX,
                       const float* gRX, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st);

@@ -1371,6 +1371,19 @@ static void recompute_iteration(ac_ctx* ctx, int it, const QStat* qs_in, QStat* 
   ac::launch_reduce_lse(ctx->nb_vertex_c + 1, P, ctx->ppart.p, ctx->pstat.p, st);
 }
 
+// MU (FP64, internal vertex order) of a recorded state into MUd, by the vertex pass's own evaluation.
+static void launch_mu(ac_ctx* ctx, const ac::BwdState& S, hipStream_t st) {
+  ac::VertexArgs va = vertex_args(ctx);
+  va.n = ctx->dn;
+  va.FMaxN = S.FMaxN;
+  va.VFm = const_cast<float*>(S.VFm);  // read only in this mode
+  va.qs = S.qs;
+  va.pst = S.pst;
+  va.first = S.first;
+  va.mu_out = ctx->MUd.p;
+  ac::launch_vertex(ac::kModeMU, va, ctx->nb_vertex, st);
+}
+
 // Host side of the weight gradient from the outer-product sums A, B (K x dim) and C (K) and the pattern features
 // pf (K x dim): dL/dw_k = sum_q A[q][k] - 2 pf[q][k] B[q][k] + pf[q][k]^2 C[q].
 static void combine_outer(const std::vector<double>& s, const std::vector<float>& pf, int K, int dim, float* out) {
@@ -1455,12 +1468,12 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
       const ac::BwdState S0{ctx->sgF.p + s * nq, ctx->sgV.p + s * nq, ctx->sgQ.p + s * Q1, ctx->sgP.p + s * P, i == 0};
       const ac::BwdState S1{ctx->sgF.p + (s + 1) * nq, ctx->sgV.p + (s + 1) * nq, ctx->sgQ.p + (s + 1) * Q1,
                             ctx->sgP.p + (s + 1) * P, 0};
-      if (!have_mu) ac::launch_mu_state(G, S1, ctx->MUd.p, st);
+      if (!have_mu) launch_mu(ctx, S1, st);
       ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
       ac::launch_bwd_out(G, ctx->MUd.p, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
       ac::launch_bwd_route(G, S1, ctx->gMU.p, 1, ctx->gFX.p, ctx->gRX.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
                            ctx->red4.p, st);
-      ac::launch_mu_state(G, S0, ctx->MUd.p, st);  // MU of the input state, reused by the next step back
+      launch_mu(ctx, S0, st);  // MU of the input state, reused by the next step back
       have_mu = true;
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVR.p, 0, nq * sizeof(float), st));
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
@@ -1490,7 +1503,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
     }
   }
   HIPCHK(ctx, hipMemcpyAsync(ctx->cq.p, cq.data(), Q1 * sizeof(double), hipMemcpyHostToDevice, st));
-  if (!have_mu) ac::launch_mu_state(G, S0, ctx->MUd.p, st);
+  if (!have_mu) launch_mu(ctx, S0, st);
   ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
   ac::launch_bwd_out(G, ctx->MUd.p, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
 

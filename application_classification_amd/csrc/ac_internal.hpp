@@ -82,6 +82,7 @@ struct VertexArgs {
   float* VR;            // [n][VRs]
   int VRs;              // row stride of VR: a power of two up to 32 floats, else a multiple of 32 (whole lines)
   float* out;           // [n][P]
+  double* mu_out;       // mode MU: [n][P] in internal vertex order
   const int* perm;      // mode OUT: internal row -> caller's row of `out` (nullptr: identity)
   const IsoCol* iso;    // [P] isolated vertices' columns (nullptr: none); modes A and C write their share as the
                         // extra partial slot gridDim.x
@@ -96,7 +97,7 @@ struct VertexArgs {
   int first;           // MU = LSM(-CV): the prep state, no FMax / RMax terms
 };
 
-enum VertexMode { kModeA = 0, kModeC = 1, kModeOut = 2 };
+enum VertexMode { kModeA = 0, kModeC = 1, kModeOut = 2, kModeMU = 3 };  // kModeMU: MU in FP64 into mu_out
 
 // Launchers (ac_kernels.hip).  All asynchronous on `st`.
 // scratch (may be null: no matrix-core path): cdist_scratch_doubles(Kstore, dim) doubles.

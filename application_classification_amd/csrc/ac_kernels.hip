@@ -1391,6 +1391,8 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
           const double mu = (x - pmu[p]) - pmu[P + p];
           if (kA)
             MUs[r * ldm + p] = mu;
+          else if (kMode == kModeMU)
+            a.mu_out[(v0 + r) * P + p] = mu;
           else
             a.out[(int64_t)(a.perm ? a.perm[v0 + r] : v0 + r) * P + p] = (float)mu;
         }
@@ -1468,6 +1470,8 @@ void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st) {
     hipLaunchKernelGGL((k_vertex<kModeC, 8>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
   else if (mode == kModeC)
     hipLaunchKernelGGL((k_vertex<kModeC, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+  else if (mode == kModeMU)
+    hipLaunchKernelGGL((k_vertex<kModeMU, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
   else
     hipLaunchKernelGGL((k_vertex<kModeOut, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
 }
