@@ -762,6 +762,7 @@ static ac::VertexArgs vertex_args(ac_ctx* ctx) {
   va.n = ctx->n_act;
   va.perm = ctx->d_perm.p;
   va.iso = ctx->n_iso > 0 ? ctx->iso.p : nullptr;
+  va.track_arg = ctx->tape_K > 0;
   va.P = ctx->P;
   va.Q = ctx->Q;
   va.Qs = ctx->Qs;

@@ -68,7 +68,6 @@ struct Bundle {
   int len, pad;
 };
 
-constexpr int kVertexBlock = 256;
 constexpr int kWalkBlock = 256;
 
 struct VertexArgs {
@@ -95,6 +94,7 @@ struct VertexArgs {
   QPart* qpart;        // [gridDim + 1][Q]
   Lse* ppart;          // [gridDim + 1][P]
   int first;           // MU = LSM(-CV): the prep state, no FMax / RMax terms
+  int track_arg;       // mode A: record the vertices of the column maxima (QStat vF / vR)
 };
 
 enum VertexMode { kModeA = 0, kModeC = 1, kModeOut = 2, kModeMU = 3 };  // kModeMU: MU in FP64 into mu_out

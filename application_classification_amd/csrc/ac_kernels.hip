@@ -1133,8 +1133,8 @@ void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry,
 //             with coalesced writes, and the per-q maxima / edge log-sum-exps accumulated in registers.
 
 constexpr int kVTMax = 64;
-constexpr int kPF4 = 2;  // float4s per thread of each staged [VT][Qs] array: VT * Qs <= 4 * kPF4 * kVertexBlock
-constexpr int kPFC = 8;  // floats per thread of the staged CV tile:       VT * P  <= kPFC * kVertexBlock
+// Staged tile limits: VT * Qs <= 2048 and VT * P <= 2048 floats, for every block size NT (256 or 512).
+constexpr int kTileFloats = 2048;
 
 static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
   size_t b = (size_t)(4 * Q + 2 * P) * sizeof(double);                       // tables
@@ -1149,8 +1149,8 @@ static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
 int vertex_tile(int P, int Q) {
   const int Qs = std::max(4, (Q + 3) / 4 * 4);
   int vt = kVTMax;
-  while (vt > 4 && (vertex_lds_bytes(kModeA, vt, P, Q, Qs) > 65536 || vt * Qs > 4 * kPF4 * kVertexBlock ||
-                    vt * P > kPFC * kVertexBlock))
+  while (vt > 4 && (vertex_lds_bytes(kModeA, vt, P, Q, Qs) > 65536 || vt * Qs > kTileFloats ||
+                    vt * P > kTileFloats))
     vt /= 2;
   return vt;
 }
@@ -1221,10 +1221,12 @@ __device__ Lse iso_lse(const VertexArgs& a, int p) {
   return m > -kInf ? Lse{m, a.iso[p].t} : lse_empty();
 }
 
-// kPW > 0 (mode C with P <= 4 * kPW): each lane accumulates its rows' log-sum-exp of its wave's kPW pattern vertices
+// NT threads per block (kVW = NT / 64 waves).  kPW > 0 (mode C with P <= kVW * kPW): each lane accumulates its rows' log-sum-exp of its wave's kPW pattern vertices
 // in registers over all tiles, merged across the wave once at the end.  kPW == 0: one wave-level reduction per tile.
-template <int kMode, int kPW>
-__global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
+// kArgMax (mode A): also track the vertices of the column maxima (the adjoint's tape needs them).
+template <int kMode, int kPW, bool kArgMax, int NT>
+__global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
+  constexpr int kVW = NT / 64, kPF4 = kTileFloats / 4 / NT, kPFC = kTileFloats / NT;
   constexpr bool kA = kMode == kModeA;
   extern __shared__ double lds[];
   const int Q = a.Q, P = a.P, Qs = a.Qs, VT = a.VT;
@@ -1247,16 +1249,16 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
   cint_t* const c_incoff = (cint_t*)a.inc_off;
   cqstat_t* const c_qs = (cqstat_t*)a.qs;
 
-  for (int i = t; i < 2 * Q; i += kVertexBlock) incs[i] = a.inc[i];
-  for (int i = t; i <= P; i += kVertexBlock) incoff[i] = a.inc_off[i];
-  for (int i = t; i < Q; i += kVertexBlock) {
+  for (int i = t; i < 2 * Q; i += NT) incs[i] = a.inc[i];
+  for (int i = t; i <= P; i += NT) incoff[i] = a.inc_off[i];
+  for (int i = t; i < Q; i += NT) {
     const QStat x = a.qs[i];
     tab[i] = x.fillF;
     tab[Q + i] = x.fillR;
     tab[2 * Q + i] = x.mRE;
     tab[3 * Q + i] = x.lRE;
   }
-  for (int i = t; i < P; i += kVertexBlock) {
+  for (int i = t; i < P; i += NT) {
     if (kMode != kModeC) {
       pmu[i] = a.pst[i].m;
       pmu[P + i] = a.pst[i].l;
@@ -1268,12 +1270,12 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
   // Block 0 also writes the isolated vertices' partial, after the blocks' own ones (slot gridDim.x).
   if (blockIdx.x == 0) {
     if (kMode == kModeA)
-      for (int i = t; i < Q; i += kVertexBlock) a.qpart[(int64_t)gridDim.x * Q + i] = iso_qpart(a, i);
+      for (int i = t; i < Q; i += NT) a.qpart[(int64_t)gridDim.x * Q + i] = iso_qpart(a, i);
     else if (kMode == kModeC)
-      for (int i = t; i < P; i += kVertexBlock) a.ppart[(int64_t)gridDim.x * P + i] = iso_lse(a, i);
+      for (int i = t; i < P; i += NT) a.ppart[(int64_t)gridDim.x * P + i] = iso_lse(a, i);
   }
 
-  const int RQ = Q > 0 ? kVertexBlock / Q : 0;
+  const int RQ = Q > 0 ? NT / Q : 0;
   const bool qthread = t < RQ * Q;
   const int q = qthread ? t % Q : 0, rq = qthread ? t / Q : 0;
   double vfmax = -kInf, vrmax = -kInf;
@@ -1298,7 +1300,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
     const int last4_ = nv_ * Qs / 4 - 1, lastc_ = nv_ * P - 1;                       \
     const int64_t b4_ = v0_ * Qs / 4, bc_ = v0_ * P;                                 \
     _Pragma("unroll") for (int k = 0; k < kPF4; ++k) {                               \
-      const int i_ = min(t + k * kVertexBlock, last4_);                              \
+      const int i_ = min(t + k * NT, last4_);                              \
       pf[0][k] = F4[b4_ + i_];                                                       \
       pf[1][k] = V4[b4_ + i_];                                                       \
       if (kA) {                                                                      \
@@ -1306,7 +1308,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
         pf[3][k] = L4v[b4_ + i_];                                                    \
       }                                                                              \
     }                                                                                \
-    _Pragma("unroll") for (int k = 0; k < kPFC; ++k) pc[k] = a.CV[bc_ + min(t + k * kVertexBlock, lastc_)]; \
+    _Pragma("unroll") for (int k = 0; k < kPFC; ++k) pc[k] = a.CV[bc_ + min(t + k * NT, lastc_)]; \
   }
   int64_t tile = blockIdx.x;
   if (tile < ntiles) AC_VX_PREFETCH(tile);
@@ -1318,7 +1320,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
       const int cnt4 = nv * Qs / 4, cntc = nv * P;
 #pragma unroll
       for (int k = 0; k < kPF4; ++k) {
-        const int i = t + k * kVertexBlock;
+        const int i = t + k * NT;
         if (i < cnt4) {
           const int e = 4 * i, r = e / Qs, c = e - r * Qs;
           const int o = r * ldq + c;
@@ -1335,7 +1337,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
       }
 #pragma unroll
       for (int k = 0; k < kPFC; ++k) {
-        const int i = t + k * kVertexBlock;
+        const int i = t + k * NT;
         if (i < cntc) {
           const int r = i / P, c = i - r * P;
           Cs[r * ldp + c] = pc[k];
@@ -1372,7 +1374,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
     if (kMode == kModeC && kPW > 0) {
 #pragma unroll
       for (int pi = 0; pi < kPW; ++pi) {
-        const int p = wave + 4 * pi;
+        const int p = wave + kVW * pi;
         if (p < P) {
           double x;
           AC_VX_MU_PRE(x, p, lane);
@@ -1380,7 +1382,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
         }
       }
     } else {
-      for (int p = wave; p < P; p += kVertexBlock / 64) {
+      for (int p = wave; p < P; p += kVW) {
         const int r = lane;
         double x;
         AC_VX_MU_PRE(x, p, r);
@@ -1414,13 +1416,18 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
           const double vf = MUs[r * ldm + q_pd] - fm;
           const double vr = MUs[r * ldm + q_ps] - rm;
           a.VR[(v0 + r) * a.VRs + q] = (float)vr;
-          if (vf > vfmax) {  // rows ascend within a thread, so the first maximum is the lowest vertex
-            vfmax = vf;
-            avf = (int)(v0 + r);
-          }
-          if (vr > vrmax) {
-            vrmax = vr;
-            avr = (int)(v0 + r);
+          if (kArgMax) {
+            if (vf > vfmax) {  // rows ascend within a thread, so the first maximum is the lowest vertex
+              vfmax = vf;
+              avf = (int)(v0 + r);
+            }
+            if (vr > vrmax) {
+              vrmax = vr;
+              avr = (int)(v0 + r);
+            }
+          } else {
+            vfmax = fmax(vfmax, vf);
+            vrmax = fmax(vrmax, vr);
           }
           const bool out_edges = mo != INFINITY;
           const double xx = vf - (double)mo;
@@ -1448,32 +1455,36 @@ __global__ __launch_bounds__(kVertexBlock) void k_vertex(VertexArgs a) {
     if (kPW > 0) {
 #pragma unroll
       for (int pi = 0; pi < kPW; ++pi) {
-        const int p = wave + 4 * pi;
+        const int p = wave + kVW * pi;
         const Lse w = wave_lse_merge(pacc[pi]);
         if (p < P && lane == 0) a.ppart[(int64_t)blockIdx.x * P + p] = w;
       }
     } else {
-      for (int i = t; i < P; i += kVertexBlock) a.ppart[(int64_t)blockIdx.x * P + i] = lacc[i];
+      for (int i = t; i < P; i += NT) a.ppart[(int64_t)blockIdx.x * P + i] = lacc[i];
     }
   }
 }
 
+// Pass A runs 256-thread blocks (its register use), the lighter modes 512-thread blocks (measured at R-MAT 22).
 void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st) {
+  constexpr int kNA = 256, kNO = 512;
   const size_t tiles = vertex_lds_bytes(mode, a.VT, a.P, a.Q, a.Qs);
-  const size_t red = mode == kModeA ? sizeof(QPart) * kVertexBlock : 0;
+  const size_t red = mode == kModeA ? sizeof(QPart) * kNA : 0;
   const size_t lds = std::max(tiles, red);
-  if (mode == kModeA)
-    hipLaunchKernelGGL((k_vertex<kModeA, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
-  else if (mode == kModeC && a.P <= 16)
-    hipLaunchKernelGGL((k_vertex<kModeC, 4>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
-  else if (mode == kModeC && a.P <= 32)
-    hipLaunchKernelGGL((k_vertex<kModeC, 8>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+  if (mode == kModeA && a.track_arg)
+    hipLaunchKernelGGL((k_vertex<kModeA, 0, true, kNA>), dim3(nblocks), dim3(kNA), lds, st, a);
+  else if (mode == kModeA)
+    hipLaunchKernelGGL((k_vertex<kModeA, 0, false, kNA>), dim3(nblocks), dim3(kNA), lds, st, a);
+  else if (mode == kModeC && a.P <= 4 * (kNO / 64))
+    hipLaunchKernelGGL((k_vertex<kModeC, 4, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
+  else if (mode == kModeC && a.P <= 8 * (kNO / 64))
+    hipLaunchKernelGGL((k_vertex<kModeC, 8, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
   else if (mode == kModeC)
-    hipLaunchKernelGGL((k_vertex<kModeC, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+    hipLaunchKernelGGL((k_vertex<kModeC, 0, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
   else if (mode == kModeMU)
-    hipLaunchKernelGGL((k_vertex<kModeMU, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+    hipLaunchKernelGGL((k_vertex<kModeMU, 0, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
   else
-    hipLaunchKernelGGL((k_vertex<kModeOut, 0>), dim3(nblocks), dim3(kVertexBlock), lds, st, a);
+    hipLaunchKernelGGL((k_vertex<kModeOut, 0, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
 }
 
 // Columns [c0, c0 + ncol) of src[n][P] into dst[n][ncol].
