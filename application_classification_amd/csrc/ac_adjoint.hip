@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_bwd_scatter(BwdGraph G, const float* __
     if (gf != 0.f) {
       const int row = FA[o];
       if (row >= 0) {
-        if (gVR) atomicAdd(&gVR[(int64_t)bsrc[row] * G.Qs + q], gf);
+        if (gVR) atomicAdd(&gVR[(int64_t)(bsrc[row] & ~kNewVertex) * G.Qs + q], gf);
         atomicAdd(&gCE[(int64_t)row * G.Qs + q], -gf);
       }
     }
@@ -327,7 +327,7 @@ __global__ void k_bwd_edge_coef(int64_t nrows, int Q, int Qs, const int* __restr
     if (q < Q && beid[row] >= 0) {
       const double ce = (double)CE[i];
       if (ce > 0.0 && ce != kInf) {
-        const double g = (double)G[i] + (cD[(int64_t)bsrc[row] * Qs + q] + cq[q]) * exp(-ce);
+        const double g = (double)G[i] + (cD[(int64_t)(bsrc[row] & ~kNewVertex) * Qs + q] + cq[q]) * exp(-ce);
         out = (float)(g / (2.0 * ce));
       }
     }

@@ -105,7 +105,7 @@ void build_runs(const std::vector<int64_t>& off, int64_t n, std::vector<Run>& ru
   splits.clear();
   nslots = 0;
   bool open = false;
-  int64_t b = 0, len = 0;
+  int64_t b = 0, len = 0, last_v = -2;
   auto close = [&](int64_t e) {
     if (open) runs.push_back(Run{(int)b, (int)e, -1, 0});
     open = false;
@@ -123,7 +123,9 @@ void build_runs(const std::vector<int64_t>& off, int64_t n, std::vector<Run>& ru
       }
       continue;
     }
-    if (open && len + deg > kRunEdges) close(off[v]);
+    // A run covers consecutive vertices only: the walk steps its destination by one at each new vertex.
+    if (open && (len + deg > kRunEdges || v != last_v + 1)) close(off[v]);
+    last_v = v;
     if (!open) {
       open = true;
       b = off[v];

@@ -69,6 +69,8 @@ struct Bundle {
 };
 
 constexpr int kWalkBlock = 256;
+// Flag on a bundle row's source id: the row starts the next vertex of its run (vertices in a run are consecutive).
+constexpr int kNewVertex = (int)0x80000000u;
 
 struct VertexArgs {
   int64_t n;

@@ -769,6 +769,10 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   };
   const int GL4 = G * L4;
   const int* srcp = bsrc + Bd.base + g;
+  // Plain walk (kWithVR, !kArg): destinations come from the sources' new-vertex flags (a run's vertices are
+  // consecutive), so only the run's first destination is read.
+  constexpr bool kFlags = kWithVR && !kArg;
+  if (kFlags) cur = bdst[Bd.base + g];
   const int* dstp = bdst + Bd.base + g;
   const f4v* cep = reinterpret_cast<const f4v*>(CE) + (Bd.base + g) * L4 + j;
   const f4v* VRv = reinterpret_cast<const f4v*>(VR);
@@ -783,13 +787,19 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   {                                                                                                      \
     const int t_ = (T) < last ? (T) : last;                                                              \
     _Pragma("unroll") for (int u = 0; u < B; ++u) C[u] = __builtin_nontemporal_load(cep + (t_ * B + u) * GL4); \
-    _Pragma("unroll") for (int u = 0; u < B; ++u) D[u] = __builtin_nontemporal_load(dstp + (t_ * B + u) * G);  \
+    _Pragma("unroll") for (int u = 0; u < B; ++u) D[u] =                                                 \
+        kFlags ? (S[u] & kNewVertex) : __builtin_nontemporal_load(dstp + (t_ * B + u) * G);              \
     _Pragma("unroll") for (int u = 0; u < B; ++u) W[u] =                                                 \
-        kWithVR ? VRv[(int64_t)S[u] * VR4s + j] : f4v{0.f, 0.f, 0.f, 0.f};                               \
+        kWithVR ? VRv[(int64_t)(S[u] & ~kNewVertex) * VR4s + j] : f4v{0.f, 0.f, 0.f, 0.f};               \
   }
 #define AC_WK_REDUCE(C, D, W, T)                                                    \
   _Pragma("unroll") for (int u = 0; u < B; ++u) {                                   \
-    if (D[u] != cur) {                                                              \
+    if (kFlags) {                                                                   \
+      if (D[u]) {                                                                   \
+        emit(cur);                                                                  \
+        ++cur;                                                                      \
+      }                                                                             \
+    } else if (D[u] != cur) {                                                       \
       if (cur >= 0) emit(cur);                                                      \
       cur = D[u];                                                                   \
     }                                                                               \
@@ -948,6 +958,7 @@ __global__ __launch_bounds__(256) void k_build_bundles(const Bundle* __restrict_
         dst = sd[k].y;
         e = eid[k];
         rowmap[k] = (int)row;
+        if (s > 0 && sd[k - 1].y != dst) src |= kNewVertex;  // the run moves on to the next vertex (dst + 1)
       } else {
         dst = sd[R.ke - 1].y;
       }
