@@ -56,6 +56,15 @@ __device__ __forceinline__ void lse_add(Lse& s, double x) {
   s.m = valid ? fmax(x, s.m) : s.m;
 }
 
+// lse_add for a finite x (no -inf term to skip): the same result with fewer selects.  An empty accumulator
+// (m = -inf) takes x as its maximum: exp(-inf) = 0 clears t.
+__device__ __forceinline__ void lse_add_finite(Lse& s, double x) {
+  const bool up = x > s.m;
+  const double e = (double)__expf((float)(fmin(x, s.m) - fmax(x, s.m)));
+  s.t = up ? (1.0 + s.t) * e : s.t + e;
+  s.m = fmax(x, s.m);
+}
+
 __device__ __forceinline__ Lse lse_merge(Lse a, Lse b) {
   if (b.m == -kInf) return a;
   if (a.m == -kInf) return b;
@@ -714,9 +723,9 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
                                                          double* __restrict__ carry, int* __restrict__ FA,
                                                          int* __restrict__ carg, const float* __restrict__ bmin) {
   const int L4 = Qs / 4, G = 64 / L4, VR4s = VRs / 4;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane / L4, j = lane - g * L4;
-  const int64_t b = (int64_t)blockIdx.x * (kWalkBlock / 64) + wave;
+  const int64_t b = (int64_t)blockIdx.x * (kWalkBlock / 64) + wave;  // wave-uniform: the bundle header is scalar
   if (g >= G || b >= nbundles) return;
   const Bundle Bd = bundles[b];
   const int slot = gslot[b * G + g];
@@ -768,25 +777,28 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
     mx0 = mx1 = mx2 = mx3 = -kInf;
   };
   const int GL4 = G * L4;
-  const int* srcp = bsrc + Bd.base + g;
+  // Streams: a wave-uniform bundle base plus small per-lane offsets (32-bit address arithmetic).
+  const int* srcb = bsrc + Bd.base;
+  const f4v* ceb = reinterpret_cast<const f4v*>(CE) + Bd.base * L4;
+  const unsigned so = (unsigned)g, co = (unsigned)(g * L4 + j);
   // Plain walk (kWithVR, !kArg): destinations come from the sources' new-vertex flags (a run's vertices are
   // consecutive), so only the run's first destination is read.
   constexpr bool kFlags = kWithVR && !kArg;
   if (kFlags) cur = bdst[Bd.base + g];
   const int* dstp = bdst + Bd.base + g;
-  const f4v* cep = reinterpret_cast<const f4v*>(CE) + (Bd.base + g) * L4 + j;
   const f4v* VRv = reinterpret_cast<const f4v*>(VR);
   const int last = Bd.len / B - 1;
 #define AC_WK_SRC(S, T)                                                          \
   if (kWithVR) {                                                                 \
     const int t_ = (T) < last ? (T) : last;                                      \
     _Pragma("unroll") for (int u = 0; u < B; ++u) S[u] =                         \
-        __builtin_nontemporal_load(srcp + (t_ * B + u) * G);                     \
+        __builtin_nontemporal_load(srcb + (so + (unsigned)((t_ * B + u) * G)));  \
   }
 #define AC_WK_BATCH(C, D, W, S, T)                                                                       \
   {                                                                                                      \
     const int t_ = (T) < last ? (T) : last;                                                              \
-    _Pragma("unroll") for (int u = 0; u < B; ++u) C[u] = __builtin_nontemporal_load(cep + (t_ * B + u) * GL4); \
+    _Pragma("unroll") for (int u = 0; u < B; ++u)                                                        \
+      C[u] = __builtin_nontemporal_load(ceb + (co + (unsigned)((t_ * B + u) * GL4)));                    \
     _Pragma("unroll") for (int u = 0; u < B; ++u) D[u] =                                                 \
         kFlags ? (S[u] & kNewVertex) : __builtin_nontemporal_load(dstp + (t_ * B + u) * G);              \
     _Pragma("unroll") for (int u = 0; u < B; ++u) W[u] =                                                 \
@@ -1426,7 +1438,6 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
           const double rm = fmax(fR, vfm == -INFINITY ? -kInf : ((double)vfm - mR) - lR);
           const double vf = MUs[r * ldm + q_pd] - fm;
           const double vr = MUs[r * ldm + q_ps] - rm;
-          a.VR[(v0 + r) * a.VRs + q] = (float)vr;
           if (kArgMax) {
             if (vf > vfmax) {  // rows ascend within a thread, so the first maximum is the lowest vertex
               vfmax = vf;
@@ -1440,11 +1451,15 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
             vfmax = fmax(vfmax, vf);
             vrmax = fmax(vrmax, vr);
           }
-          const bool out_edges = mo != INFINITY;
-          const double xx = vf - (double)mo;
-          a.VFm[i] = out_edges ? (float)xx : -INFINITY;
-          lse_add(lre, out_edges ? xx + (double)lo : -kInf);
-          lse_add(lfe, out_edges ? (vr - (double)mo) + (double)lo : -kInf);
+          // A vertex without out-edges keeps VFm = -inf (as filled in the prep), is no gathered source and adds no
+          // edge term.
+          if (mo != INFINITY) {
+            const double xx = vf - (double)mo;
+            a.VR[(v0 + r) * a.VRs + q] = (float)vr;
+            a.VFm[i] = (float)xx;
+            lse_add_finite(lre, xx + (double)lo);
+            lse_add_finite(lfe, (vr - (double)mo) + (double)lo);
+          }
         }
       }
     }
