@@ -1161,7 +1161,7 @@ constexpr int kTileFloats = 2048;
 
 static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
   size_t b = (size_t)(4 * Q + 2 * P) * sizeof(double);                       // tables
-  if (mode == kModeA) b += (size_t)(P + 1) * VT * sizeof(double);            // MU
+  if (mode != kModeC) b += (size_t)(P + 1) * VT * sizeof(double);            // MU
   if (mode == kModeC) b += (size_t)P * sizeof(Lse);                          // column accumulators
   const int nst = mode == kModeA ? 4 : 2;                                    // staged [VT][Qs] arrays
   b += (size_t)VT * (nst * (Qs + 1) + (P + 1)) * sizeof(float);              // staged rows
@@ -1251,14 +1251,15 @@ template <int kMode, int kPW, bool kArgMax, int NT>
 __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   constexpr int kVW = NT / 64, kPF4 = kTileFloats / 4 / NT, kPFC = kTileFloats / NT;
   constexpr bool kA = kMode == kModeA;
+  constexpr bool kRows = kMode == kModeOut || kMode == kModeMU;  // MU tile staged, written back row by row
   extern __shared__ double lds[];
   const int Q = a.Q, P = a.P, Qs = a.Qs, VT = a.VT;
   const int ldq = Qs + 1, ldp = P + 1;
   double* tab = lds;                                   // [4][Q]: fillF, fillR, mRE, lRE
   double* pmu = tab + 4 * Q;                           // [2][P]: m, l of LSE_MU
-  double* MUs = pmu + 2 * P;                           // [VT][P + 1] (mode A)
+  double* MUs = pmu + 2 * P;                           // [VT][P + 1] (modes A, OUT, MU)
   const int ldm = P + 1;
-  Lse* lacc = reinterpret_cast<Lse*>(MUs + (kA ? ldm * VT : 0));              // [P] (mode C)
+  Lse* lacc = reinterpret_cast<Lse*>(MUs + (kA || kRows ? ldm * VT : 0));    // [P] (mode C)
   float* Fs = reinterpret_cast<float*>(lacc + (kMode == kModeC ? P : 0));    // [VT][ldq] FMaxN
   float* Vs = Fs + VT * ldq;                           // [VT][ldq] VFm
   float* Ms = Vs + VT * ldq;                           // [VT][ldq] minCEo (mode A)
@@ -1414,16 +1415,22 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
           if (lane == 0) lacc[p] = lse_merge(lacc[p], w);
         } else if (r < nv) {
           const double mu = (x - pmu[p]) - pmu[P + p];
-          if (kA)
-            MUs[r * ldm + p] = mu;
-          else if (kMode == kModeMU)
-            a.mu_out[(v0 + r) * P + p] = mu;
-          else
-            a.out[(int64_t)(a.perm ? a.perm[v0 + r] : v0 + r) * P + p] = (float)mu;
+          MUs[r * ldm + p] = mu;
         }
       }
     }
 #undef AC_VX_MU_PRE
+    if (kRows) {
+      // The tile's rows go out whole: consecutive threads write consecutive columns of a row.
+      __syncthreads();
+      for (int e = t; e < nv * P; e += NT) {
+        const int r = e / P, c = e - r * P;
+        if (kMode == kModeMU)
+          a.mu_out[(v0 + r) * P + c] = MUs[r * ldm + c];
+        else
+          a.out[(int64_t)(a.perm ? a.perm[v0 + r] : v0 + r) * P + c] = (float)MUs[r * ldm + c];
+      }
+    }
     if (kA) {
       __syncthreads();
       if (qthread) {
