@@ -39,15 +39,16 @@ int run_edges() {
 
 int round8(int x) { return (x + 7) / 8 * 8; }
 
-// Most blocks of vertex pass A / C.  Each block walks its tiles grid-stride and leaves one partial per pattern edge
-// (A) or vertex (C), merged serially afterwards, so fewer blocks make a cheaper merge: a few resident rounds of
-// blocks on 256 CUs (measured at R-MAT scale 22).
-int vertex_blocks(bool pass_c) {
-  if (const char* e = std::getenv(pass_c ? "AC_VERTEX_BLOCKS_C" : "AC_VERTEX_BLOCKS")) {  // tuning override
+// Blocks of a vertex pass.  Each block walks its tiles grid-stride, and passes A and C leave one partial per pattern
+// edge / vertex that is merged serially afterwards: one round of resident blocks (no partly filled last round, few
+// partials) is fastest for them at R-MAT scale 22; the output passes, which leave no partials, use two rounds.
+int vertex_blocks(const char* env, int resident, int rounds, int64_t ntiles) {
+  if (const char* e = std::getenv(env)) {  // tuning override
     const int v = std::atoi(e);
-    if (v >= 64 && v <= 65536) return v;
+    if (v >= 64 && v <= 65536) return (int)std::min<int64_t>(v, std::max<int64_t>(ntiles, 1));
   }
-  return pass_c ? 1024 : 2048;
+  const int64_t want = resident > 0 ? (int64_t)resident * rounds : 1024;
+  return (int)std::max<int64_t>(1, std::min(want, ntiles));
 }
 
 // Row stride of the gathered VR rows: whole 128-byte lines (or an exact fraction of one), so that a gather of one
@@ -205,7 +206,7 @@ struct ac_ctx {
   DevBuf<QStat> qstat;  // [3][Q]: prep state, then by iteration parity
   DevBuf<Lse2> pstat, lse_cv, lse_ce;
   DevBuf<IsoCol> iso;   // [P] the isolated vertices' columns of CV (when n_iso > 0)
-  int nb_vertex = 0, nb_vertex_c = 0, nb_col = 0, nb_src = 0, VT = 0, VRs = 0;
+  int nb_vertex = 0, nb_vertex_c = 0, nb_vertex_o = 0, nb_col = 0, nb_src = 0, VT = 0, VRs = 0;
   int64_t work_key[5] = {-1, -1, -1, -1, -1};
 
   bool have_result = false;
@@ -585,11 +586,18 @@ static int ensure_work(ac_ctx* ctx) {
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
   ctx->VT = ac::vertex_tile(P, Q);
   const int64_t ntiles = (std::max<int64_t>(ctx->n_act, 1) + ctx->VT - 1) / ctx->VT;
-  // A few tiles per block at least, so that small graphs do not pay for merging many near-empty partials.
-  ctx->nb_vertex = (int)std::min<int64_t>(std::max<int64_t>(std::min<int64_t>(ntiles, 512), ntiles / 4),
-                                          vertex_blocks(false));
-  ctx->nb_vertex_c = (int)std::min<int64_t>(std::max<int64_t>(std::min<int64_t>(ntiles, 512), ntiles / 8),
-                                            vertex_blocks(true));
+  {
+    ac::VertexArgs va{};
+    va.P = P;
+    va.Q = Q;
+    va.Qs = Qs;
+    va.VT = ctx->VT;
+    const int64_t all_tiles = (n + ctx->VT - 1) / ctx->VT;
+    ctx->nb_vertex = vertex_blocks("AC_VERTEX_BLOCKS", ac::vertex_resident_blocks(ac::kModeA, va), 1, ntiles);
+    ctx->nb_vertex_c = vertex_blocks("AC_VERTEX_BLOCKS_C", ac::vertex_resident_blocks(ac::kModeC, va), 1, ntiles);
+    ctx->nb_vertex_o = vertex_blocks("AC_VERTEX_BLOCKS_O", ac::vertex_resident_blocks(ac::kModeOut, va), 2,
+                                     all_tiles);
+  }
   ctx->nb_col = (int)std::min<int64_t>((n + 255) / 256, 1024);
   ctx->nb_src = std::max(1, ac::walk_blocks(ctx->nruns_out, Qs));
   const int nb_l = std::max(ctx->nb_col, ctx->nb_src);
@@ -823,7 +831,7 @@ static int enqueue_output(ac_ctx* ctx, ac::VertexArgs& va, int done, const QStat
   vo.n = ctx->dn;  // every row, isolated ones included, written to the caller's row
   vo.first = done == 0;
   vo.qs = qs_state;
-  ac::launch_vertex(ac::kModeOut, vo, ctx->nb_vertex, ctx->stream);
+  ac::launch_vertex(ac::kModeOut, vo, ctx->nb_vertex_o, ctx->stream);
   return AC_OK;
 }
 
@@ -1384,7 +1392,7 @@ static void launch_mu(ac_ctx* ctx, const ac::BwdState& S, hipStream_t st) {
   va.pst = S.pst;
   va.first = S.first;
   va.mu_out = ctx->MUd.p;
-  ac::launch_vertex(ac::kModeMU, va, ctx->nb_vertex, st);
+  ac::launch_vertex(ac::kModeMU, va, ctx->nb_vertex_o, st);
 }
 
 // Host side of the weight gradient from the outer-product sums A, B (K x dim) and C (K) and the pattern features

@@ -136,6 +136,8 @@ void launch_walk_max_fixup32(int nchunk, const Split* chunks, int nms, const Spl
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
                            float* FMaxN, hipStream_t st, const int* carg = nullptr, int* FA = nullptr);
 void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st);
+// Blocks of vertex pass `mode` resident at once on the current device (0 if unknown); a.VT/P/Q/Qs set its LDS.
+int vertex_resident_blocks(int mode, const VertexArgs& a);
 // out[i][c] = in[perm[i]][c] for the n rows of ncol floats.
 void launch_gather_rows(int64_t n, int ncol, const int* perm, const float* in, float* out, hipStream_t st);
 void launch_copy_cols(int64_t n, int P, int c0, int ncol, const float* src, float* dst, hipStream_t st);

@@ -1520,6 +1520,24 @@ void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st) {
     hipLaunchKernelGGL((k_vertex<kModeOut, 0, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
 }
 
+// Blocks of a vertex pass resident at once on the device (CUs x blocks per CU at the pass's registers and LDS).
+int vertex_resident_blocks(int mode, const VertexArgs& a) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  const size_t lds = vertex_lds_bytes(mode, a.VT, a.P, a.Q, a.Qs);
+  hipError_t e;
+  if (mode == kModeA)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vertex<kModeA, 0, false, 256>, 256,
+                                                     std::max(lds, sizeof(QPart) * 256));
+  else if (mode == kModeC)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vertex<kModeC, 4, false, 512>, 512, lds);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vertex<kModeOut, 0, false, 512>, 512, lds);
+  return e == hipSuccess ? cus * std::max(per, 1) : 0;
+}
+
 // Columns [c0, c0 + ncol) of src[n][P] into dst[n][ncol].
 __global__ void k_copy_cols(int64_t n, int P, int c0, int ncol, const float* __restrict__ src, float* __restrict__ dst) {
   const int64_t total = n * ncol;
