@@ -162,7 +162,7 @@ struct ac_ctx {
   // class in ascending caller order.  perm maps an internal vertex to the caller's; the vertex passes stream the
   // first n_act = dn - n_iso rows only.
   DevBuf<int> d_perm;
-  int64_t n_act = 0, n_iso = 0;
+  int64_t n_act = 0, n_iso = 0, n_both = 0, n_inonly = 0;
   int64_t nruns_in = 0, nruns_out = 0;
   int nsplit_in = 0, nsplit_out = 0, nslots_in = 0, nslots_out = 0;
 
@@ -315,7 +315,7 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
     return ctx->fail(AC_ERR_ARG, "ac_set_graph: more than 2^31 edges is not supported");
   // Internal order by degree class (see ac_ctx::d_perm); every structure below uses internal vertex ids.
   std::vector<int> perm, inv(nV);
-  int64_t n_iso = 0;
+  int64_t n_iso = 0, n_both = 0, n_inonly = 0;
   std::vector<int32_t> isrc(nE), idst(nE);
   {
     std::vector<uint8_t> cls(nV, 3);  // bit 0 clear: has in-edges; bit 1 clear: has out-edges
@@ -332,6 +332,8 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
           inv[v] = (int)perm.size();
           perm.push_back((int)v);
           if (c == 3) ++n_iso;
+          if (c == 0) ++n_both;
+          if (c == 1) ++n_inonly;
         }
     for (int64_t e = 0; e < nE; ++e) {
       isrc[e] = inv[src[e]];
@@ -404,6 +406,8 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   ctx->nfix_parts = nparts;
   ctx->dn = nV;
   ctx->n_iso = n_iso;
+  ctx->n_both = n_both;
+  ctx->n_inonly = n_inonly;
   ctx->n_act = nV - n_iso;
   ctx->dV = dV;
   ctx->de = nE;
@@ -771,6 +775,10 @@ static ac::VertexArgs vertex_args(ac_ctx* ctx) {
   ac::VertexArgs va{};
   va.n = ctx->n_act;
   va.perm = ctx->d_perm.p;
+  va.f_end = ctx->n_both + ctx->n_inonly;
+  va.o_lo = ctx->n_both;
+  va.o_hi = ctx->n_both + ctx->n_inonly;
+  va.o_end = ctx->n_act;
   va.iso = ctx->n_iso > 0 ? ctx->iso.p : nullptr;
   va.track_arg = ctx->tape_K > 0;
   va.P = ctx->P;

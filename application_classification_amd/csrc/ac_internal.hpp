@@ -85,6 +85,9 @@ struct VertexArgs {
   float* out;           // [n][P]
   double* mu_out;       // mode MU: [n][P] in internal vertex order
   const int* perm;      // mode OUT: internal row -> caller's row of `out` (nullptr: identity)
+  // Constant rows, not read by mode A: rows >= f_end have no in-edge (FMaxN = -inf); rows in [o_lo, o_hi) and
+  // >= o_end have no out-edge (VFm = -inf, minCEo = +inf, lsO = 0).
+  int64_t f_end, o_lo, o_hi, o_end;
   const IsoCol* iso;    // [P] isolated vertices' columns (nullptr: none); modes A and C write their share as the
                         // extra partial slot gridDim.x
   const int* ps;

@@ -1311,6 +1311,8 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   const f4v* M4 = reinterpret_cast<const f4v*>(a.minCEo);
   const f4v* L4v = reinterpret_cast<const f4v*>(a.lsO);
   const int64_t ntiles = (a.n + VT - 1) / VT;
+  const int64_t L4q = Qs / 4, f_end4 = a.f_end * L4q, o_lo4 = a.o_lo * L4q, o_hi4 = a.o_hi * L4q,
+                o_end4 = a.o_end * L4q;
   Lse pacc[kPW > 0 ? kPW : 1];
 #pragma unroll
   for (int pi = 0; pi < (kPW > 0 ? kPW : 1); ++pi) pacc[pi] = lse_empty();
@@ -1324,12 +1326,13 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
     const int last4_ = nv_ * Qs / 4 - 1, lastc_ = nv_ * P - 1;                       \
     const int64_t b4_ = v0_ * Qs / 4, bc_ = v0_ * P;                                 \
     _Pragma("unroll") for (int k = 0; k < kPF4; ++k) {                               \
-      const int i_ = min(t + k * NT, last4_);                              \
-      pf[0][k] = F4[b4_ + i_];                                                       \
-      pf[1][k] = V4[b4_ + i_];                                                       \
+      const int64_t g_ = b4_ + min(t + k * NT, last4_);                              \
+      const bool hasF_ = !kA || g_ < f_end4, hasO_ = !kA || g_ < o_lo4 || (g_ >= o_hi4 && g_ < o_end4);  \
+      pf[0][k] = hasF_ ? F4[g_] : f4v{-INFINITY, -INFINITY, -INFINITY, -INFINITY};   \
+      pf[1][k] = hasO_ ? V4[g_] : f4v{-INFINITY, -INFINITY, -INFINITY, -INFINITY};   \
       if (kA) {                                                                      \
-        pf[2][k] = M4[b4_ + i_];                                                     \
-        pf[3][k] = L4v[b4_ + i_];                                                    \
+        pf[2][k] = hasO_ ? M4[g_] : f4v{INFINITY, INFINITY, INFINITY, INFINITY};     \
+        pf[3][k] = hasO_ ? L4v[g_] : f4v{0.f, 0.f, 0.f, 0.f};                       \
       }                                                                              \
     }                                                                                \
     _Pragma("unroll") for (int k = 0; k < kPFC; ++k) pc[k] = a.CV[bc_ + min(t + k * NT, lastc_)]; \
