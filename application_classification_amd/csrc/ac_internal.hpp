@@ -133,7 +133,7 @@ void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, 
                      const float* bmin = nullptr);
 // Plain-path fixup (FP32 normalised pieces): chunks of at most kFixChunk pieces, then the vertices of several
 // chunks from their partials (part: one [Qs] row per partial).
-constexpr int kFixChunk = 8;
+constexpr int kFixChunk = 16;
 void launch_walk_max_fixup32(int nchunk, const Split* chunks, int nms, const Split* ms, const double* carry, int Qs,
                              float* part, float* FMaxN, hipStream_t st);
 void launch_walk_max_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, const QStat* qs,
