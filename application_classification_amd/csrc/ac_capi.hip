@@ -788,6 +788,7 @@ static ac::VertexArgs vertex_args(ac_ctx* ctx) {
   va.CV = ctx->CV.p;
   va.FMaxN = ctx->FMaxN.p;
   va.VFm = ctx->VFm.p;
+  va.VFm_out = ctx->VFm.p;
   va.minCEo = ctx->minCEo.p;
   va.lsO = ctx->lsO.p;
   va.VR = ctx->VR.p;
@@ -1368,26 +1369,48 @@ extern "C" int ac_batch_autotune(ac_ctx* ctx, int probe_iters, int32_t* choice_o
 // ---------------------------------------------------------------------------------------------------------------
 // Adjoint.
 
-// One iteration of the recompute: scalars in qs_in -> qs_out, the walk recording its argmax rows into FA.
-static void recompute_iteration(ac_ctx* ctx, int it, const QStat* qs_in, QStat* qs_out, int* FA) {
+// One iteration of the recompute, from the recorded state In into state Out (separate buffers: nothing is copied),
+// the walk recording its argmax rows into FA.  Out's rows that no kernel writes (FMaxN of vertices without
+// in-edges, VFm of vertices without out-edges) must already hold -inf.
+struct StateBufs {
+  float* F;
+  float* V;
+  QStat* qs;
+  Lse2* pst;
+};
+
+static int recompute_iteration(ac_ctx* ctx, int it, const StateBufs& In, const StateBufs& Out, int* FA) {
   hipStream_t st = ctx->stream;
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
+  const size_t nq = (size_t)ctx->dn * Qs;
   ac::VertexArgs va = vertex_args(ctx);
   if (Q > 0) {
     va.first = it == 0;
-    va.qs = qs_in;
+    va.qs = In.qs;
+    va.FMaxN = In.F;
+    va.VFm = In.V;
+    va.VFm_out = Out.V;
+    va.pst = In.pst;
     ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
-    ac::launch_reduce_q(ctx->nb_vertex + 1, Q, ctx->qpart.p, qs_out, st);
+    ac::launch_reduce_q(ctx->nb_vertex + 1, Q, ctx->qpart.p, Out.qs, st);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
-                        ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_out, ctx->FMaxN.p, ctx->carry_in.p,
-                        walk_batch_of(ctx), st, FA, ctx->carg_in.p, ctx->prune ? ctx->bmin.p : nullptr);
-    ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_out, ctx->FMaxN.p, st,
+                        ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, Out.qs, Out.F, ctx->carry_in.p, walk_batch_of(ctx),
+                        st, FA, ctx->carg_in.p, ctx->prune ? ctx->bmin.p : nullptr);
+    ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, Out.qs, Out.F, st,
                               ctx->carg_in.p, FA);
+  } else {
+    HIPCHK(ctx, hipMemcpyAsync(Out.F, In.F, nq * sizeof(float), hipMemcpyDeviceToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(Out.V, In.V, nq * sizeof(float), hipMemcpyDeviceToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(Out.qs, In.qs, sizeof(QStat), hipMemcpyDeviceToDevice, st));
   }
   va.first = 0;
-  va.qs = Q > 0 ? qs_out : qs_in;
+  va.qs = Out.qs;
+  va.FMaxN = Out.F;
+  va.VFm = Out.V;
+  va.VFm_out = Out.V;
   ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex_c, st);
-  ac::launch_reduce_lse(ctx->nb_vertex_c + 1, P, ctx->ppart.p, ctx->pstat.p, st);
+  ac::launch_reduce_lse(ctx->nb_vertex_c + 1, P, ctx->ppart.p, Out.pst, st);
+  return AC_OK;
 }
 
 // MU (FP64, internal vertex order) of a recorded state into MUd, by the vertex pass's own evaluation.
@@ -1458,35 +1481,32 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
 
   ac::BwdGraph G{n, P, Q, Qs, ctx->CV.p, ctx->minCEo.p, ctx->lsO.p, ctx->d_ps.p, ctx->d_pd.p, ctx->d_inc_off.p,
                  ctx->d_inc.p};
-  auto cpy = [&](void* dst, const void* src, size_t bytes) {
-    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
-  };
+  // Segment states: state 0 is the checkpoint itself, states 1..K live in the segment buffers, whose constant rows
+  // (-inf) are set once here and never written afterwards.
+  ac::launch_fill(ctx->sgF.p, (int64_t)(K + 1) * nq, -INFINITY, st);
+  ac::launch_fill(ctx->sgV.p, (int64_t)(K + 1) * nq, -INFINITY, st);
   const int nck = n_iter == 0 ? 0 : (n_iter + K - 1) / K;
   bool have_mu = false;  // MUd holds MU of the output state of the next iteration to reverse
   for (int c = nck - 1; c >= 0; --c) {
     const int i0 = c * K, i1 = std::min(i0 + K, n_iter);
+    auto state = [&](int s) {
+      return s == 0 ? StateBufs{ctx->ckF.p + c * nq, ctx->ckV.p + c * nq, ctx->ckQ.p + (size_t)c * Q1,
+                                ctx->ckP.p + (size_t)c * P}
+                    : StateBufs{ctx->sgF.p + s * nq, ctx->sgV.p + s * nq, ctx->sgQ.p + (size_t)s * Q1,
+                                ctx->sgP.p + (size_t)s * P};
+    };
     // Recompute the segment from its checkpoint, keeping every state and the walk's argmax rows.
-    HIPCHK(ctx, cpy(ctx->sgF.p, ctx->ckF.p + c * nq, nq * sizeof(float)));
-    HIPCHK(ctx, cpy(ctx->sgV.p, ctx->ckV.p + c * nq, nq * sizeof(float)));
-    HIPCHK(ctx, cpy(ctx->sgQ.p, ctx->ckQ.p + (size_t)c * Q1, Q1 * sizeof(QStat)));
-    HIPCHK(ctx, cpy(ctx->sgP.p, ctx->ckP.p + (size_t)c * P, P * sizeof(Lse2)));
-    HIPCHK(ctx, cpy(ctx->FMaxN.p, ctx->sgF.p, nq * sizeof(float)));
-    HIPCHK(ctx, cpy(ctx->VFm.p, ctx->sgV.p, nq * sizeof(float)));
-    HIPCHK(ctx, cpy(ctx->pstat.p, ctx->sgP.p, P * sizeof(Lse2)));
     for (int i = i0; i < i1; ++i) {
-      const size_t s = (size_t)(i - i0);
-      recompute_iteration(ctx, i, ctx->sgQ.p + s * Q1, ctx->sgQ.p + (s + 1) * Q1, ctx->tapeFA.p + s * nq);
-      HIPCHK(ctx, cpy(ctx->sgF.p + (s + 1) * nq, ctx->FMaxN.p, nq * sizeof(float)));
-      HIPCHK(ctx, cpy(ctx->sgV.p + (s + 1) * nq, ctx->VFm.p, nq * sizeof(float)));
-      HIPCHK(ctx, cpy(ctx->sgP.p + (s + 1) * P, ctx->pstat.p, P * sizeof(Lse2)));
-      if (Q == 0) HIPCHK(ctx, cpy(ctx->sgQ.p + (s + 1) * Q1, ctx->sgQ.p + s * Q1, Q1 * sizeof(QStat)));
+      const int s = i - i0;
+      const int rc = recompute_iteration(ctx, i, state(s), state(s + 1), ctx->tapeFA.p + (size_t)s * nq);
+      if (rc != AC_OK) return rc;
     }
     // Reverse through the segment.
     for (int i = i1 - 1; i >= i0; --i) {
       const size_t s = (size_t)(i - i0);
-      const ac::BwdState S0{ctx->sgF.p + s * nq, ctx->sgV.p + s * nq, ctx->sgQ.p + s * Q1, ctx->sgP.p + s * P, i == 0};
-      const ac::BwdState S1{ctx->sgF.p + (s + 1) * nq, ctx->sgV.p + (s + 1) * nq, ctx->sgQ.p + (s + 1) * Q1,
-                            ctx->sgP.p + (s + 1) * P, 0};
+      const StateBufs B0 = state((int)s), B1 = state((int)s + 1);
+      const ac::BwdState S0{B0.F, B0.V, B0.qs, B0.pst, i == 0};
+      const ac::BwdState S1{B1.F, B1.V, B1.qs, B1.pst, 0};
       if (!have_mu) launch_mu(ctx, S1, st);
       ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
       ac::launch_bwd_out(G, ctx->MUd.p, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);

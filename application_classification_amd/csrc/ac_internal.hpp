@@ -78,6 +78,7 @@ struct VertexArgs {
   const float* CV;      // [n][P]  vertex costs
   const float* FMaxN;   // [n][Qs] normalised destination maxima (-inf: no in-edge)
   float* VFm;           // [n][Qs] VF - min out-edge cost (-inf: no out-edge)
+  float* VFm_out;       // mode A writes the new VFm here (== VFm: in place)
   const float* minCEo;  // [n][Qs] min out-edge cost (+inf: none)
   const float* lsO;     // [n][Qs] log1p(sum over other out-edges of exp(min - cost))
   float* VR;            // [n][VRs]

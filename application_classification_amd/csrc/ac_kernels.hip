@@ -1466,7 +1466,7 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
           if (mo != INFINITY) {
             const double xx = vf - (double)mo;
             a.VR[(v0 + r) * a.VRs + q] = (float)vr;
-            a.VFm[i] = (float)xx;
+            a.VFm_out[i] = (float)xx;
             lse_add_finite(lre, xx + (double)lo);
             lse_add_finite(lfe, (vr - (double)mo) + (double)lo);
           }
