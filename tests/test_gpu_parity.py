@@ -136,6 +136,45 @@ def test_data_without_edges(small):
     assert_parity(mu, ref, "data without edges")
 
 
+def _class_mix_graph(seed):
+    """2,000 vertices in four interleaved degree classes (isolated, in-edges only, out-edges only, both) plus one hub
+    with an in-edge from every source (over 1,024: pieces in several fixup chunks) and one with 400 out-edges."""
+    r = np.random.default_rng(seed)
+    n = 2000
+    cls = r.choice(4, size=n, p=[0.25, 0.15, 0.15, 0.45])  # 0 isolated, 1 in only, 2 out only, 3 both
+    srcs = np.flatnonzero((cls == 2) | (cls == 3))
+    dsts = np.flatnonzero((cls == 1) | (cls == 3))
+    s = r.choice(srcs, 12000)
+    d = r.choice(dsts, 12000)
+    hub_in, hub_out = dsts[0], srcs[1]
+    s = np.concatenate([s, srcs, np.full(400, hub_out)])
+    d = np.concatenate([d, np.full(len(srcs), hub_in), r.choice(dsts, 400)])
+    keep = s != d
+    key = np.unique(s[keep].astype(np.int64) * n + d[keep])
+    s, d = (key // n).astype(np.int32), (key % n).astype(np.int32)
+    data = W.Graph(r.normal(size=(n, 13)).astype(np.float32), r.normal(size=(len(s), 13)).astype(np.float32), s, d)
+    return data
+
+
+@pytest.mark.parametrize("prune", [0, 1])
+def test_vertex_classes_and_hubs(prune):
+    """The internal degree-class order, the isolated vertices' closed form, the flagged walk destinations and the
+    chunked fixup against the oracle, on a graph where every class and both hub kinds occur."""
+    from application_classification_amd.matcher import AC_OPT_PRUNE
+    data = _class_mix_graph(7)
+    indeg = np.bincount(data.dst, minlength=data.num_nodes)
+    outdeg = np.bincount(data.src, minlength=data.num_nodes)
+    assert ((indeg == 0) & (outdeg == 0)).sum() > 300 and indeg.max() > 1024
+    assert ((indeg > 0) & (outdeg == 0)).any() and ((indeg == 0) & (outdeg > 0)).any()
+    patt = W.planted_pattern(data, 8, 12, seed=3)
+    with Matcher(0) as m:
+        m.set_option(AC_OPT_PRUNE, prune)
+        m.set_data(data)
+        m.set_pattern(patt)
+        mu = m.run()
+    assert_parity(mu, O.run(g64(data), g64(patt)), f"degree classes and hubs, prune={prune}")
+
+
 def test_single_data_vertex():
     r = np.random.default_rng(2)
     data = W.Graph(r.normal(size=(1, 3)).astype(np.float32), r.normal(size=(1, 2)).astype(np.float32),
