@@ -26,6 +26,7 @@ AC_OPT_SLAB_EDGES = 3
 AC_OPT_TAPE = 4
 AC_OPT_PRUNE = 5
 AC_OPT_CDIST_MFMA = 6
+AC_OPT_PACK_COSTS = 7
 STAGES = ["prep", "vertex_a", "reduce_a", "edge_walk", "walk_fixup", "vertex_c", "reduce_c", "output"]
 
 ERROR_NAMES = {
@@ -89,6 +90,7 @@ SIGNATURES = {
     "ac_get_match": (I, [P, F, P]),
     "ac_get_sizes": (I, [P, P]),
     "ac_get_timing": (I, [P, ctypes.POINTER(D), ctypes.POINTER(D)]),
+    "ac_get_cost_packing": (I, [P, P]),
     "ac_stream": (P, [P]),
     "ac_run_profiled": (I, [P, I, P]),
     "ac_set_option": (I, [P, I, I]),

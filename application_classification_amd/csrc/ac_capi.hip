@@ -196,6 +196,9 @@ struct ac_ctx {
   int walk_B = 0;          // steps per batch of the edge walk (0: default)
   bool use_graph = false;  // replay iteration pairs from a captured graph
   bool prune = true;       // skip bundles whose edges cannot raise FMax above the fill (exact)
+  bool pack_costs = true;  // the plain walk reads 24-bit codes of the costs where a bundle allows them (exact)
+  DevBuf<unsigned> CEp;    // [rows * Qs * 3 / 4] packed costs of the bundles whose base is nonzero
+  DevBuf<unsigned> bpack;  // [bundles] base of a bundle's codes, 0 = FP32 rows
   bool use_mfma_cdist = true;  // feature distances on FP64 matrix cores (GEMM form, cancellation-guarded)
   DevBuf<double> cd_scratch;   // the matrix-core operands of the pattern features
   DevBuf<float> bmin;      // [bundles][Qs] least cost per bundle and pattern edge
@@ -745,6 +748,11 @@ static int enqueue_prep(ac_ctx* ctx) {
     ac::launch_prep_scalars(P, Q, ctx->d_ps.p, ctx->d_pd.p, ctx->lse_cv.p, ctx->lse_ce.p, qs_prep, st);
     HIPCHK(ctx, ctx->bmin.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles) * Qs));
     ac::launch_bundle_min(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Qs, ctx->bmin.p, st);
+    if (ctx->pack_costs) {
+      HIPCHK(ctx, ctx->CEp.alloc((size_t)std::max<int64_t>(1, ctx->bs->brows * (Qs / 4) * 3)));
+      HIPCHK(ctx, ctx->bpack.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles)));
+      ac::launch_pack_costs(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Q, Qs, ctx->bpack.p, ctx->CEp.p, st);
+    }
     const bool tape = ctx->tape_K > 0;
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, nullptr, ctx->VRs, Q, Qs, qs_prep, ctx->FMaxN.p, ctx->carry_in.p,
@@ -819,7 +827,8 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
     MARK(AC_STAGE_REDUCE_A);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p,
-                        walk_batch_of(ctx), st, nullptr, nullptr, ctx->prune ? ctx->bmin.p : nullptr);
+                        walk_batch_of(ctx), st, nullptr, nullptr, ctx->prune ? ctx->bmin.p : nullptr,
+                        ctx->pack_costs ? ctx->CEp.p : nullptr, ctx->pack_costs ? ctx->bpack.p : nullptr);
     MARK(AC_STAGE_EDGE_WALK);
     launch_fixup32(ctx, st);
     MARK(AC_STAGE_WALK_FIXUP);
@@ -996,6 +1005,23 @@ extern "C" int ac_get_sizes(const ac_ctx* ctx, int64_t* out4) {
   return AC_OK;
 }
 
+extern "C" int ac_get_cost_packing(ac_ctx* ctx, int64_t* out2) {
+  if (!ctx || !out2) return AC_ERR_ARG;
+  if (!ctx->have_result || !ctx->bs || ctx->Q == 0)
+    return ctx->fail(AC_ERR_STATE, "ac_get_cost_packing: no run with pattern edges yet");
+  const int64_t nb = ctx->bs->nbundles;
+  int64_t packed = 0;
+  if (ctx->pack_costs && nb > 0) {
+    std::vector<unsigned> h((size_t)nb);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(h.data(), ctx->bpack.p, sizeof(unsigned) * (size_t)nb, hipMemcpyDeviceToHost));
+    for (unsigned b : h) packed += b != 0u;
+  }
+  out2[0] = nb;
+  out2[1] = packed;
+  return AC_OK;
+}
+
 extern "C" int ac_get_timing(const ac_ctx* ctx, double* prep_ms, double* loop_ms) {
   if (!ctx) return AC_ERR_ARG;
   if (!ctx->have_result) return AC_ERR_STATE;
@@ -1084,6 +1110,9 @@ extern "C" int ac_set_option(ac_ctx* ctx, int option, int value) {
     case AC_OPT_CDIST_MFMA:
       ctx->use_mfma_cdist = value != 0;
       return AC_OK;
+    case AC_OPT_PACK_COSTS:
+      ctx->pack_costs = value != 0;
+      return AC_OK;
     default:
       return ctx->fail(AC_ERR_ARG, "ac_set_option: unknown option " + std::to_string(option));
   }
@@ -1109,6 +1138,9 @@ extern "C" int ac_get_option(const ac_ctx* ctx, int option, int* value) {
       return AC_OK;
     case AC_OPT_CDIST_MFMA:
       *value = ctx->use_mfma_cdist ? 1 : 0;
+      return AC_OK;
+    case AC_OPT_PACK_COSTS:
+      *value = ctx->pack_costs ? 1 : 0;
       return AC_OK;
     default:
       return AC_ERR_ARG;

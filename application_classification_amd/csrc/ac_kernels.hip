@@ -14,9 +14,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cmath>
 #include <climits>
+#include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 #include "ac_internal.hpp"
 
@@ -45,13 +46,35 @@ __device__ __forceinline__ int2 nt_load_int2(const int2* p) {
   return make_int2(x.x, x.y);
 }
 
+// Packed costs: four 24-bit codes in three dwords, c0 | c1 << 24, c1 >> 8 | c2 << 16, c2 >> 16 | c3 << 8.  A code
+// is the cost's bit pattern minus the bundle's base (a biased exponent << 23, so one bundle spans two binades);
+// code 0 stands for +inf (padding rows and columns past Q).
+struct U3 {
+  unsigned x, y, z;
+};
+__device__ __forceinline__ U3 nt_load_u3(const unsigned* p) {
+  return U3{__builtin_nontemporal_load(p), __builtin_nontemporal_load(p + 1), __builtin_nontemporal_load(p + 2)};
+}
+__device__ __forceinline__ float decode_cost(unsigned c, unsigned base) {
+  return c ? __uint_as_float(base + c) : INFINITY;
+}
+__device__ __forceinline__ f4v decode_costs(U3 w, unsigned base) {
+  const unsigned c0 = w.x & 0xFFFFFFu;
+  const unsigned c1 = __builtin_amdgcn_alignbit(w.y, w.x, 24) & 0xFFFFFFu;
+  const unsigned c2 = __builtin_amdgcn_alignbit(w.z, w.y, 16) & 0xFFFFFFu;
+  const unsigned c3 = w.z >> 8;
+  return f4v{decode_cost(c0, base), decode_cost(c1, base), decode_cost(c2, base), decode_cost(c3, base)};
+}
+
 // One exp per value and no branches: the exponent is always min - max of (x, m), which rescales t when x is a new
-// maximum and is the new term otherwise.  exp(-inf) = 0 covers the empty accumulator.
+// maximum and is the new term otherwise.  exp(-inf) = 0 covers the empty accumulator.  min - max is formed as
+// -|x - m|: rounding to nearest is sign-symmetric, so fl(min - max) = -|fl(x - m)| exactly, and x > m exactly when
+// fl(x - m) > 0 (the difference of two distinct finite doubles never rounds to zero; m = -inf gives +inf).
 __device__ __forceinline__ void lse_add(Lse& s, double x) {
   const bool valid = x != -kInf;
-  const bool up = x > s.m;
-  const double e = (double)__expf((float)(fmin(x, s.m) - fmax(x, s.m)));
-  const double t = up ? (1.0 + s.t) * e : s.t + e;
+  const double d = x - s.m;  // NaN only for x = m = -inf, which is not valid
+  const double e = (double)__expf((float)(-fabs(d)));
+  const double t = d > 0.0 ? (1.0 + s.t) * e : s.t + e;
   s.t = valid ? t : s.t;
   s.m = valid ? fmax(x, s.m) : s.m;
 }
@@ -59,9 +82,9 @@ __device__ __forceinline__ void lse_add(Lse& s, double x) {
 // lse_add for a finite x (no -inf term to skip): the same result with fewer selects.  An empty accumulator
 // (m = -inf) takes x as its maximum: exp(-inf) = 0 clears t.
 __device__ __forceinline__ void lse_add_finite(Lse& s, double x) {
-  const bool up = x > s.m;
-  const double e = (double)__expf((float)(fmin(x, s.m) - fmax(x, s.m)));
-  s.t = up ? (1.0 + s.t) * e : s.t + e;
+  const double d = x - s.m;
+  const double e = (double)__expf((float)(-fabs(d)));
+  s.t = d > 0.0 ? (1.0 + s.t) * e : s.t + e;
   s.m = fmax(x, s.m);
 }
 
@@ -721,7 +744,9 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
                                                          const float* __restrict__ VR, int VRs, int Q, int Qs,
                                                          const QStat* __restrict__ qs, float* __restrict__ FMaxN,
                                                          double* __restrict__ carry, int* __restrict__ FA,
-                                                         int* __restrict__ carg, const float* __restrict__ bmin) {
+                                                         int* __restrict__ carg, const float* __restrict__ bmin,
+                                                         const unsigned* __restrict__ CEp,
+                                                         const unsigned* __restrict__ bpack) {
   const int L4 = Qs / 4, G = 64 / L4, VR4s = VRs / 4;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane / L4, j = lane - g * L4;
@@ -763,19 +788,19 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   double mx0 = -kInf, mx1 = -kInf, mx2 = -kInf, mx3 = -kInf;
   int a0 = -1, a1 = -1, a2 = -1, a3 = -1;
   int cur = -1;
-  auto emit = [&](int v) {
-    float4 o;
-    o.x = (float)(mx0 - sl[0]);
-    o.y = (float)(mx1 - sl[1]);
-    o.z = (float)(mx2 - sl[2]);
-    o.w = (float)(mx3 - sl[3]);
-    reinterpret_cast<float4*>(FMaxN)[(int64_t)v * L4 + j] = o;
-    if (kArg) {
-      reinterpret_cast<int4*>(FA)[(int64_t)v * L4 + j] = make_int4(a0, a1, a2, a3);
-      a0 = a1 = a2 = a3 = -1;
-    }
-    mx0 = mx1 = mx2 = mx3 = -kInf;
-  };
+  // Finished vertex v: its row of maxima, normalised (a macro rather than a lambda, so that the walk's loop-carried
+  // state is not captured by reference twice over and stays in registers).
+#define AC_WK_EMIT(V)                                                                           \
+  {                                                                                             \
+    const f4v o_ = f4v{(float)(mx0 - sl[0]), (float)(mx1 - sl[1]), (float)(mx2 - sl[2]),         \
+                       (float)(mx3 - sl[3])};                                                   \
+    reinterpret_cast<f4v*>(FMaxN)[(int64_t)(V) * L4 + j] = o_;                                  \
+    if (kArg) {                                                                                 \
+      reinterpret_cast<int4*>(FA)[(int64_t)(V) * L4 + j] = make_int4(a0, a1, a2, a3);           \
+      a0 = a1 = a2 = a3 = -1;                                                                   \
+    }                                                                                           \
+    mx0 = mx1 = mx2 = mx3 = -kInf;                                                              \
+  }
   const int GL4 = G * L4;
   // Streams: a wave-uniform bundle base plus small per-lane offsets (32-bit address arithmetic).
   const int* srcb = bsrc + Bd.base;
@@ -788,6 +813,9 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   const int* dstp = bdst + Bd.base + g;
   const f4v* VRv = reinterpret_cast<const f4v*>(VR);
   const int last = Bd.len / B - 1;
+  // Packed costs (plain walk only, CEp != nullptr): the bundle's base, 0 when it keeps its FP32 rows.
+  const unsigned pbase = kFlags && CEp ? bpack[b] : 0u;
+  const unsigned* cpb = kFlags && CEp ? CEp + Bd.base * L4 * 3 : nullptr;
 #define AC_WK_SRC(S, T)                                                          \
   if (kWithVR) {                                                                 \
     const int t_ = (T) < last ? (T) : last;                                      \
@@ -797,8 +825,12 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
 #define AC_WK_BATCH(C, D, W, S, T)                                                                       \
   {                                                                                                      \
     const int t_ = (T) < last ? (T) : last;                                                              \
-    _Pragma("unroll") for (int u = 0; u < B; ++u)                                                        \
-      C[u] = __builtin_nontemporal_load(ceb + (co + (unsigned)((t_ * B + u) * GL4)));                    \
+    _Pragma("unroll") for (int u = 0; u < B; ++u) {                                                      \
+      if constexpr (kP)                                                                                  \
+        C[u] = nt_load_u3(cpb + 3 * (co + (unsigned)((t_ * B + u) * GL4)));                              \
+      else                                                                                               \
+        C[u] = __builtin_nontemporal_load(ceb + (co + (unsigned)((t_ * B + u) * GL4)));                  \
+    }                                                                                                    \
     _Pragma("unroll") for (int u = 0; u < B; ++u) D[u] =                                                 \
         kFlags ? (S[u] & kNewVertex) : __builtin_nontemporal_load(dstp + (t_ * B + u) * G);              \
     _Pragma("unroll") for (int u = 0; u < B; ++u) W[u] =                                                 \
@@ -808,17 +840,22 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   _Pragma("unroll") for (int u = 0; u < B; ++u) {                                   \
     if (kFlags) {                                                                   \
       if (D[u]) {                                                                   \
-        emit(cur);                                                                  \
+        AC_WK_EMIT(cur);                                                            \
         ++cur;                                                                      \
       }                                                                             \
     } else if (D[u] != cur) {                                                       \
-      if (cur >= 0) emit(cur);                                                      \
+      if (cur >= 0) AC_WK_EMIT(cur);                                                \
       cur = D[u];                                                                   \
     }                                                                               \
-    const double d0 = (double)W[u].x - (double)C[u].x;                              \
-    const double d1 = (double)W[u].y - (double)C[u].y;                              \
-    const double d2 = (double)W[u].z - (double)C[u].z;                              \
-    const double d3 = (double)W[u].w - (double)C[u].w;                              \
+    f4v c_;                                                                         \
+    if constexpr (kP)                                                               \
+      c_ = decode_costs(C[u], pbase);                                               \
+    else                                                                            \
+      c_ = C[u];                                                                    \
+    const double d0 = (double)W[u].x - (double)c_.x;                                \
+    const double d1 = (double)W[u].y - (double)c_.y;                                \
+    const double d2 = (double)W[u].z - (double)c_.z;                                \
+    const double d3 = (double)W[u].w - (double)c_.w;                                \
     if (kArg) {                                                                     \
       const int row_ = (int)(Bd.base + g) + ((T) * B + u) * G;                       \
       if (d0 > mx0) mx0 = d0, a0 = row_;                                            \
@@ -832,24 +869,37 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
       mx3 = fmax(mx3, d3);                                                          \
     }                                                                               \
   }
-  int SA[B], SB[B], DA[B], DB[B];
-  f4v CA[B], CB[B], WA[B], WB[B];
-#pragma unroll
-  for (int u = 0; u < B; ++u) SA[u] = SB[u] = 0;
-  AC_WK_SRC(SA, 0);
-  AC_WK_SRC(SB, 1);
-  AC_WK_BATCH(CA, DA, WA, SA, 0);
-  AC_WK_SRC(SA, 2);
-  for (int t = 0;; t += 2) {
-    AC_WK_BATCH(CB, DB, WB, SB, t + 1);
-    AC_WK_SRC(SB, t + 3);
-    AC_WK_REDUCE(CA, DA, WA, t);
-    if (t + 1 > last) break;
-    AC_WK_BATCH(CA, DA, WA, SA, t + 2);
-    AC_WK_SRC(SA, t + 4);
-    AC_WK_REDUCE(CB, DB, WB, t + 1);
-    if (t + 2 > last) break;
+  // The pipelined walk over the bundle, for FP32 cost rows (kP false) or packed ones (decoded when reduced, so the
+  // loads stay in flight as raw dwords).  kP is written in terms of kFlags so that it is value-dependent: the
+  // discarded `if constexpr` branches are then not instantiated.
+#define AC_WK_PIPELINE(KP)                                                   \
+  {                                                                          \
+    constexpr bool kP = (KP);                                                \
+    typedef typename std::conditional<kP, U3, f4v>::type CostRow;            \
+    int SA[B], SB[B], DA[B], DB[B];                                          \
+    CostRow CA[B], CB[B];                                                    \
+    f4v WA[B], WB[B];                                                        \
+    _Pragma("unroll") for (int u = 0; u < B; ++u) SA[u] = SB[u] = 0;         \
+    AC_WK_SRC(SA, 0);                                                        \
+    AC_WK_SRC(SB, 1);                                                        \
+    AC_WK_BATCH(CA, DA, WA, SA, 0);                                          \
+    AC_WK_SRC(SA, 2);                                                        \
+    for (int t = 0;; t += 2) {                                               \
+      AC_WK_BATCH(CB, DB, WB, SB, t + 1);                                    \
+      AC_WK_SRC(SB, t + 3);                                                  \
+      AC_WK_REDUCE(CA, DA, WA, t);                                           \
+      if (t + 1 > last) break;                                               \
+      AC_WK_BATCH(CA, DA, WA, SA, t + 2);                                    \
+      AC_WK_SRC(SA, t + 4);                                                  \
+      AC_WK_REDUCE(CB, DB, WB, t + 1);                                       \
+      if (t + 2 > last) break;                                               \
+    }                                                                        \
   }
+  if (kFlags && pbase)
+    AC_WK_PIPELINE(kFlags)
+  else
+    AC_WK_PIPELINE(false && kFlags)
+#undef AC_WK_PIPELINE
 #undef AC_WK_SRC
 #undef AC_WK_BATCH
 #undef AC_WK_REDUCE
@@ -866,48 +916,51 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
     reinterpret_cast<f4v*>(carry)[(int64_t)slot * L4 + j] = f4v{(float)(mx0 - sl[0]), (float)(mx1 - sl[1]),
                                                                 (float)(mx2 - sl[2]), (float)(mx3 - sl[3])};
   } else if (cur >= 0) {
-    emit(cur);
+    AC_WK_EMIT(cur);
   }
+#undef AC_WK_EMIT
 }
 
 template <bool kWithVR, bool kArg>
 static void launch_walk_max_t(int B, int nb, hipStream_t st, int64_t nbundles, const Bundle* bundles,
                               const int* gslot, const int* bsrc, const int* bdst, const float* CE, const float* VR,
                               int VRs, int Q, int Qs, const QStat* qs, float* FMaxN, double* carry, int* FA,
-                              int* carg, const float* bmin) {
+                              int* carg, const float* bmin, const unsigned* CEp, const unsigned* bpack) {
   switch (B) {
     case 1:
       hipLaunchKernelGGL((k_walk_max<kWithVR, 1, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
-                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin);
+                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
       break;
     case 4:
       hipLaunchKernelGGL((k_walk_max<kWithVR, 4, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
-                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin);
+                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
       break;
     default:
       hipLaunchKernelGGL((k_walk_max<kWithVR, 2, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
-                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin);
+                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
       break;
   }
 }
 
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
-                     double* carry, int B, hipStream_t st, int* FA, int* carg, const float* bmin) {
+                     double* carry, int B, hipStream_t st, int* FA, int* carg, const float* bmin,
+                     const unsigned* CEp, const unsigned* bpack) {
   if (nbundles == 0) return;
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
+  if (!bpack) CEp = nullptr;
   if (VR && FA)
     launch_walk_max_t<true, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                  carry, FA, carg, bmin);
+                                  carry, FA, carg, bmin, nullptr, nullptr);
   else if (VR)
     launch_walk_max_t<true, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                   carry, FA, carg, bmin);
+                                   carry, FA, carg, bmin, CEp, bpack);
   else if (FA)
     launch_walk_max_t<false, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                   carry, FA, carg, bmin);
+                                   carry, FA, carg, bmin, nullptr, nullptr);
   else
     launch_walk_max_t<false, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                    carry, FA, carg, bmin);
+                                    carry, FA, carg, bmin, nullptr, nullptr);
 }
 
 // Per bundle and pattern edge, the least cost among the bundle's rows (padding rows are +inf): bmin[b][Qs].  One
@@ -945,6 +998,72 @@ void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE,
   if (nbundles == 0) return;
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
   hipLaunchKernelGGL(k_bundle_min, dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, CE, Qs, bmin);
+}
+
+// Lossless 24-bit coding of a bundle's costs for the plain walk (one wave per bundle, lanes as in the walk).  The
+// costs of columns q < Q other than +inf must lie strictly inside two binades (2^e, 2^(e+2)) of one biased exponent
+// e >= 1: then each is its bit pattern minus base = e << 23, a code in [1, 2^24), and +inf (padding rows) codes as
+// 0.  Positive floats order as their bit patterns, so the bundle's least and largest patterns decide: a least cost
+// that is a power of two widens the window down one binade, and negative, zero, subnormal or NaN costs fall outside
+// every window.  bpack[b] = base, or 0 when the bundle keeps its FP32 rows (the walk then reads CE).
+__global__ __launch_bounds__(kWalkBlock) void k_pack_costs(int64_t nbundles, const Bundle* __restrict__ bundles,
+                                                           const float* __restrict__ CE, int Q, int Qs,
+                                                           unsigned* __restrict__ bpack, unsigned* __restrict__ CEp) {
+  const int L4 = Qs / 4, G = 64 / L4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane / L4, j = lane - g * L4;
+  const int64_t b = (int64_t)blockIdx.x * (kWalkBlock / 64) + wave;
+  if (b >= nbundles) return;  // wave-uniform
+  const Bundle Bd = bundles[b];
+  const bool act = g < G;
+  const f4v* ce = reinterpret_cast<const f4v*>(CE) + (Bd.base + g) * L4 + j;
+  unsigned lo = 0xFFFFFFFFu, hi = 0u;
+  if (act)
+    for (int s = 0; s < Bd.len; ++s) {
+      const f4v c = __builtin_nontemporal_load(ce + (int64_t)s * G * L4);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const unsigned x = __float_as_uint(c[u]);
+        if (4 * j + u < Q && x != 0x7F800000u) {
+          lo = min(lo, x);
+          hi = max(hi, x);
+        }
+      }
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, (unsigned)__shfl_xor((int)lo, o, 64));
+    hi = max(hi, (unsigned)__shfl_xor((int)hi, o, 64));
+  }
+  unsigned base = 0u;
+  if (lo <= hi && hi < 0x7F800000u) {
+    unsigned e = lo >> 23;
+    if ((lo & 0x7FFFFFu) == 0u) e = e > 0u ? e - 1u : 0u;  // code 0 is +inf: the least cost must lie above the base
+    if (e >= 1u && hi - (e << 23) < (1u << 24)) base = e << 23;
+  }
+  if (lane == 0) bpack[b] = base;
+  if (base == 0u || !act) return;
+  unsigned* cp = CEp + ((Bd.base + g) * L4 + j) * 3;
+  for (int s = 0; s < Bd.len; ++s) {
+    const f4v c = __builtin_nontemporal_load(ce + (int64_t)s * G * L4);
+    unsigned k[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const unsigned x = __float_as_uint(c[u]);
+      k[u] = (4 * j + u < Q && x != 0x7F800000u) ? x - base : 0u;
+    }
+    unsigned* o = cp + (int64_t)s * G * L4 * 3;
+    o[0] = k[0] | (k[1] << 24);
+    o[1] = (k[1] >> 8) | (k[2] << 16);
+    o[2] = (k[2] >> 16) | (k[3] << 8);
+  }
+}
+
+void launch_pack_costs(int64_t nbundles, const Bundle* bundles, const float* CE, int Q, int Qs, unsigned* bpack,
+                       unsigned* CEp, hipStream_t st) {
+  if (nbundles == 0) return;
+  const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
+  hipLaunchKernelGGL(k_pack_costs, dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, CE, Q, Qs, bpack, CEp);
 }
 
 // Bundle layout from the CSC arrays: one block per bundle writes its rows (step-major, run-minor).  rowmap[k] is
@@ -1247,9 +1366,12 @@ __device__ Lse iso_lse(const VertexArgs& a, int p) {
 // NT threads per block (kVW = NT / 64 waves).  kPW > 0 (mode C with P <= kVW * kPW): each lane accumulates its rows' log-sum-exp of its wave's kPW pattern vertices
 // in registers over all tiles, merged across the wave once at the end.  kPW == 0: one wave-level reduction per tile.
 // kArgMax (mode A): also track the vertices of the column maxima (the adjoint's tape needs them).
-template <int kMode, int kPW, bool kArgMax, int NT>
+// kPFC: CV values each thread prefetches per tile (VT * P <= NT * kPFC); fewer registers allow more resident waves.
+// kDepth: tiles whose loads are in flight per block (1 or 2 register sets, used by alternate tiles).
+template <int kMode, int kPW, bool kArgMax, int NT, int kPFC = kTileFloats / NT, int kDepth = 1>
 __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
-  constexpr int kVW = NT / 64, kPF4 = kTileFloats / 4 / NT, kPFC = kTileFloats / NT;
+  static_assert(kDepth == 1 || kDepth == 2, "one or two prefetch register sets");
+  constexpr int kVW = NT / 64, kPF4 = kTileFloats / 4 / NT;
   constexpr bool kA = kMode == kModeA;
   constexpr bool kRows = kMode == kModeOut || kMode == kModeMU;  // MU tile staged, written back row by row
   extern __shared__ double lds[];
@@ -1302,6 +1424,9 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   const int RQ = Q > 0 ? NT / Q : 0;
   const bool qthread = t < RQ * Q;
   const int q = qthread ? t % Q : 0, rq = qthread ? t / Q : 0;
+  // Read once: loads inside the tile loop would follow the loop's stores (possible aliases) and wait for every
+  // outstanding load, the next tile's prefetch included.
+  const int q_ps = kA && qthread ? a.ps[q] : 0, q_pd = kA && qthread ? a.pd[q] : 0;
   double vfmax = -kInf, vrmax = -kInf;
   int avf = -1, avr = -1;
   Lse lre = lse_empty(), lfe = lse_empty();
@@ -1316,30 +1441,33 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   Lse pacc[kPW > 0 ? kPW : 1];
 #pragma unroll
   for (int pi = 0; pi < (kPW > 0 ? kPW : 1); ++pi) pacc[pi] = lse_empty();
-  f4v pf[4][kPF4];
-  float pc[kPFC];
-  // Loads of a tile into registers; indexes past the tile are clamped to its last element (loaded, never used).
-#define AC_VX_PREFETCH(TILE)                                                         \
-  {                                                                                  \
-    const int64_t v0_ = (TILE) * VT;                                                 \
-    const int nv_ = (int)((a.n - v0_) < VT ? (a.n - v0_) : VT);                      \
-    const int last4_ = nv_ * Qs / 4 - 1, lastc_ = nv_ * P - 1;                       \
-    const int64_t b4_ = v0_ * Qs / 4, bc_ = v0_ * P;                                 \
-    _Pragma("unroll") for (int k = 0; k < kPF4; ++k) {                               \
-      const int64_t g_ = b4_ + min(t + k * NT, last4_);                              \
-      const bool hasF_ = !kA || g_ < f_end4, hasO_ = !kA || g_ < o_lo4 || (g_ >= o_hi4 && g_ < o_end4);  \
-      pf[0][k] = hasF_ ? F4[g_] : f4v{-INFINITY, -INFINITY, -INFINITY, -INFINITY};   \
-      pf[1][k] = hasO_ ? V4[g_] : f4v{-INFINITY, -INFINITY, -INFINITY, -INFINITY};   \
-      if (kA) {                                                                      \
-        pf[2][k] = hasO_ ? M4[g_] : f4v{INFINITY, INFINITY, INFINITY, INFINITY};     \
-        pf[3][k] = hasO_ ? L4v[g_] : f4v{0.f, 0.f, 0.f, 0.f};                       \
-      }                                                                              \
-    }                                                                                \
-    _Pragma("unroll") for (int k = 0; k < kPFC; ++k) pc[k] = a.CV[bc_ + min(t + k * NT, lastc_)]; \
-  }
-  int64_t tile = blockIdx.x;
-  if (tile < ntiles) AC_VX_PREFETCH(tile);
-  for (; tile < ntiles; tile += gridDim.x) {
+  typedef f4v PfSet[4][kPF4];  // FMaxN, VFm (and in pass A minCEo, lsO)
+  PfSet pfA, pfB;
+  float pcA[kPFC], pcB[kPFC];
+  // Loads of a tile into one register set; indexes past the tile are clamped to its last element (loaded, never
+  // used).
+  auto prefetch = [&](int64_t tl, PfSet& pf, float (&pc)[kPFC]) __attribute__((always_inline)) {
+    const int64_t v0_ = tl * VT;
+    const int nv_ = (int)((a.n - v0_) < VT ? (a.n - v0_) : VT);
+    const int last4_ = nv_ * Qs / 4 - 1, lastc_ = nv_ * P - 1;
+    const int64_t b4_ = v0_ * Qs / 4, bc_ = v0_ * P;
+#pragma unroll
+    for (int k = 0; k < kPF4; ++k) {
+      const int64_t g_ = b4_ + min(t + k * NT, last4_);
+      const bool hasF_ = !kA || g_ < f_end4, hasO_ = !kA || g_ < o_lo4 || (g_ >= o_hi4 && g_ < o_end4);
+      pf[0][k] = hasF_ ? F4[g_] : f4v{-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      pf[1][k] = hasO_ ? V4[g_] : f4v{-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      if (kA) {
+        pf[2][k] = hasO_ ? M4[g_] : f4v{INFINITY, INFINITY, INFINITY, INFINITY};
+        pf[3][k] = hasO_ ? L4v[g_] : f4v{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPFC; ++k) pc[k] = a.CV[bc_ + min(t + k * NT, lastc_)];
+  };
+  const int64_t G = gridDim.x;
+  // One tile: stage its register set in LDS, refill the set with the tile kDepth tiles ahead, compute.
+  auto run_tile = [&](int64_t tile, PfSet& pf, float (&pc)[kPFC]) __attribute__((always_inline)) {
     const int64_t v0 = tile * VT;
     const int nv = (int)((a.n - v0) < VT ? (a.n - v0) : VT);
     __syncthreads();  // tables written (first tile) / previous tile consumed
@@ -1372,7 +1500,7 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
       }
     }
     __syncthreads();
-    if (tile + gridDim.x < ntiles) AC_VX_PREFETCH(tile + gridDim.x);
+    if (tile + kDepth * G < ntiles) prefetch(tile + kDepth * G, pf, pc);
     // p-phase: wave-uniform pattern vertex, lane = tile row.
     // MU_pre of row `r` and pattern vertex `p` (-inf for rows past the tile).
 #define AC_VX_MU_PRE(X, P_, R_)                                                                              \
@@ -1386,10 +1514,11 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
             const int code_ = c_inc[k_];                                                                   \
             const int qq_ = code_ >> 1;                                                                    \
             if (code_ & 1) {                                                                               \
+              /* scalar reads first, so the -inf guard below is a select, not a branch */                  \
+              const double fR_ = c_qs[qq_].fillR, mR_ = c_qs[qq_].mRE, lR_ = c_qs[qq_].lRE;                \
               const float vfm_ = Vs[(R_) * ldq + qq_];                                                     \
-              const double rn_ =                                                                           \
-                  vfm_ == -INFINITY ? -kInf : ((double)vfm_ - c_qs[qq_].mRE) - c_qs[qq_].lRE;              \
-              acc_ += fmax(c_qs[qq_].fillR, rn_);                                                          \
+              const double d_ = ((double)vfm_ - mR_) - lR_;                                                \
+              acc_ += fmax(fR_, vfm_ == -INFINITY ? -kInf : d_);                                           \
             } else {                                                                                       \
               acc_ += fmax(c_qs[qq_].fillF, (double)Fs[(R_) * ldq + qq_]);                                 \
             }                                                                                              \
@@ -1438,7 +1567,6 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
       __syncthreads();
       if (qthread) {
         const double fF = tab[q], fR = tab[Q + q], mR = tab[2 * Q + q], lR = tab[3 * Q + q];
-        const int q_ps = a.ps[q], q_pd = a.pd[q];
         for (int r = rq; r < nv; r += RQ) {
           const int64_t i = (v0 + r) * Qs + q;
           const float mo = Ms[r * ldq + q];
@@ -1473,8 +1601,14 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
         }
       }
     }
+  };
+  int64_t tile = blockIdx.x;
+  if (tile < ntiles) prefetch(tile, pfA, pcA);
+  if (kDepth > 1 && tile + G < ntiles) prefetch(tile + G, pfB, pcB);
+  for (; tile < ntiles; tile += kDepth * G) {
+    run_tile(tile, pfA, pcA);
+    if (kDepth > 1 && tile + G < ntiles) run_tile(tile + G, pfB, pcB);
   }
-#undef AC_VX_PREFETCH
   __syncthreads();
 
   // Block partials, merged in a fixed order.
@@ -1502,25 +1636,60 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
 }
 
 // Pass A runs 256-thread blocks (its register use), the lighter modes 512-thread blocks (measured at R-MAT 22).
-void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st) {
-  constexpr int kNA = 256, kNO = 512;
+// Pass A tiles whose CV rows fit 4 values per thread take the variant with the shorter CV prefetch: 120 instead of
+// 132 VGPRs, 4 instead of 3 resident waves per SIMD.
+constexpr int kNA = 256, kNO = 512;
+static bool short_cv(const VertexArgs& a) { return a.VT * a.P <= kNA * 4; }
+
+typedef void (*VertexKernel)(VertexArgs);
+
+// Prefetch depth of a pass: 1 or 2 tiles in flight per block (tuning override AC_VERTEX_DEPTH_A / _C).
+static int vertex_depth(const char* env, int dflt) {
+  if (const char* e = std::getenv(env)) {
+    const int d = std::atoi(e);
+    if (d == 1 || d == 2) return d;
+  }
+  return dflt;
+}
+
+template <bool kArg, int kPFC>
+static VertexKernel kernel_a(int depth) {
+  return depth == 2 ? k_vertex<kModeA, 0, kArg, kNA, kPFC, 2> : k_vertex<kModeA, 0, kArg, kNA, kPFC, 1>;
+}
+template <int kPW>
+static VertexKernel kernel_c(int depth) {
+  return depth == 2 ? k_vertex<kModeC, kPW, false, kNO, kTileFloats / kNO, 2>
+                    : k_vertex<kModeC, kPW, false, kNO, kTileFloats / kNO, 1>;
+}
+
+// The kernel a pass launches, its block size and dynamic LDS bytes.
+static VertexKernel vertex_kernel(int mode, const VertexArgs& a, int* nt, size_t* lds) {
+  static const int depth_a = vertex_depth("AC_VERTEX_DEPTH_A", 1), depth_c = vertex_depth("AC_VERTEX_DEPTH_C", 1);
   const size_t tiles = vertex_lds_bytes(mode, a.VT, a.P, a.Q, a.Qs);
-  const size_t red = mode == kModeA ? sizeof(QPart) * kNA : 0;
-  const size_t lds = std::max(tiles, red);
-  if (mode == kModeA && a.track_arg)
-    hipLaunchKernelGGL((k_vertex<kModeA, 0, true, kNA>), dim3(nblocks), dim3(kNA), lds, st, a);
-  else if (mode == kModeA)
-    hipLaunchKernelGGL((k_vertex<kModeA, 0, false, kNA>), dim3(nblocks), dim3(kNA), lds, st, a);
-  else if (mode == kModeC && a.P <= 4 * (kNO / 64))
-    hipLaunchKernelGGL((k_vertex<kModeC, 4, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
-  else if (mode == kModeC && a.P <= 8 * (kNO / 64))
-    hipLaunchKernelGGL((k_vertex<kModeC, 8, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
-  else if (mode == kModeC)
-    hipLaunchKernelGGL((k_vertex<kModeC, 0, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
-  else if (mode == kModeMU)
-    hipLaunchKernelGGL((k_vertex<kModeMU, 0, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
-  else
-    hipLaunchKernelGGL((k_vertex<kModeOut, 0, false, kNO>), dim3(nblocks), dim3(kNO), lds, st, a);
+  *nt = mode == kModeA ? kNA : kNO;
+  *lds = mode == kModeA ? std::max(tiles, sizeof(QPart) * kNA) : tiles;
+  switch (mode) {
+    case kModeA:
+      if (a.track_arg) return short_cv(a) ? kernel_a<true, 4>(depth_a) : kernel_a<true, kTileFloats / kNA>(depth_a);
+      return short_cv(a) ? kernel_a<false, 4>(depth_a) : kernel_a<false, kTileFloats / kNA>(depth_a);
+    case kModeC:
+      if (a.P <= 4 * (kNO / 64)) return kernel_c<4>(depth_c);
+      if (a.P <= 8 * (kNO / 64)) return kernel_c<8>(depth_c);
+      return kernel_c<0>(depth_c);
+    case kModeMU:
+      return k_vertex<kModeMU, 0, false, kNO>;
+    default:
+      return k_vertex<kModeOut, 0, false, kNO>;
+  }
+}
+
+void launch_vertex(int mode, const VertexArgs& a, int nblocks, hipStream_t st) {
+  int nt = 0;
+  size_t lds = 0;
+  const VertexKernel k = vertex_kernel(mode, a, &nt, &lds);
+  VertexArgs arg = a;
+  void* args[] = {&arg};
+  (void)hipLaunchKernel(reinterpret_cast<const void*>(k), dim3(nblocks), dim3(nt), args, lds, st);
 }
 
 // Blocks of a vertex pass resident at once on the device (CUs x blocks per CU at the pass's registers and LDS).
@@ -1529,15 +1698,11 @@ int vertex_resident_blocks(int mode, const VertexArgs& a) {
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
-  const size_t lds = vertex_lds_bytes(mode, a.VT, a.P, a.Q, a.Qs);
-  hipError_t e;
-  if (mode == kModeA)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vertex<kModeA, 0, false, 256>, 256,
-                                                     std::max(lds, sizeof(QPart) * 256));
-  else if (mode == kModeC)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vertex<kModeC, 4, false, 512>, 512, lds);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vertex<kModeOut, 0, false, 512>, 512, lds);
+  int nt = 0;
+  size_t lds = 0;
+  const VertexKernel k = vertex_kernel(mode, a, &nt, &lds);
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k), nt, lds);
   return e == hipSuccess ? cus * std::max(per, 1) : 0;
 }
 

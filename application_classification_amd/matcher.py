@@ -14,8 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (AC_DATA, AC_OPT_GRAPH, AC_OPT_PRUNE, AC_OPT_SLAB_EDGES, AC_OPT_TAPE, AC_OPT_WALK_BATCH,  # noqa: F401
-                   AC_PATTERN,
+from ._lib import (AC_DATA, AC_OPT_GRAPH, AC_OPT_PACK_COSTS, AC_OPT_PRUNE, AC_OPT_SLAB_EDGES,  # noqa: F401
+                   AC_OPT_TAPE, AC_OPT_WALK_BATCH, AC_PATTERN,
                    ACError, HostGraph)
 
 
@@ -187,6 +187,12 @@ class Matcher:
     def sizes(self):
         out = (ctypes.c_int64 * 4)()
         self._check(self._L.ac_get_sizes(self._h, out))
+        return tuple(out)
+
+    def cost_packing(self):
+        """(edge bundles, bundles whose costs the last run's walk read as 24-bit codes; AC_OPT_PACK_COSTS)."""
+        out = (ctypes.c_int64 * 2)()
+        self._check(self._L.ac_get_cost_packing(self._h, out))
         return tuple(out)
 
     @property

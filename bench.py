@@ -326,7 +326,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-pruned", action="store_true",
-                    help="skip the pruned run (profiled commands then hold the headline launches only)")
+                    help="skip the comparison runs, pruned and with FP32 costs (profiled commands then hold the "
+                         "headline launches only)")
     ap.add_argument("--summarize-profile", metavar="DIR",
                     help="summarise rocprofv3 outputs under DIR into profiles/ and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -372,7 +373,7 @@ def main():
     dn, de, pn, pe = d.num_nodes, d.num_edges, p.num_nodes, p.num_edges
     log(f"workload {w.name}: dn={dn} de={de} pn={pn} pe={pe} generated in {time.time() - t:.1f} s")
 
-    from application_classification_amd.matcher import AC_OPT_PRUNE
+    from application_classification_amd.matcher import AC_OPT_PACK_COSTS, AC_OPT_PRUNE
 
     m = Matcher(0)
     m.set_option(AC_OPT_PRUNE, 0)  # the headline walks every edge; the exact pruning is measured separately below
@@ -408,6 +409,9 @@ def main():
     loop_ms /= args.steps
     prep_ms /= args.steps
     log(f"timed {args.steps} steps: {wall_ms:.3f} ms/step (device: prep {prep_ms:.3f} ms, loop {loop_ms:.3f} ms)")
+    nb, packed = m.cost_packing()
+    packing = {"bundles": nb, "packed_bundles": packed, "frac": packed / max(nb, 1)}
+    log(f"cost packing (AC_OPT_PACK_COSTS): {packed} of {nb} edge bundles walked as 24-bit codes")
 
     mu = m.mu()
     checks = {
@@ -454,6 +458,28 @@ def main():
         log(f"pruned: loop {pr_loop:.3f} ms ({pruned['speedup_loop']:.2f}x), identical: {pruned['bit_identical']}")
         m.set_option(AC_OPT_PRUNE, 0)
 
+    # The same headline run with every edge cost read as FP32 (AC_OPT_PACK_COSTS = 0): the 24-bit codes are
+    # lossless, so the output must be bit-identical.
+    fp32_costs = None
+    if not args.no_pruned:
+        m.set_option(AC_OPT_PACK_COSTS, 0)
+        m.run_async()
+        m.sync()
+        u_loop = u_prep = 0.0
+        for i in range(args.steps):
+            m.run_async()
+            m.sync()
+            a, b = m.timing()
+            u_prep += a
+            u_loop += b
+        u_loop /= args.steps
+        u_prep /= args.steps
+        fp32_costs = {"value": updates / (u_loop * 1e-3), "loop_ms_per_step": u_loop, "prep_ms_per_step": u_prep,
+                      "hbm_roofline_frac_iteration": B["iteration"] * pn / (u_loop * 1e-3) / HBM_PEAK,
+                      "bit_identical": bool(np.array_equal(m.mu(), mu))}
+        log(f"FP32 costs: loop {u_loop:.3f} ms, identical: {fp32_costs['bit_identical']}")
+        m.set_option(AC_OPT_PACK_COSTS, 1)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -484,6 +510,8 @@ def main():
         "hbm_roofline_frac_iteration": iter_frac,
         "alg_bytes_per_iteration": B["iteration"],
         "roofline": roofline,
+        "cost_packing": packing,
+        "fp32_costs": fp32_costs,
         "pruned": pruned,
         "cpu_baseline": cpu,
         "stage_ms": stages,

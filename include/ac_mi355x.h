@@ -123,6 +123,9 @@ int ac_get_match(ac_ctx* ctx, float log_tau, uint8_t* host_out);
 int ac_get_sizes(const ac_ctx* ctx, int64_t* out4);
 /* Device time of the last run, from HIP events: prep (src/main.cu:107-148) and loop (src/main.cu:160-245). */
 int ac_get_timing(const ac_ctx* ctx, double* prep_ms, double* loop_ms);
+/* No counterpart in the reference: {edge bundles, bundles whose costs the last run's walk read as 24-bit codes}
+ * (AC_OPT_PACK_COSTS); synchronises the context's stream. */
+int ac_get_cost_packing(ac_ctx* ctx, int64_t* out2);
 /* Device stream the context runs on (a hipStream_t). */
 void* ac_stream(ac_ctx* ctx);
 
@@ -154,6 +157,9 @@ int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms);
                               * result is bit-identical; 0: every edge is walked                                  */
 #define AC_OPT_CDIST_MFMA 6  /* 1 (default): feature distances on FP64 matrix cores in GEMM form, pairs that lose
                               * digits to cancellation recomputed directly; 0: direct form on the vector units     */
+#define AC_OPT_PACK_COSTS 7  /* 1 (default): the iteration's edge walk reads the edge costs of a bundle as 24-bit codes
+                              * (bit pattern minus a per-bundle exponent base) when all of them lie within two
+                              * binades, losslessly, so results are bit-identical; 0: FP32 costs throughout        */
 int ac_set_option(ac_ctx* ctx, int option, int value);
 int ac_get_option(const ac_ctx* ctx, int option, int* value);
 

@@ -136,7 +136,7 @@ def test_data_without_edges(small):
     assert_parity(mu, ref, "data without edges")
 
 
-def _class_mix_graph(seed):
+def _class_mix_graph(seed, dim=13):
     """2,000 vertices in four interleaved degree classes (isolated, in-edges only, out-edges only, both) plus one hub
     with an in-edge from every source (over 1,024: pieces in several fixup chunks) and one with 400 out-edges."""
     r = np.random.default_rng(seed)
@@ -152,7 +152,7 @@ def _class_mix_graph(seed):
     keep = s != d
     key = np.unique(s[keep].astype(np.int64) * n + d[keep])
     s, d = (key // n).astype(np.int32), (key % n).astype(np.int32)
-    data = W.Graph(r.normal(size=(n, 13)).astype(np.float32), r.normal(size=(len(s), 13)).astype(np.float32), s, d)
+    data = W.Graph(r.normal(size=(n, dim)).astype(np.float32), r.normal(size=(len(s), dim)).astype(np.float32), s, d)
     return data
 
 
@@ -314,6 +314,81 @@ def test_pruning_is_exact(small, case):
                 m.set_pattern(p)
                 out.append(m.run())
         np.testing.assert_array_equal(out[0], out[1])
+
+
+def _wide_cost_graph(seed):
+    """ER-1,000 data graph whose edge features carry per-edge scales spread over many binades, so that almost no
+    bundle's costs fit one two-binade window."""
+    base = W.small(seed).data
+    r = np.random.default_rng(seed + 100)
+    scale = np.exp2(r.uniform(-12, 12, size=(base.num_edges, 1))).astype(np.float32)
+    return W.Graph(base.vfeat, (base.efeat * scale).astype(np.float32), base.src, base.dst)
+
+
+@pytest.fixture(scope="module")
+def rmat12_d64():
+    """R-MAT scale 12 with 64-dim features: distances near sqrt(128), so most bundles pack; the planted edges' near-zero
+    costs keep their bundles in FP32."""
+    return W.rmat(12, 64, 8, 12, seed=1)
+
+
+@pytest.mark.parametrize("case", ["rmat12", "rmat12_q10", "rmat12_wide_pattern", "s18", "class_mix", "small",
+                                  "wide_costs"])
+def test_cost_packing_is_exact(small, rmat12_d64, case):
+    """The walk over 24-bit cost codes gives bit-identical results to the walk over FP32 costs, with pruning off and
+    on.  Covers runs where bundles that pack and bundles that keep FP32 rows mix, pattern edge counts that are not a
+    multiple of 4 (padding columns), few runs per bundle (a wide pattern), split hubs, 13-dim features whose distances
+    span more than two binades, and per-edge feature scales spread so widely that nothing packs."""
+    from application_classification_amd.matcher import AC_OPT_PACK_COSTS, AC_OPT_PRUNE
+
+    expect = "some"
+    if case == "rmat12":
+        d, p = rmat12_d64.data, rmat12_d64.pattern
+        expect = "most"
+    elif case == "rmat12_q10":
+        d = rmat12_d64.data
+        p = W.planted_pattern(d, 8, 10, seed=21)
+        expect = "most"
+    elif case == "rmat12_wide_pattern":
+        d = rmat12_d64.data
+        p = W.planted_pattern(d, 40, 70, seed=22)
+    elif case == "s18":
+        w = W.s18(0)
+        d, p = w.data, w.pattern
+        expect = "most"
+    elif case == "class_mix":
+        d = _class_mix_graph(7, dim=64)
+        p = W.planted_pattern(d, 8, 12, seed=3)
+    elif case == "small":
+        d, p = small.data, small.pattern
+        expect = None
+    else:
+        d = _wide_cost_graph(1)
+        p = W.planted_pattern(d, 8, 12, seed=4)
+        expect = "none"
+    for prune in (0, 1):
+        out, info = [], None
+        for pack in (0, 1):
+            with Matcher(0) as m:
+                m.set_option(AC_OPT_PRUNE, prune)
+                m.set_option(AC_OPT_PACK_COSTS, pack)
+                assert m.get_option(AC_OPT_PACK_COSTS) == pack
+                m.set_data(d)
+                m.set_pattern(p)
+                out.append(m.run())
+                if pack:
+                    info = m.cost_packing()
+                else:
+                    assert m.cost_packing()[1] == 0
+        nb, packed = info
+        say(f"{case}, prune={prune}: {packed} of {nb} bundles packed")
+        np.testing.assert_array_equal(out[0], out[1])
+        if expect == "most":
+            assert packed >= 0.75 * nb
+        elif expect == "some":
+            assert 0 < packed < nb
+        elif expect == "none":
+            assert packed <= 0.01 * nb
 
 
 @pytest.mark.parametrize("sigma", [0.0, 0.01])
