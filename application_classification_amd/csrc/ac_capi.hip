@@ -196,7 +196,8 @@ struct ac_ctx {
   int walk_B = 0;          // steps per batch of the edge walk (0: default)
   bool use_graph = false;  // replay iteration pairs from a captured graph
   bool prune = true;       // skip bundles whose edges cannot raise FMax above the fill (exact)
-  bool pack_costs = true;  // the plain walk reads 24-bit codes of the costs where a bundle allows them (exact)
+  bool pack_costs = true;  // walks read 24-bit codes of the costs where a bundle allows them (exact)
+  bool costs_packed = false;  // the last prep wrote CEp / bpack for the current CE
   DevBuf<unsigned> CEp;    // [rows * Qs * 3 / 4] packed costs of the bundles whose base is nonzero
   DevBuf<unsigned> bpack;  // [bundles] base of a bundle's codes, 0 = FP32 rows
   bool use_mfma_cdist = true;  // feature distances on FP64 matrix cores (GEMM form, cancellation-guarded)
@@ -748,10 +749,12 @@ static int enqueue_prep(ac_ctx* ctx) {
     ac::launch_prep_scalars(P, Q, ctx->d_ps.p, ctx->d_pd.p, ctx->lse_cv.p, ctx->lse_ce.p, qs_prep, st);
     HIPCHK(ctx, ctx->bmin.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles) * Qs));
     ac::launch_bundle_min(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Qs, ctx->bmin.p, st);
+    ctx->costs_packed = false;
     if (ctx->pack_costs) {
       HIPCHK(ctx, ctx->CEp.alloc((size_t)std::max<int64_t>(1, ctx->bs->brows * (Qs / 4) * 3)));
       HIPCHK(ctx, ctx->bpack.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles)));
       ac::launch_pack_costs(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Q, Qs, ctx->bpack.p, ctx->CEp.p, st);
+      ctx->costs_packed = true;
     }
     const bool tape = ctx->tape_K > 0;
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
@@ -828,7 +831,7 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p,
                         walk_batch_of(ctx), st, nullptr, nullptr, ctx->prune ? ctx->bmin.p : nullptr,
-                        ctx->pack_costs ? ctx->CEp.p : nullptr, ctx->pack_costs ? ctx->bpack.p : nullptr);
+                        ctx->costs_packed ? ctx->CEp.p : nullptr, ctx->costs_packed ? ctx->bpack.p : nullptr);
     MARK(AC_STAGE_EDGE_WALK);
     launch_fixup32(ctx, st);
     MARK(AC_STAGE_WALK_FIXUP);
@@ -1011,7 +1014,7 @@ extern "C" int ac_get_cost_packing(ac_ctx* ctx, int64_t* out2) {
     return ctx->fail(AC_ERR_STATE, "ac_get_cost_packing: no run with pattern edges yet");
   const int64_t nb = ctx->bs->nbundles;
   int64_t packed = 0;
-  if (ctx->pack_costs && nb > 0) {
+  if (ctx->costs_packed && nb > 0) {
     std::vector<unsigned> h((size_t)nb);
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     HIPCHK(ctx, hipMemcpy(h.data(), ctx->bpack.p, sizeof(unsigned) * (size_t)nb, hipMemcpyDeviceToHost));
@@ -1427,7 +1430,8 @@ static int recompute_iteration(ac_ctx* ctx, int it, const StateBufs& In, const S
     ac::launch_reduce_q(ctx->nb_vertex + 1, Q, ctx->qpart.p, Out.qs, st);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, Out.qs, Out.F, ctx->carry_in.p, walk_batch_of(ctx),
-                        st, FA, ctx->carg_in.p, ctx->prune ? ctx->bmin.p : nullptr);
+                        st, FA, ctx->carg_in.p, ctx->prune ? ctx->bmin.p : nullptr,
+                        ctx->costs_packed ? ctx->CEp.p : nullptr, ctx->costs_packed ? ctx->bpack.p : nullptr);
     ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, Out.qs, Out.F, st,
                               ctx->carg_in.p, FA);
   } else {

@@ -128,8 +128,7 @@ void launch_walk_src_fixup(int nsplit, const Split* splits, const double* carry,
 void launch_build_bundles(int64_t nbundles, const Bundle* bundles, const int* brun, const Run* runs, const int2* sd,
                           const int* eid, int G, int* bsrc, int* bdst, int* beid, int* rowmap, hipStream_t st);
 void launch_remap_ks(int64_t m, const int2* ks, const int* rowmap, int2* out, hipStream_t st);
-// CEp / bpack (from launch_pack_costs): the plain walk (VR given, no FA) reads the packed rows of the bundles
-// whose base is nonzero.
+// CEp / bpack (from launch_pack_costs): walks with VR read the packed rows of the bundles whose base is nonzero.
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
                      double* carry, int B, hipStream_t st, int* FA = nullptr, int* carg = nullptr,

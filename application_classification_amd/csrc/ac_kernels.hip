@@ -813,9 +813,10 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   const int* dstp = bdst + Bd.base + g;
   const f4v* VRv = reinterpret_cast<const f4v*>(VR);
   const int last = Bd.len / B - 1;
-  // Packed costs (plain walk only, CEp != nullptr): the bundle's base, 0 when it keeps its FP32 rows.
-  const unsigned pbase = kFlags && CEp ? bpack[b] : 0u;
-  const unsigned* cpb = kFlags && CEp ? CEp + Bd.base * L4 * 3 : nullptr;
+  // Packed costs (walks with VR, CEp != nullptr): the bundle's base, 0 when it keeps its FP32 rows.  Decoding is
+  // exact, so maxima and recorded argmax rows are those of the FP32 walk.
+  const unsigned pbase = kWithVR && CEp ? bpack[b] : 0u;
+  const unsigned* cpb = kWithVR && CEp ? CEp + Bd.base * L4 * 3 : nullptr;
 #define AC_WK_SRC(S, T)                                                          \
   if (kWithVR) {                                                                 \
     const int t_ = (T) < last ? (T) : last;                                      \
@@ -870,7 +871,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
     }                                                                               \
   }
   // The pipelined walk over the bundle, for FP32 cost rows (kP false) or packed ones (decoded when reduced, so the
-  // loads stay in flight as raw dwords).  kP is written in terms of kFlags so that it is value-dependent: the
+  // loads stay in flight as raw dwords).  kP is written in terms of kWithVR so that it is value-dependent: the
   // discarded `if constexpr` branches are then not instantiated.
 #define AC_WK_PIPELINE(KP)                                                   \
   {                                                                          \
@@ -895,10 +896,10 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
       if (t + 2 > last) break;                                               \
     }                                                                        \
   }
-  if (kFlags && pbase)
-    AC_WK_PIPELINE(kFlags)
+  if (kWithVR && pbase)
+    AC_WK_PIPELINE(kWithVR)
   else
-    AC_WK_PIPELINE(false && kFlags)
+    AC_WK_PIPELINE(false && kWithVR)
 #undef AC_WK_PIPELINE
 #undef AC_WK_SRC
 #undef AC_WK_BATCH
@@ -951,7 +952,7 @@ void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, 
   if (!bpack) CEp = nullptr;
   if (VR && FA)
     launch_walk_max_t<true, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                  carry, FA, carg, bmin, nullptr, nullptr);
+                                  carry, FA, carg, bmin, CEp, bpack);
   else if (VR)
     launch_walk_max_t<true, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
                                    carry, FA, carg, bmin, CEp, bpack);

@@ -145,6 +145,34 @@ def test_taped_forward_equals_plain_forward(small):
     np.testing.assert_array_equal(mu, Matcher_plain(d, p))
 
 
+def test_packed_costs_give_identical_gradients():
+    """The taped forward and the recompute walks of ac_backward read the 24-bit cost codes (AC_OPT_PACK_COSTS) where
+    bundles pack; the forward and both gradients equal those with FP32 costs exactly.  64-dim features, so most
+    bundles pack."""
+    from application_classification_amd.matcher import AC_OPT_PACK_COSTS
+
+    w = W.rmat(11, 64, 10, 14, seed=6)
+    d, p = w.data, w.pattern
+    wv, we = weights(7, 64, 64)
+    res = []
+    for pack in (0, 1):
+        with Matcher(0) as m:
+            m.set_option(AC_OPT_PACK_COSTS, pack)
+            m.set_option(AC_OPT_TAPE, 3)
+            m.set_data(d)
+            m.set_weights(wv, we)
+            m.set_pattern(p)
+            mu = m.run()
+            nb, packed = m.cost_packing()
+            gv, ge = m.backward(loss_grad_mu(d.num_nodes, p.vmap))
+            res.append((mu, gv, ge))
+            say(f"pack={pack}: {packed} of {nb} bundles packed")
+            if pack:
+                assert packed > nb // 2
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_backward_requires_a_taped_run(small):
     with Matcher(0) as m:
         m.set_data(small.data)
