@@ -66,7 +66,8 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     main_src = os.path.join(CSRC, "ac_main.cpp")
     if os.path.exists(main_src):
         tmpx = EXE + ".tmp"
-        _run([HIPCC, *COMMON, main_src, "-o", tmpx, f"-L{HERE}", "-lac_mi355x", "-Wl,-rpath,$ORIGIN"], verbose)
+        _run([HIPCC, *COMMON, main_src, "-o", tmpx, f"-L{HERE}", "-lac_mi355x", "-Wl,-rpath,$ORIGIN", "-pthread"],
+             verbose)
         os.replace(tmpx, EXE)
     return LIB
 
