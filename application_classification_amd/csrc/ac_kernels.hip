@@ -1286,6 +1286,7 @@ static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
   const int nst = mode == kModeA ? 4 : 2;                                    // staged [VT][Qs] arrays
   b += (size_t)VT * (nst * (Qs + 1) + (P + 1)) * sizeof(float);              // staged rows
   b += (size_t)(2 * Q + P + 1) * sizeof(int);                                // incidence lists
+  if (mode == kModeOut) b += (size_t)VT * sizeof(int);                       // the tile's caller ids
   return (b + 15) / 16 * 16;
 }
 
@@ -1390,6 +1391,7 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   float* Cs = Ls + (kA ? VT * ldq : 0);                // [VT][ldp] CV
   int* incs = reinterpret_cast<int*>(Cs + VT * ldp);   // [2Q]
   int* incoff = incs + 2 * Q;                          // [P + 1]
+  int* Prow = incoff + (P + 1);                        // [VT] caller id of each tile row (mode OUT)
   const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const bool sums = !a.first;
   cint_t* const c_inc = (cint_t*)a.inc;
@@ -1445,9 +1447,10 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   typedef f4v PfSet[4][kPF4];  // FMaxN, VFm (and in pass A minCEo, lsO)
   PfSet pfA, pfB;
   float pcA[kPFC], pcB[kPFC];
+  int ppA = 0, ppB = 0;  // mode OUT: the caller id of tile row t
   // Loads of a tile into one register set; indexes past the tile are clamped to its last element (loaded, never
   // used).
-  auto prefetch = [&](int64_t tl, PfSet& pf, float (&pc)[kPFC]) __attribute__((always_inline)) {
+  auto prefetch = [&](int64_t tl, PfSet& pf, float (&pc)[kPFC], int& pp) __attribute__((always_inline)) {
     const int64_t v0_ = tl * VT;
     const int nv_ = (int)((a.n - v0_) < VT ? (a.n - v0_) : VT);
     const int last4_ = nv_ * Qs / 4 - 1, lastc_ = nv_ * P - 1;
@@ -1465,10 +1468,11 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < kPFC; ++k) pc[k] = a.CV[bc_ + min(t + k * NT, lastc_)];
+    if (kMode == kModeOut && a.perm) pp = a.perm[v0_ + min(t, nv_ - 1)];
   };
   const int64_t G = gridDim.x;
   // One tile: stage its register set in LDS, refill the set with the tile kDepth tiles ahead, compute.
-  auto run_tile = [&](int64_t tile, PfSet& pf, float (&pc)[kPFC]) __attribute__((always_inline)) {
+  auto run_tile = [&](int64_t tile, PfSet& pf, float (&pc)[kPFC], int& pp) __attribute__((always_inline)) {
     const int64_t v0 = tile * VT;
     const int nv = (int)((a.n - v0) < VT ? (a.n - v0) : VT);
     __syncthreads();  // tables written (first tile) / previous tile consumed
@@ -1499,9 +1503,10 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
           Cs[r * ldp + c] = pc[k];
         }
       }
+      if (kMode == kModeOut && a.perm && t < nv) Prow[t] = pp;
     }
     __syncthreads();
-    if (tile + kDepth * G < ntiles) prefetch(tile + kDepth * G, pf, pc);
+    if (tile + kDepth * G < ntiles) prefetch(tile + kDepth * G, pf, pc, pp);
     // p-phase: wave-uniform pattern vertex, lane = tile row.
     // MU_pre of row `r` and pattern vertex `p` (-inf for rows past the tile).
 #define AC_VX_MU_PRE(X, P_, R_)                                                                              \
@@ -1561,7 +1566,7 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
         if (kMode == kModeMU)
           a.mu_out[(v0 + r) * P + c] = MUs[r * ldm + c];
         else
-          a.out[(int64_t)(a.perm ? a.perm[v0 + r] : v0 + r) * P + c] = (float)MUs[r * ldm + c];
+          a.out[(int64_t)(a.perm ? Prow[r] : v0 + r) * P + c] = (float)MUs[r * ldm + c];
       }
     }
     if (kA) {
@@ -1604,11 +1609,11 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
     }
   };
   int64_t tile = blockIdx.x;
-  if (tile < ntiles) prefetch(tile, pfA, pcA);
-  if (kDepth > 1 && tile + G < ntiles) prefetch(tile + G, pfB, pcB);
+  if (tile < ntiles) prefetch(tile, pfA, pcA, ppA);
+  if (kDepth > 1 && tile + G < ntiles) prefetch(tile + G, pfB, pcB, ppB);
   for (; tile < ntiles; tile += kDepth * G) {
-    run_tile(tile, pfA, pcA);
-    if (kDepth > 1 && tile + G < ntiles) run_tile(tile + G, pfB, pcB);
+    run_tile(tile, pfA, pcA, ppA);
+    if (kDepth > 1 && tile + G < ntiles) run_tile(tile + G, pfB, pcB, ppB);
   }
   __syncthreads();
 
