@@ -1279,13 +1279,21 @@ constexpr int kVTMax = 64;
 // Staged tile limits: VT * Qs <= 2048 and VT * P <= 2048 floats, for every block size NT (256 or 512).
 constexpr int kTileFloats = 2048;
 
+// One incidence of a pattern vertex in the MU sums: its term is fmax(fill, guard(x - m - l)) with x the staged value
+// at column col of the adjacent FMaxN | VFm tiles.  An FMax term has m = l = 0 (x - 0 - 0 = x exactly, -inf stays
+// -inf), so both kinds take the same branch-free sequence.
+struct IncRec {
+  double fill, m, l;
+  int col, pad;
+};
+
 static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
   size_t b = (size_t)(4 * Q + 2 * P) * sizeof(double);                       // tables
+  b += (size_t)(2 * Q + 1) * sizeof(IncRec);                                 // incidence records (+ one pad)
   if (mode != kModeC) b += (size_t)(P + 1) * VT * sizeof(double);            // MU
   if (mode == kModeC) b += (size_t)P * sizeof(Lse);                          // column accumulators
   const int nst = mode == kModeA ? 4 : 2;                                    // staged [VT][Qs] arrays
   b += (size_t)VT * (nst * (Qs + 1) + (P + 1)) * sizeof(float);              // staged rows
-  b += (size_t)(2 * Q + P + 1) * sizeof(int);                                // incidence lists
   if (mode == kModeOut) b += (size_t)VT * sizeof(int);                       // the tile's caller ids
   return (b + 15) / 16 * 16;
 }
@@ -1381,7 +1389,8 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   const int ldq = Qs + 1, ldp = P + 1;
   double* tab = lds;                                   // [4][Q]: fillF, fillR, mRE, lRE
   double* pmu = tab + 4 * Q;                           // [2][P]: m, l of LSE_MU
-  double* MUs = pmu + 2 * P;                           // [VT][P + 1] (modes A, OUT, MU)
+  IncRec* recs = reinterpret_cast<IncRec*>(pmu + 2 * P);  // [2Q + 1] incidence records, inc order
+  double* MUs = reinterpret_cast<double*>(recs + 2 * Q + 1);  // [VT][P + 1] (modes A, OUT, MU)
   const int ldm = P + 1;
   Lse* lacc = reinterpret_cast<Lse*>(MUs + (kA || kRows ? ldm * VT : 0));    // [P] (mode C)
   float* Fs = reinterpret_cast<float*>(lacc + (kMode == kModeC ? P : 0));    // [VT][ldq] FMaxN
@@ -1389,17 +1398,20 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   float* Ms = Vs + VT * ldq;                           // [VT][ldq] minCEo (mode A)
   float* Ls = Ms + (kA ? VT * ldq : 0);                // [VT][ldq] lsO (mode A)
   float* Cs = Ls + (kA ? VT * ldq : 0);                // [VT][ldp] CV
-  int* incs = reinterpret_cast<int*>(Cs + VT * ldp);   // [2Q]
-  int* incoff = incs + 2 * Q;                          // [P + 1]
-  int* Prow = incoff + (P + 1);                        // [VT] caller id of each tile row (mode OUT)
+  int* Prow = reinterpret_cast<int*>(Cs + VT * ldp);   // [VT] caller id of each tile row (mode OUT)
   const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const bool sums = !a.first;
-  cint_t* const c_inc = (cint_t*)a.inc;
   cint_t* const c_incoff = (cint_t*)a.inc_off;
-  cqstat_t* const c_qs = (cqstat_t*)a.qs;
 
-  for (int i = t; i < 2 * Q; i += NT) incs[i] = a.inc[i];
-  for (int i = t; i <= P; i += NT) incoff[i] = a.inc_off[i];
+  for (int k = t; k <= 2 * Q; k += NT) {
+    IncRec r{-kInf, 0.0, 0.0, 0, 0};  // k = 2Q: read ahead of the last incidence, never used
+    if (k < 2 * Q) {
+      const int code = a.inc[k], qq = code >> 1;
+      const QStat x = a.qs[qq];
+      r = (code & 1) ? IncRec{x.fillR, x.mRE, x.lRE, VT * ldq + qq, 0} : IncRec{x.fillF, 0.0, 0.0, qq, 0};
+    }
+    recs[k] = r;
+  }
   for (int i = t; i < Q; i += NT) {
     const QStat x = a.qs[i];
     tab[i] = x.fillF;
@@ -1516,18 +1528,14 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
         double acc_ = -(double)Cs[(R_) * ldp + (P_)];                                                      \
         if (sums) {                                                                                        \
           const int k1_ = c_incoff[(P_) + 1];                                                              \
-          for (int k_ = c_incoff[P_]; k_ < k1_; ++k_) {                                                    \
-            const int code_ = c_inc[k_];                                                                   \
-            const int qq_ = code_ >> 1;                                                                    \
-            if (code_ & 1) {                                                                               \
-              /* scalar reads first, so the -inf guard below is a select, not a branch */                  \
-              const double fR_ = c_qs[qq_].fillR, mR_ = c_qs[qq_].mRE, lR_ = c_qs[qq_].lRE;                \
-              const float vfm_ = Vs[(R_) * ldq + qq_];                                                     \
-              const double d_ = ((double)vfm_ - mR_) - lR_;                                                \
-              acc_ += fmax(fR_, vfm_ == -INFINITY ? -kInf : d_);                                           \
-            } else {                                                                                       \
-              acc_ += fmax(c_qs[qq_].fillF, (double)Fs[(R_) * ldq + qq_]);                                 \
-            }                                                                                              \
+          int k_ = c_incoff[P_];                                                                           \
+          IncRec rc_ = recs[k_];                                                                           \
+          for (; k_ < k1_; ++k_) {                                                                         \
+            const IncRec nx_ = recs[k_ + 1]; /* in flight while this term is formed */                     \
+            const float x_ = Fs[(R_) * ldq + rc_.col];                                                     \
+            const double d_ = ((double)x_ - rc_.m) - rc_.l;                                                \
+            acc_ += fmax(rc_.fill, x_ == -INFINITY ? -kInf : d_);                                          \
+            rc_ = nx_;                                                                                     \
           }                                                                                                \
         }                                                                                                  \
         X = acc_;                                                                                          \
