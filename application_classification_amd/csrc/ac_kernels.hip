@@ -1334,7 +1334,6 @@ __device__ __forceinline__ Lse wave_lse_merge(Lse s) {
 // Read-only inputs seen through the constant address space: loads at wave-uniform indexes become scalar loads, and
 // branches on their values scalar branches.
 typedef const __attribute__((address_space(4))) int cint_t;
-typedef const __attribute__((address_space(4))) QStat cqstat_t;
 
 // The isolated vertices' share, in closed form (block 0 of a vertex pass writes it as one extra partial).  Their
 // FMaxN = VFm = -inf rows make MU_pre = -CV + K_p, K_p the sum of the fills of p's incidences (in the vertex pass's
@@ -1377,10 +1376,8 @@ __device__ Lse iso_lse(const VertexArgs& a, int p) {
 // in registers over all tiles, merged across the wave once at the end.  kPW == 0: one wave-level reduction per tile.
 // kArgMax (mode A): also track the vertices of the column maxima (the adjoint's tape needs them).
 // kPFC: CV values each thread prefetches per tile (VT * P <= NT * kPFC); fewer registers allow more resident waves.
-// kDepth: tiles whose loads are in flight per block (1 or 2 register sets, used by alternate tiles).
-template <int kMode, int kPW, bool kArgMax, int NT, int kPFC = kTileFloats / NT, int kDepth = 1>
+template <int kMode, int kPW, bool kArgMax, int NT, int kPFC = kTileFloats / NT>
 __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
-  static_assert(kDepth == 1 || kDepth == 2, "one or two prefetch register sets");
   constexpr int kVW = NT / 64, kPF4 = kTileFloats / 4 / NT;
   constexpr bool kA = kMode == kModeA;
   constexpr bool kRows = kMode == kModeOut || kMode == kModeMU;  // MU tile staged, written back row by row
@@ -1457,9 +1454,9 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
 #pragma unroll
   for (int pi = 0; pi < (kPW > 0 ? kPW : 1); ++pi) pacc[pi] = lse_empty();
   typedef f4v PfSet[4][kPF4];  // FMaxN, VFm (and in pass A minCEo, lsO)
-  PfSet pfA, pfB;
-  float pcA[kPFC], pcB[kPFC];
-  int ppA = 0, ppB = 0;  // mode OUT: the caller id of tile row t
+  PfSet pfA;
+  float pcA[kPFC];
+  int ppA = 0;  // mode OUT: the caller id of tile row t
   // Loads of a tile into one register set; indexes past the tile are clamped to its last element (loaded, never
   // used).
   auto prefetch = [&](int64_t tl, PfSet& pf, float (&pc)[kPFC], int& pp) __attribute__((always_inline)) {
@@ -1483,7 +1480,7 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
     if (kMode == kModeOut && a.perm) pp = a.perm[v0_ + min(t, nv_ - 1)];
   };
   const int64_t G = gridDim.x;
-  // One tile: stage its register set in LDS, refill the set with the tile kDepth tiles ahead, compute.
+  // One tile: stage its register set in LDS, refill the set with the block's next tile, compute.
   auto run_tile = [&](int64_t tile, PfSet& pf, float (&pc)[kPFC], int& pp) __attribute__((always_inline)) {
     const int64_t v0 = tile * VT;
     const int nv = (int)((a.n - v0) < VT ? (a.n - v0) : VT);
@@ -1518,7 +1515,7 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
       if (kMode == kModeOut && a.perm && t < nv) Prow[t] = pp;
     }
     __syncthreads();
-    if (tile + kDepth * G < ntiles) prefetch(tile + kDepth * G, pf, pc, pp);
+    if (tile + G < ntiles) prefetch(tile + G, pf, pc, pp);
     // p-phase: wave-uniform pattern vertex, lane = tile row.
     // MU_pre of row `r` and pattern vertex `p` (-inf for rows past the tile).
 #define AC_VX_MU_PRE(X, P_, R_)                                                                              \
@@ -1618,11 +1615,7 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   };
   int64_t tile = blockIdx.x;
   if (tile < ntiles) prefetch(tile, pfA, pcA, ppA);
-  if (kDepth > 1 && tile + G < ntiles) prefetch(tile + G, pfB, pcB, ppB);
-  for (; tile < ntiles; tile += kDepth * G) {
-    run_tile(tile, pfA, pcA, ppA);
-    if (kDepth > 1 && tile + G < ntiles) run_tile(tile + G, pfB, pcB, ppB);
-  }
+  for (; tile < ntiles; tile += G) run_tile(tile, pfA, pcA, ppA);
   __syncthreads();
 
   // Block partials, merged in a fixed order.
@@ -1657,39 +1650,19 @@ static bool short_cv(const VertexArgs& a) { return a.VT * a.P <= kNA * 4; }
 
 typedef void (*VertexKernel)(VertexArgs);
 
-// Prefetch depth of a pass: 1 or 2 tiles in flight per block (tuning override AC_VERTEX_DEPTH_A / _C).
-static int vertex_depth(const char* env, int dflt) {
-  if (const char* e = std::getenv(env)) {
-    const int d = std::atoi(e);
-    if (d == 1 || d == 2) return d;
-  }
-  return dflt;
-}
-
-template <bool kArg, int kPFC>
-static VertexKernel kernel_a(int depth) {
-  return depth == 2 ? k_vertex<kModeA, 0, kArg, kNA, kPFC, 2> : k_vertex<kModeA, 0, kArg, kNA, kPFC, 1>;
-}
-template <int kPW>
-static VertexKernel kernel_c(int depth) {
-  return depth == 2 ? k_vertex<kModeC, kPW, false, kNO, kTileFloats / kNO, 2>
-                    : k_vertex<kModeC, kPW, false, kNO, kTileFloats / kNO, 1>;
-}
-
 // The kernel a pass launches, its block size and dynamic LDS bytes.
 static VertexKernel vertex_kernel(int mode, const VertexArgs& a, int* nt, size_t* lds) {
-  static const int depth_a = vertex_depth("AC_VERTEX_DEPTH_A", 1), depth_c = vertex_depth("AC_VERTEX_DEPTH_C", 1);
   const size_t tiles = vertex_lds_bytes(mode, a.VT, a.P, a.Q, a.Qs);
   *nt = mode == kModeA ? kNA : kNO;
   *lds = mode == kModeA ? std::max(tiles, sizeof(QPart) * kNA) : tiles;
   switch (mode) {
     case kModeA:
-      if (a.track_arg) return short_cv(a) ? kernel_a<true, 4>(depth_a) : kernel_a<true, kTileFloats / kNA>(depth_a);
-      return short_cv(a) ? kernel_a<false, 4>(depth_a) : kernel_a<false, kTileFloats / kNA>(depth_a);
+      if (a.track_arg) return short_cv(a) ? k_vertex<kModeA, 0, true, kNA, 4> : k_vertex<kModeA, 0, true, kNA>;
+      return short_cv(a) ? k_vertex<kModeA, 0, false, kNA, 4> : k_vertex<kModeA, 0, false, kNA>;
     case kModeC:
-      if (a.P <= 4 * (kNO / 64)) return kernel_c<4>(depth_c);
-      if (a.P <= 8 * (kNO / 64)) return kernel_c<8>(depth_c);
-      return kernel_c<0>(depth_c);
+      if (a.P <= 4 * (kNO / 64)) return k_vertex<kModeC, 4, false, kNO>;
+      if (a.P <= 8 * (kNO / 64)) return k_vertex<kModeC, 8, false, kNO>;
+      return k_vertex<kModeC, 0, false, kNO>;
     case kModeMU:
       return k_vertex<kModeMU, 0, false, kNO>;
     default:
