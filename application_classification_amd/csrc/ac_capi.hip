@@ -158,6 +158,7 @@ struct ac_ctx {
   DevBuf<Split> fix_chunks, fix_multi;
   int nfix_chunks = 0, nfix_multi = 0, nfix_parts = 0;
   std::vector<Run> h_runs_in;
+  std::vector<int> h_run_src0;  // first (least) source of each in-edge run
   // Internal vertex order: vertices with in- and out-edges, in-edges only, out-edges only, then isolated ones, each
   // class in ascending caller order.  perm maps an internal vertex to the caller's; the vertex passes stream the
   // first n_act = dn - n_iso rows only.
@@ -346,7 +347,9 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   }
   src = isrc.data();
   dst = idst.data();
-  // CSC (by destination, stable) and CSR (by source, stable) views.
+  // CSR (by source, stable) and CSC (by destination, then source) views.  The CSC is filled from the edges in CSR
+  // order, so each vertex's in-edges ascend by source: a piece of a long in-edge list then gathers from one band of
+  // sources (the maxima do not depend on the order).
   std::vector<int64_t> in_off(nV + 1, 0), out_off(nV + 1, 0);
   for (int64_t e = 0; e < nE; ++e) {
     in_off[dst[e] + 1]++;
@@ -357,20 +360,22 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
     out_off[v + 1] += out_off[v];
   }
   std::vector<int2> csc_sd(nE), csr_ks(nE);
-  std::vector<int> csc_eid(nE), csc_pos(nE);
+  std::vector<int> csc_eid(nE), csc_pos(nE), by_src(nE);
+  {
+    std::vector<int64_t> fill(out_off.begin(), out_off.end() - 1);
+    for (int64_t e = 0; e < nE; ++e) by_src[fill[src[e]]++] = (int)e;
+  }
   {
     std::vector<int64_t> fill(in_off.begin(), in_off.end() - 1);
-    for (int64_t e = 0; e < nE; ++e) {
+    for (int64_t i = 0; i < nE; ++i) {
+      const int64_t e = by_src[i];
       const int64_t k = fill[dst[e]]++;
       csc_eid[k] = (int)e;
       csc_sd[k] = make_int2(src[e], dst[e]);
       csc_pos[e] = (int)k;
     }
   }
-  {
-    std::vector<int64_t> fill(out_off.begin(), out_off.end() - 1);
-    for (int64_t e = 0; e < nE; ++e) csr_ks[fill[src[e]]++] = make_int2(csc_pos[e], src[e]);
-  }
+  for (int64_t i = 0; i < nE; ++i) csr_ks[i] = make_int2(csc_pos[by_src[i]], src[by_src[i]]);
   std::vector<Run> rin, rout;
   std::vector<Split> sin, sout;
   int slin = 0, slout = 0;
@@ -422,6 +427,8 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   ctx->nsplit_out = (int)sout.size();
   ctx->nslots_in = slin;
   ctx->nslots_out = slout;
+  ctx->h_run_src0.resize(rin.size());
+  for (size_t i = 0; i < rin.size(); ++i) ctx->h_run_src0[i] = csc_sd[rin[i].kb].x;
   ctx->h_runs_in = std::move(rin);
   ctx->bsets.clear();  // bundle layouts follow the graph, not only its sizes
   ctx->bs = nullptr;
@@ -507,8 +514,9 @@ constexpr int kMaxWalkBatch = 4;
 constexpr size_t kBundleCache = 8;
 
 // Selects the bundle layout for the current pattern width, building it if it is not cached: the
-// in-edge runs, longest first (ties in CSC order), are dealt out to bundles of G = 64 / (Qs / 4) runs, then the rows
-// of every bundle are laid out on the device (k_build_bundles) and the CSR walk is re-targeted at them.
+// in-edge runs, longest first (ties by first source, so that the bundles walked at the same time gather from nearby
+// sources), are dealt out to bundles of G = 64 / (Qs / 4) runs, then the rows of every bundle are laid out on the
+// device (k_build_bundles) and the CSR walk is re-targeted at them.
 static int select_bundles(ac_ctx* ctx) {
   const int G = ac::walk_groups(ctx->Qs), Bt = kMaxWalkBatch;
   for (size_t i = 0; i < ctx->bsets.size(); ++i)
@@ -523,8 +531,11 @@ static int select_bundles(ac_ctx* ctx) {
   const int64_t nr = (int64_t)R.size();
   std::vector<int> order(nr);
   for (int64_t i = 0; i < nr; ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return R[a].ke - R[a].kb > R[b].ke - R[b].kb; });
+  const std::vector<int>& S0 = ctx->h_run_src0;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const int la = R[a].ke - R[a].kb, lb = R[b].ke - R[b].kb;
+    return la > lb || (la == lb && S0[a] < S0[b]);
+  });
   const int64_t nb = (nr + G - 1) / G;
   std::vector<Bundle> bun(nb);
   std::vector<int> brun((size_t)std::max<int64_t>(1, nb * G), -1), gslot((size_t)std::max<int64_t>(1, nb * G), -1);
