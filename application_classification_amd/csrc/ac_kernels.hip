@@ -1605,6 +1605,8 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
           if (mo != INFINITY) {
             const double xx = vf - (double)mo;
             a.VR[(v0 + r) * a.VRs + q] = (float)vr;
+            // The row's padding columns too (never read), so that whole 128-byte lines are written.
+            if (q < a.VRs - Q) a.VR[(v0 + r) * a.VRs + Q + q] = 0.f;
             a.VFm_out[i] = (float)xx;
             lse_add_finite(lre, xx + (double)lo);
             lse_add_finite(lfe, (vr - (double)mo) + (double)lo);
