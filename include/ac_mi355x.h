@@ -148,7 +148,7 @@ int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms);
  * Tuning options.  No counterpart in the reference (its launch shapes are fixed); used by the autotuner below and
  * by callers that pin a configuration.
  * ------------------------------------------------------------------------------------------------------------- */
-#define AC_OPT_WALK_BATCH 1  /* edge-walk steps per software-pipelined batch: 1, 2 or 4 (0 = default, 2)         */
+#define AC_OPT_WALK_BATCH 1  /* edge-walk steps per software-pipelined batch: 1, 2 or 4 (0 = default, 4)         */
 #define AC_OPT_GRAPH 2       /* 1: replay iteration pairs from a captured HIP graph (fewer launches); 0: launch each */
 #define AC_OPT_SLAB_EDGES 3  /* batched runs: most pattern edges packed into one slab, 4..256 (default 128)         */
 #define AC_OPT_TAPE 4        /* > 0: runs record what ac_backward needs, checkpointing every this many iterations  */
