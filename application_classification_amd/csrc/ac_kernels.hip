@@ -33,18 +33,8 @@ __device__ __forceinline__ int64_t ce_index(int64_t k, int q, int Qs) { return k
 // Native 4-wide vector (register-resident across loop iterations, unlike the HIP float4 wrapper struct).
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-// Streaming loads: data read once per pass (edge costs, edge ids) is loaded non-temporally so that it does not
-// evict the gathered rows that are reused from L2.
-__device__ __forceinline__ float4 nt_load_float4(const float4* p) {
-  typedef float v4 __attribute__((ext_vector_type(4)));
-  const v4 x = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
-  return make_float4(x.x, x.y, x.z, x.w);
-}
-__device__ __forceinline__ int2 nt_load_int2(const int2* p) {
-  typedef int v2 __attribute__((ext_vector_type(2)));
-  const v2 x = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
-  return make_int2(x.x, x.y);
-}
+// Streaming loads (__builtin_nontemporal_load): data read once per pass (edge costs, edge ids) is loaded
+// non-temporally so that it does not evict the gathered rows that are reused from L2.
 
 // Packed costs: four 24-bit codes in three dwords, c0 | c1 << 24, c1 >> 8 | c2 << 16, c2 >> 16 | c3 << 8.  A code
 // is the cost's bit pattern minus the bundle's base (a biased exponent << 23, so one bundle spans two binades);
