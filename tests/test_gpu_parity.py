@@ -410,3 +410,45 @@ def test_matrix_core_cdist(small, sigma):
     assert_parity(out[0], ref, f"matrix-core cdist, sigma={sigma}")
     assert_parity(out[0], out[1], f"matrix-core vs direct cdist, sigma={sigma}")
     np.testing.assert_array_equal(out[0].argmax(axis=0), p.vmap)
+
+
+def test_past_int32_cost_elements():
+    """R-MAT scale 22 (65 M edges) against a planted 30-vertex / 40-edge pattern: 2.6 G edge costs, so flat cost
+    offsets pass 2^31.  The oracle cannot run at this size; the checks are the ones the problem fixes at any size:
+    the planted vertices are recovered, every column of MU is a normalised log-distribution (log-sum-exp 0, no
+    positive entry), and the FP32-cost walk, the pruned walk and walk batch 1 give bit-identical results."""
+    from scipy.special import logsumexp
+
+    from application_classification_amd.matcher import AC_OPT_PACK_COSTS, AC_OPT_PRUNE, AC_OPT_WALK_BATCH
+
+    w = W.rmat(22, 16, 30, 40, seed=11)
+    d, p = w.data, w.pattern
+    assert d.num_edges * p.num_edges > 2**31
+    say(f"rmat22 d16: {d.num_edges} edges x {p.num_edges} pattern edges = {d.num_edges * p.num_edges} costs")
+    variants = [("default", {}), ("FP32 costs", {AC_OPT_PACK_COSTS: 0}), ("pruned", {AC_OPT_PRUNE: 1}),
+                ("walk batch 1", {AC_OPT_WALK_BATCH: 1})]
+    with Matcher(0) as m:
+        m.set_option(AC_OPT_PRUNE, 0)
+        m.set_data(d)
+        m.set_pattern(p)
+        base = None
+        for name, opts in variants:
+            saved = {k: m.get_option(k) for k in opts}
+            for k, v in opts.items():
+                m.set_option(k, v)
+            mu = m.run()
+            for k, v in saved.items():
+                m.set_option(k, v)
+            if base is None:
+                base = mu
+                nb, packed = m.cost_packing()
+                say(f"{packed} of {nb} bundles packed")
+                assert mu.shape == (d.num_nodes, p.num_nodes) and np.isfinite(mu).all()
+                assert (mu <= 0).all()
+                lse = logsumexp(mu.astype(np.float64), axis=0)
+                say(f"max |column log-sum-exp| = {np.abs(lse).max():.3e}")
+                assert np.abs(lse).max() <= 1e-5
+                np.testing.assert_array_equal(mu.argmax(axis=0), p.vmap)
+            else:
+                say(f"{name}: bit-identical = {np.array_equal(mu, base)}")
+                np.testing.assert_array_equal(mu, base)
