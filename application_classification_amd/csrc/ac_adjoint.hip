@@ -88,7 +88,18 @@ __global__ __launch_bounds__(256) void k_sum_parts(int nparts, int C, const doub
   const int c = blockIdx.x * CB + t % CB, l = t / CB;
   double a = 0.0;
   if (l < L && c < C)
-    for (int b = l; b < nparts; b += L) a += part[(int64_t)b * C + c];
+    for (int b0 = l; b0 < nparts; b0 += 8 * L) {
+      // Eight loads in flight before the adds, which keep their order (out-of-range slots are not added).
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int b = b0 + u * L;
+        v[u] = part[(int64_t)(b < nparts ? b : nparts - 1) * C + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (b0 + u * L < nparts) a += v[u];
+    }
   red[t] = a;
   __syncthreads();
   if (t < CB && c < C) {

@@ -401,13 +401,31 @@ void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks
   hipLaunchKernelGGL(k_col_lse_neg, dim3(nblocks), dim3(256), 0, st, n, P, X, part);
 }
 
+// Partials a thread of a fixed-order reduction loads before merging them: one memory round trip per batch rather
+// than one per partial (and per field).  Slots past the last partial load the last partial (an in-bounds address)
+// and then hold the empty partial, whose merge is an exact identity, so the merge sequence is the same for every
+// batch size.
+constexpr int kRedBatchLse = 8, kRedBatchQ = 4;
+
 // Reduce per-block partials [nparts][ncol] column by column in a fixed order (deterministic).
 __global__ __launch_bounds__(256) void k_reduce_lse(int nparts, int ncol, const Lse* __restrict__ part,
                                                     Lse2* __restrict__ out) {
   __shared__ Lse red[256];
   const int c = blockIdx.x, t = threadIdx.x;
   Lse s = lse_empty();
-  for (int b = t; b < nparts; b += 256) s = lse_merge(s, part[(int64_t)b * ncol + c]);
+  for (int b0 = t; b0 < nparts; b0 += 256 * kRedBatchLse) {
+    Lse v[kRedBatchLse];
+#pragma unroll
+    for (int u = 0; u < kRedBatchLse; ++u) {
+      const int b = b0 + u * 256;
+      v[u] = part[(int64_t)(b < nparts ? b : nparts - 1) * ncol + c];
+    }
+#pragma unroll
+    for (int u = 0; u < kRedBatchLse; ++u)
+      if (b0 + u * 256 >= nparts) v[u].m = -kInf;
+#pragma unroll
+    for (int u = 0; u < kRedBatchLse; ++u) s = lse_merge(s, v[u]);
+  }
   red[t] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -426,8 +444,21 @@ __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart
                                                   QStat* __restrict__ out) {
   __shared__ QPart red[256];
   const int q = blockIdx.x, t = threadIdx.x;
-  QPart s{-kInf, -kInf, lse_empty(), lse_empty(), -1, -1};
-  for (int b = t; b < nparts; b += 256) qpart_merge(s, part[(int64_t)b * Q + q]);
+  const QPart empty{-kInf, -kInf, lse_empty(), lse_empty(), -1, -1};
+  QPart s = empty;
+  for (int b0 = t; b0 < nparts; b0 += 256 * kRedBatchQ) {
+    QPart v[kRedBatchQ];
+#pragma unroll
+    for (int u = 0; u < kRedBatchQ; ++u) {
+      const int b = b0 + u * 256;
+      v[u] = part[(int64_t)(b < nparts ? b : nparts - 1) * Q + q];
+    }
+#pragma unroll
+    for (int u = 0; u < kRedBatchQ; ++u)
+      if (b0 + u * 256 >= nparts) v[u] = empty;
+#pragma unroll
+    for (int u = 0; u < kRedBatchQ; ++u) qpart_merge(s, v[u]);
+  }
   red[t] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
