@@ -759,11 +759,12 @@ static int enqueue_prep(ac_ctx* ctx) {
     }
     ac::launch_prep_scalars(P, Q, ctx->d_ps.p, ctx->d_pd.p, ctx->lse_cv.p, ctx->lse_ce.p, qs_prep, st);
     HIPCHK(ctx, ctx->bmin.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles) * Qs));
-    ac::launch_bundle_min(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Qs, ctx->bmin.p, st);
+    if (ctx->pack_costs) HIPCHK(ctx, ctx->bpack.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles)));
+    ac::launch_bundle_min(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Q, Qs, ctx->bmin.p,
+                          ctx->pack_costs ? ctx->bpack.p : nullptr, st);
     ctx->costs_packed = false;
     if (ctx->pack_costs) {
       HIPCHK(ctx, ctx->CEp.alloc((size_t)std::max<int64_t>(1, ctx->bs->brows * (Qs / 4) * 3)));
-      HIPCHK(ctx, ctx->bpack.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles)));
       ac::launch_pack_costs(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Q, Qs, ctx->bpack.p, ctx->CEp.p, st);
       ctx->costs_packed = true;
     }

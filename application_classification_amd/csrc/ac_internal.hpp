@@ -117,7 +117,9 @@ void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream
 // nparts * P entries each.
 void launch_iso_cols(int64_t n, int P, const float* X, int base, Lse* lpart, float* pv, int* pi, int nparts,
                      IsoCol* out, hipStream_t st);
-void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE, int Qs, float* bmin, hipStream_t st);
+// Per-bundle least costs bmin[nbundles][Qs]; with bpack != nullptr also each bundle's cost-code base (0 = not packed).
+void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE, int Q, int Qs, float* bmin,
+                       unsigned* bpack, hipStream_t st);
 void launch_prep_scalars(int P, int Q, const int* ps, const int* pd, const Lse2* cv, const Lse2* ce, QStat* qs,
                          hipStream_t st);
 void launch_fill(float* p, int64_t count, float value, hipStream_t st);
@@ -128,14 +130,15 @@ void launch_walk_src_fixup(int nsplit, const Split* splits, const double* carry,
 void launch_build_bundles(int64_t nbundles, const Bundle* bundles, const int* brun, const Run* runs, const int2* sd,
                           const int* eid, int G, int* bsrc, int* bdst, int* beid, int* rowmap, hipStream_t st);
 void launch_remap_ks(int64_t m, const int2* ks, const int* rowmap, int2* out, hipStream_t st);
-// CEp / bpack (from launch_pack_costs): walks with VR read the packed rows of the bundles whose base is nonzero.
+// CEp / bpack (launch_bundle_min, launch_pack_costs): walks with VR read the packed rows of the bundles whose base is
+// nonzero.
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
                      double* carry, int B, hipStream_t st, int* FA = nullptr, int* carg = nullptr,
                      const float* bmin = nullptr, const unsigned* CEp = nullptr, const unsigned* bpack = nullptr);
-// Lossless 24-bit coding of the bundle-ordered costs: bpack[nbundles] (a bundle's base, 0 = not packed) and
-// CEp[rows * Qs * 3 / 4] (three dwords per four costs).
-void launch_pack_costs(int64_t nbundles, const Bundle* bundles, const float* CE, int Q, int Qs, unsigned* bpack,
+// Lossless 24-bit coding of the bundle-ordered costs into CEp[rows * Qs * 3 / 4] (three dwords per four costs), for
+// the bundles whose base bpack[b] (written by launch_bundle_min) is nonzero.
+void launch_pack_costs(int64_t nbundles, const Bundle* bundles, const float* CE, int Q, int Qs, const unsigned* bpack,
                        unsigned* CEp, hipStream_t st);
 // Plain-path fixup (FP32 normalised pieces): chunks of at most kFixChunk pieces, then the vertices of several
 // chunks from their partials (part: one [Qs] row per partial).

@@ -985,24 +985,57 @@ void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, 
                                     carry, FA, carg, bmin, nullptr, nullptr);
 }
 
+// The base of a bundle's 24-bit cost codes from its least and largest cost bit patterns over columns q < Q, +inf
+// excluded (lo > hi: no such cost).  The costs must lie strictly inside two binades (2^e, 2^(e+2)) of one biased
+// exponent e >= 1: then each is its bit pattern minus base = e << 23, a code in [1, 2^24), and +inf (padding rows)
+// codes as 0.  Positive floats order as their bit patterns, so the least and largest patterns decide: a least cost
+// that is a power of two widens the window down one binade, and negative, zero, subnormal or NaN costs fall outside
+// every window.  0 = the bundle keeps its FP32 rows.
+__device__ __forceinline__ unsigned pack_base(unsigned lo, unsigned hi) {
+  if (lo > hi || hi >= 0x7F800000u) return 0u;
+  unsigned e = lo >> 23;
+  if ((lo & 0x7FFFFFu) == 0u) e = e > 0u ? e - 1u : 0u;  // code 0 is +inf: the least cost must lie above the base
+  return e >= 1u && hi - (e << 23) < (1u << 24) ? e << 23 : 0u;
+}
+
 // Per bundle and pattern edge, the least cost among the bundle's rows (padding rows are +inf): bmin[b][Qs].  One
-// wave per bundle, as the walk; the groups' minima are merged through LDS.
+// wave per bundle, as the walk; the groups' minima are merged through LDS.  With bpack != nullptr it also writes
+// the bundle's cost-code base (pack_base), from the same pass over the rows.
 __global__ __launch_bounds__(kWalkBlock) void k_bundle_min(int64_t nbundles, const Bundle* __restrict__ bundles,
-                                                           const float* __restrict__ CE, int Qs,
-                                                           float* __restrict__ bmin) {
+                                                           const float* __restrict__ CE, int Q, int Qs,
+                                                           float* __restrict__ bmin, unsigned* __restrict__ bpack) {
   __shared__ f4v red[kWalkBlock / 64][64];
   const int L4 = Qs / 4, G = 64 / L4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane / L4, j = lane - g * L4;
   const int64_t b = (int64_t)blockIdx.x * (kWalkBlock / 64) + wave;
   f4v m = f4v{INFINITY, INFINITY, INFINITY, INFINITY};
+  unsigned lo = 0xFFFFFFFFu, hi = 0u;
   if (g < G && b < nbundles) {
     const Bundle Bd = bundles[b];
     const f4v* ce = reinterpret_cast<const f4v*>(CE) + (Bd.base + g) * L4 + j;
     for (int s = 0; s < Bd.len; ++s) {
       const f4v c = __builtin_nontemporal_load(ce + (int64_t)s * G * L4);
       m = f4v{fminf(m[0], c[0]), fminf(m[1], c[1]), fminf(m[2], c[2]), fminf(m[3], c[3])};
+      if (bpack) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const unsigned x = __float_as_uint(c[u]);
+          if (4 * j + u < Q && x != 0x7F800000u) {
+            lo = min(lo, x);
+            hi = max(hi, x);
+          }
+        }
+      }
     }
+  }
+  if (bpack) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = min(lo, (unsigned)__shfl_xor((int)lo, o, 64));
+      hi = max(hi, (unsigned)__shfl_xor((int)hi, o, 64));
+    }
+    if (lane == 0 && b < nbundles) bpack[b] = pack_base(lo, hi);
   }
   red[wave][lane] = m;
   __syncthreads();
@@ -1016,55 +1049,29 @@ __global__ __launch_bounds__(kWalkBlock) void k_bundle_min(int64_t nbundles, con
   }
 }
 
-void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE, int Qs, float* bmin, hipStream_t st) {
+void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE, int Q, int Qs, float* bmin,
+                       unsigned* bpack, hipStream_t st) {
   if (nbundles == 0) return;
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
-  hipLaunchKernelGGL(k_bundle_min, dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, CE, Qs, bmin);
+  hipLaunchKernelGGL(k_bundle_min, dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, CE, Q, Qs, bmin, bpack);
 }
 
-// Lossless 24-bit coding of a bundle's costs for the plain walk (one wave per bundle, lanes as in the walk).  The
-// costs of columns q < Q other than +inf must lie strictly inside two binades (2^e, 2^(e+2)) of one biased exponent
-// e >= 1: then each is its bit pattern minus base = e << 23, a code in [1, 2^24), and +inf (padding rows) codes as
-// 0.  Positive floats order as their bit patterns, so the bundle's least and largest patterns decide: a least cost
-// that is a power of two widens the window down one binade, and negative, zero, subnormal or NaN costs fall outside
-// every window.  bpack[b] = base, or 0 when the bundle keeps its FP32 rows (the walk then reads CE).
+// Lossless 24-bit coding of a bundle's costs for the plain walk (one wave per bundle, lanes as in the walk), with
+// the base k_bundle_min wrote to bpack[b] (pack_base; 0: the bundle keeps its FP32 rows and the walk reads CE).
+// Each cost of a column q < Q other than +inf codes as its bit pattern minus the base, +inf and columns q >= Q as 0.
 __global__ __launch_bounds__(kWalkBlock) void k_pack_costs(int64_t nbundles, const Bundle* __restrict__ bundles,
                                                            const float* __restrict__ CE, int Q, int Qs,
-                                                           unsigned* __restrict__ bpack, unsigned* __restrict__ CEp) {
+                                                           const unsigned* __restrict__ bpack,
+                                                           unsigned* __restrict__ CEp) {
   const int L4 = Qs / 4, G = 64 / L4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane / L4, j = lane - g * L4;
   const int64_t b = (int64_t)blockIdx.x * (kWalkBlock / 64) + wave;
   if (b >= nbundles) return;  // wave-uniform
+  const unsigned base = bpack[b];
+  if (base == 0u || g >= G) return;
   const Bundle Bd = bundles[b];
-  const bool act = g < G;
   const f4v* ce = reinterpret_cast<const f4v*>(CE) + (Bd.base + g) * L4 + j;
-  unsigned lo = 0xFFFFFFFFu, hi = 0u;
-  if (act)
-    for (int s = 0; s < Bd.len; ++s) {
-      const f4v c = __builtin_nontemporal_load(ce + (int64_t)s * G * L4);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const unsigned x = __float_as_uint(c[u]);
-        if (4 * j + u < Q && x != 0x7F800000u) {
-          lo = min(lo, x);
-          hi = max(hi, x);
-        }
-      }
-    }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = min(lo, (unsigned)__shfl_xor((int)lo, o, 64));
-    hi = max(hi, (unsigned)__shfl_xor((int)hi, o, 64));
-  }
-  unsigned base = 0u;
-  if (lo <= hi && hi < 0x7F800000u) {
-    unsigned e = lo >> 23;
-    if ((lo & 0x7FFFFFu) == 0u) e = e > 0u ? e - 1u : 0u;  // code 0 is +inf: the least cost must lie above the base
-    if (e >= 1u && hi - (e << 23) < (1u << 24)) base = e << 23;
-  }
-  if (lane == 0) bpack[b] = base;
-  if (base == 0u || !act) return;
   unsigned* cp = CEp + ((Bd.base + g) * L4 + j) * 3;
   for (int s = 0; s < Bd.len; ++s) {
     const f4v c = __builtin_nontemporal_load(ce + (int64_t)s * G * L4);
@@ -1081,7 +1088,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_pack_costs(int64_t nbundles, con
   }
 }
 
-void launch_pack_costs(int64_t nbundles, const Bundle* bundles, const float* CE, int Q, int Qs, unsigned* bpack,
+void launch_pack_costs(int64_t nbundles, const Bundle* bundles, const float* CE, int Q, int Qs, const unsigned* bpack,
                        unsigned* CEp, hipStream_t st) {
   if (nbundles == 0) return;
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
