@@ -175,52 +175,80 @@ __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int
 // accuracy (relative error of d^2 below 1e-11 elsewhere).
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-template <int D4>
+template <int D4, bool kIdx>
 __global__ __launch_bounds__(256) void k_cdist_mfma(int64_t nrows, const int* __restrict__ idx,
                                                     const float* __restrict__ feats, int dim,
                                                     const double* __restrict__ patm, const double* __restrict__ pp,
                                                     const double* __restrict__ w, const double* __restrict__ pat,
                                                     int K, int Kstore, float* __restrict__ out, int out_stride) {
-  extern __shared__ double bsh[];  // [npt * 16][4 D4] pattern operands (the block's copy of patm)
+  // LDS: [npt * 16][4 D4] pattern operands (the block's copy of patm), [4 D4] weights, [npt * 16] pattern norms.
+  extern __shared__ double bsh[];
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int64_t nw = (int64_t)gridDim.x * 4, step = nw * 16;
   const int npt = (Kstore + 15) / 16;
+  double* wsh = bsh + npt * 16 * 4 * D4;
+  double* psh = wsh + 4 * D4;
   for (int e = threadIdx.x; e < npt * 16 * 4 * D4; e += 256) bsh[e] = patm[e];
+  if (w)
+    for (int d = threadIdx.x; d < 4 * D4; d += 256) wsh[d] = d < dim ? w[d] : 0.0;
+  for (int c = threadIdx.x; c < npt * 16; c += 256) psh[c] = pp[c];
   __syncthreads();
   // Whole 16-float chunks load as float4s when the row width is exactly 4 D4 (e.g. 32 or 64 features).
   const bool vec = dim == 4 * D4 && (D4 % 4) == 0;
-  auto load_a = [&](int64_t r0, double* a, int64_t& src) {
-    const int64_t row = r0 + i;
-    src = row < nrows ? (idx ? (int64_t)idx[row] : row) : -1;
+  // Feature row of lane row i in the tile at r (-1: past the end or a padding row), in two steps so that the id load
+  // stays in flight: row_raw loads idx at a valid (clamped) row, row_of interprets the loaded value later.
+  auto row_raw = [&](int64_t r) -> int {
+    const int64_t row = r + i;
+    return kIdx ? idx[row < nrows ? row : nrows - 1] : 0;
+  };
+  auto row_of = [&](int64_t r, int raw) -> int64_t {
+    const int64_t row = r + i;
+    return row < nrows ? (kIdx ? (int64_t)raw : row) : -1;
+  };
+  // This lane's chunk [kk D4, (kk + 1) D4) of row src as raw floats.  Every load has a valid address (row 0 for
+  // src < 0, the last feature past dim) and no lane's register is written twice, so the loads stay in flight until
+  // the values are converted; convert() zeroes what lies past dim or belongs to no row.
+  auto load_x = [&](int64_t src, float* x) {
+    const float* xs = feats + (src >= 0 ? src : 0) * dim;
     if (vec) {
-      const f4v* x4 = reinterpret_cast<const f4v*>(feats + (src >= 0 ? src : 0) * dim + kk * D4);
+      const f4v* x4 = reinterpret_cast<const f4v*>(xs + kk * D4);
 #pragma unroll
       for (int u = 0; u < D4 / 4; ++u) {
-        const f4v v = src >= 0 ? x4[u] : f4v{0.f, 0.f, 0.f, 0.f};
+        const f4v v = x4[u];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) a[4 * u + c] = (double)v[c];
+        for (int c = 0; c < 4; ++c) x[4 * u + c] = v[c];
       }
     } else {
-      const float* x = feats + (src >= 0 ? src : 0) * dim;
 #pragma unroll
       for (int s2 = 0; s2 < D4; ++s2) {
         const int d = kk * D4 + s2;
-        a[s2] = (src >= 0 && d < dim) ? (double)x[d] : 0.0;
+        x[s2] = xs[d < dim ? d : dim - 1];
       }
     }
   };
-  double a[D4], an[D4];
-  int64_t src = -1, srcn = -1;
+  auto convert = [&](int64_t src, const float* x, double* a) {
+#pragma unroll
+    for (int s2 = 0; s2 < D4; ++s2) a[s2] = (src >= 0 && kk * D4 + s2 < dim) ? (double)x[s2] : 0.0;
+  };
+  // The next tile's features are in flight (raw floats) during this tile's products and are converted at the end of
+  // the iteration; its row ids are loaded a tile earlier still, so that its gather issues at once.
+  float xr[D4];
+  double a[D4];
   int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-  if (r0 < nrows) load_a(r0, a, src);
-  for (; r0 < nrows; r0 += nw * 16) {
-    if (r0 + nw * 16 < nrows) load_a(r0 + nw * 16, an, srcn);  // the next tile's rows, in flight meanwhile
+  int64_t src = -1, srcn = -1;
+  if (r0 < nrows) {
+    src = row_of(r0, row_raw(r0));
+    srcn = row_of(r0 + step, row_raw(r0 + step));
+    load_x(src, xr);
+    convert(src, xr, a);
+  }
+  for (; r0 < nrows; r0 += step) {
+    const bool more = r0 + step < nrows;
+    if (more) load_x(srcn, xr);
+    const int rawnn = row_raw(r0 + 2 * step);
     double xx = 0.0;
 #pragma unroll
-    for (int s2 = 0; s2 < D4; ++s2) {
-      const int d = kk * D4 + s2;
-      xx = fma(w ? w[d < dim ? d : 0] * a[s2] : a[s2], a[s2], xx);
-    }
+    for (int s2 = 0; s2 < D4; ++s2) xx = fma(w ? wsh[kk * D4 + s2] * a[s2] : a[s2], a[s2], xx);
     xx += __shfl_xor(xx, 16, 64);
     xx += __shfl_xor(xx, 32, 64);  // lanes i, i + 16, ...: |x|^2 of row i
     double xxr[4];
@@ -254,13 +282,13 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(int64_t nrows, const int* __
           } else if (c >= K) {
             v = 0.f;
           } else {
-            const double ref = xxr[r] + pp[c];
+            const double ref = xxr[r] + psh[c];
             double d2 = ref - 2.0 * acc[r];
             if (d2 < 0x1p-10 * ref) {  // cancellation: recompute directly
-              const float* xr = feats + srcr[r] * dim;
+              const float* xs = feats + srcr[r] * dim;
               d2 = 0.0;
               for (int d = 0; d < dim; ++d) {
-                const double e = (double)xr[d] - pat[((int64_t)(c / 8) * dim + d) * 8 + c % 8];
+                const double e = (double)xs[d] - pat[((int64_t)(c / 8) * dim + d) * 8 + c % 8];
                 d2 = fma(w ? w[d] * e : e, e, d2);
               }
             }
@@ -270,9 +298,12 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(int64_t nrows, const int* __
         }
       }
     }
-#pragma unroll
-    for (int s2 = 0; s2 < D4; ++s2) a[s2] = an[s2];
+    if (more) convert(srcn, xr, a);
+    // An empty register barrier: the loaded id is first read here, so its load stays in flight until now.
+    int raw = rawnn;
+    if (kIdx) asm volatile("" : "+v"(raw));
     src = srcn;
+    srcn = row_of(r0 + 2 * step, raw);
   }
 }
 
@@ -333,18 +364,23 @@ void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, co
                   int Kstore, float* out, int out_stride, const double* w, double* scratch, hipStream_t st) {
   if (nrows == 0 || Kstore == 0) return;
   const int D4 = cdist_mfma_d4(dim);
-  // The pattern operands are staged in LDS: up to 64 KB.
-  if (scratch && D4 && dim > 0 && (size_t)((Kstore + 15) / 16 * 16) * 4 * D4 * sizeof(double) <= 65536) {
+  // The pattern operands, weights and norms are staged in LDS: up to 64 KB.
+  if (scratch && D4 && dim > 0 &&
+      ((size_t)((Kstore + 15) / 16 * 16 + 1) * 4 * D4 + (Kstore + 15) / 16 * 16) * sizeof(double) <= 65536) {
     const int Kp16 = (Kstore + 15) / 16 * 16;
     double* patm = scratch;
     double* pp = scratch + (size_t)Kp16 * 4 * D4;
     hipLaunchKernelGGL(k_build_patm, dim3((Kp16 + 63) / 64), dim3(64), 0, st, K, Kp16, dim, D4, pat, w, patm, pp);
     const int blocks = (int)std::min<int64_t>((nrows + 63) / 64, 8192);
-    const size_t lds = (size_t)Kp16 * 4 * D4 * sizeof(double);
+    const size_t lds = ((size_t)(Kp16 + 1) * 4 * D4 + Kp16) * sizeof(double);  // operands, weights, norms
 #define AC_CD_MFMA(N)                                                                                            \
   case N:                                                                                                        \
-    hipLaunchKernelGGL(k_cdist_mfma<N>, dim3(blocks), dim3(256), lds, st, nrows, idx, feats, dim, patm, pp, w, pat, \
-                       K, Kstore, out, out_stride);                                                              \
+    if (idx)                                                                                                     \
+      hipLaunchKernelGGL((k_cdist_mfma<N, true>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats, dim, patm, \
+                         pp, w, pat, K, Kstore, out, out_stride);                                                \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_cdist_mfma<N, false>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats, dim,      \
+                         patm, pp, w, pat, K, Kstore, out, out_stride);                                          \
     return;
     switch (D4) {
       AC_CD_MFMA(4)
