@@ -362,7 +362,7 @@ __global__ void k_bwd_vertex_coef(int64_t total, const float* __restrict__ CV, f
 // lane/16][lane%16] sits in register r.  Products of floats are exact in FP64; the sums are FP64.  The block's four
 // waves are summed in order.
 typedef double d4v __attribute__((ext_vector_type(4)));
-template <int NQT, int NKT>
+template <int NQT, int NKT, bool kIdx>
 __global__ __launch_bounds__(256) void k_outer_mfma(int64_t nrows, const int* __restrict__ idx,
                                                     const float* __restrict__ feats, int dim,
                                                     const float* __restrict__ G, int Gs, int K, int q0, int k0,
@@ -379,40 +379,71 @@ __global__ __launch_bounds__(256) void k_outer_mfma(int64_t nrows, const int* __
 #pragma unroll
   for (int a = 0; a < NQT; ++a) cs[a] = 0.0;
   // Steps of kOS x 4 rows, software-pipelined: the raw values of the next step are loaded while the current one
-  // runs on the matrix cores.
+  // runs on the matrix cores, and the row ids one step earlier still, so that every load of a step issues at once.
+  // All loads use valid (clamped) addresses and write each register once; the values that lie past K or dim or
+  // belong to no row are zeroed when they are converted, so the loads stay in flight until then.
   constexpr int kOS = 4;
   const int64_t stride = (int64_t)gridDim.x * 4 * (4 * kOS);
   float gf[kOS][NQT], xf[kOS][NKT];
+  int64_t srcl[kOS];  // source rows of the loaded step (-1: none)
+  int sid[kOS];       // raw ids of the step after it
+  auto load_ids = [&](int64_t r0, int* sv) {
+#pragma unroll
+    for (int h = 0; h < kOS; ++h) {
+      const int64_t row = r0 + 4 * h + kr;
+      sv[h] = kIdx ? idx[row < nrows ? row : nrows - 1] : 0;
+    }
+  };
+  auto src_of = [&](int64_t r0, int h, int raw) -> int64_t {
+    const int64_t row = r0 + 4 * h + kr;
+    return row < nrows ? (kIdx ? (int64_t)raw : row) : -1;
+  };
   auto load = [&](int64_t r0) {
 #pragma unroll
     for (int h = 0; h < kOS; ++h) {
       const int64_t row = r0 + 4 * h + kr;
-      int64_t src = -1;
-      if (row < nrows) src = idx ? (int64_t)idx[row] : row;
+      const int64_t rc = row < nrows ? row : nrows - 1, sc = srcl[h] >= 0 ? srcl[h] : 0;
 #pragma unroll
       for (int a = 0; a < NQT; ++a) {
         const int q = q0 + 16 * a + i;
-        gf[h][a] = (src >= 0 && q < K) ? G[row * Gs + q] : 0.f;
+        gf[h][a] = G[rc * Gs + (q < K ? q : K - 1)];
       }
 #pragma unroll
       for (int b = 0; b < NKT; ++b) {
         const int k = k0 + 16 * b + i;
-        xf[h][b] = (src >= 0 && k < dim) ? feats[src * dim + k] : 0.f;
+        xf[h][b] = feats[sc * dim + (k < dim ? k : dim - 1)];
       }
     }
   };
   int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * (4 * kOS);
-  if (r0 < nrows) load(r0);
+  if (r0 < nrows) {
+    int s0[kOS];
+    load_ids(r0, s0);
+#pragma unroll
+    for (int h = 0; h < kOS; ++h) srcl[h] = src_of(r0, h, s0[h]);
+    load(r0);
+    load_ids(r0 + stride, sid);
+  }
   for (; r0 < nrows; r0 += stride) {
     double g[kOS][NQT], x[kOS][NKT];
 #pragma unroll
     for (int h = 0; h < kOS; ++h) {
+      const bool ok = srcl[h] >= 0;
 #pragma unroll
-      for (int a = 0; a < NQT; ++a) g[h][a] = gf[h][a];
+      for (int a = 0; a < NQT; ++a) g[h][a] = (ok && q0 + 16 * a + i < K) ? (double)gf[h][a] : 0.0;
 #pragma unroll
-      for (int b = 0; b < NKT; ++b) x[h][b] = xf[h][b];
+      for (int b = 0; b < NKT; ++b) x[h][b] = (ok && k0 + 16 * b + i < dim) ? (double)xf[h][b] : 0.0;
     }
-    if (r0 + stride < nrows) load(r0 + stride);
+    // The next step's loads are issued unconditionally (past the last step they read clamped, valid rows and are
+    // never used), so that no branch separates them from the loop back-edge.
+#pragma unroll
+    for (int h = 0; h < kOS; ++h) {
+      int raw = sid[h];
+      if (kIdx) asm volatile("" : "+v"(raw));  // an empty register barrier: the id is first needed here
+      srcl[h] = src_of(r0 + stride, h, raw);
+    }
+    load(r0 + stride);
+    load_ids(r0 + 2 * stride, sid);
 #pragma unroll
     for (int h = 0; h < kOS; ++h)
 #pragma unroll
@@ -559,8 +590,12 @@ void launch_outer_acc(int64_t nrows, const int* idx, const float* feats, int dim
   // Matrix-core path in (q, k) blocks of 32 x 32 (each block of the grid covers every row once per launch).
   for (int q0 = 0; q0 < K; q0 += 32)
     for (int k0 = 0; k0 < dim; k0 += 32)
-      hipLaunchKernelGGL((k_outer_mfma<2, 2>), dim3(nparts), dim3(256), 0, st, nrows, idx, feats, dim, G, Gs, K, q0,
-                         k0, part);
+      if (idx)
+        hipLaunchKernelGGL((k_outer_mfma<2, 2, true>), dim3(nparts), dim3(256), 0, st, nrows, idx, feats, dim, G, Gs,
+                           K, q0, k0, part);
+      else
+        hipLaunchKernelGGL((k_outer_mfma<2, 2, false>), dim3(nparts), dim3(256), 0, st, nrows, idx, feats, dim, G,
+                           Gs, K, q0, k0, part);
 }
 
 }  // namespace ac
