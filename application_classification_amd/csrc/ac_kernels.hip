@@ -682,17 +682,29 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_src(int64_t nruns, const Ru
         s[c] = lse_empty();
       }
     };
-    for (int k = R.kb; k < R.ke; ++k) {
-      const int2 e = ks[k];
-      if (e.y != cur) {
-        emit(cur);
-        cur = e.y;
+    // Steps in batches of kSrcBatch: the batch's (row, vertex) pairs, then its cost rows, are loaded together (two
+    // memory round trips per batch rather than per step); the steps are then taken in order.  Slots past the run's
+    // end load its last pair (a valid address) and are not taken.
+    constexpr int kSrcBatch = 8;
+    for (int k0 = R.kb; k0 < R.ke; k0 += kSrcBatch) {
+      int2 e[kSrcBatch];
+      float4 c[kSrcBatch];
+#pragma unroll
+      for (int u = 0; u < kSrcBatch; ++u) e[u] = ks[k0 + u < R.ke ? k0 + u : R.ke - 1];
+#pragma unroll
+      for (int u = 0; u < kSrcBatch; ++u) c[u] = reinterpret_cast<const float4*>(CE + ce_index(e[u].x, 4 * j, Qs))[0];
+#pragma unroll
+      for (int u = 0; u < kSrcBatch; ++u) {
+        if (k0 + u >= R.ke) break;
+        if (e[u].y != cur) {
+          emit(cur);
+          cur = e[u].y;
+        }
+        lse_add(s[0], -(double)c[u].x);
+        lse_add(s[1], -(double)c[u].y);
+        lse_add(s[2], -(double)c[u].z);
+        lse_add(s[3], -(double)c[u].w);
       }
-      const float4 c = reinterpret_cast<const float4*>(CE + ce_index(e.x, 4 * j, Qs))[0];
-      lse_add(s[0], -(double)c.x);
-      lse_add(s[1], -(double)c.y);
-      lse_add(s[2], -(double)c.z);
-      lse_add(s[3], -(double)c.w);
     }
     if (R.slot >= 0) {
 #pragma unroll
