@@ -1,12 +1,16 @@
 """Benchmark of the matcher on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config s22|s18|small] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config s22|s18|small|batched|diff]
+                    [--no-cpu-baseline] [--no-pruned] [--no-profile]
 
 A step is one full match of the workload: prep (cdist and edge-cost statistics, src/main.cu:107-148) plus pn
 iterations of the loop (src/main.cu:160-245) plus the output write, with the graphs already resident in HBM.
 Throughput is the reference's unit of work, one (data edge, pattern edge) pair per iteration:
-    value = de * pe * pn / t_loop   (edge-pair updates per second, t_loop = device time of the iterations).
-The default workload is R-MAT scale 22 (64-dim features, planted 16-vertex / 24-edge pattern).
+    value = de * pe * pn / t_loop   (edge-pair updates per second; t_loop = device time of the iterations and the
+                                     output write, the reference's timed region without its prep).
+The default workload is R-MAT scale 22 (64-dim features, planted 16-vertex / 24-edge pattern).  --config batched
+scores 256 uneven patterns against R-MAT 20 (batched against sequential runs); --config diff times the adjoint
+(forward+backward against forward, 64 patterns on R-MAT 18).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
