@@ -197,29 +197,127 @@ __global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const do
   }
 }
 
-// Sparse edge terms: gE_F to its in-edge row FA (gVR of the row's source, -gE_F to the row's cost) and gE_R to the
-// minimum-cost out-edge row AO (-gE_R to its cost).  gVR == nullptr: the prep, whose edge terms involve no VR.
-__global__ __launch_bounds__(256) void k_bwd_scatter(BwdGraph G, const float* __restrict__ gEF,
-                                                     const float* __restrict__ gER, const int* __restrict__ FA,
-                                                     const int* __restrict__ AO, const int* __restrict__ bsrc,
-                                                     float* __restrict__ gVR, float* __restrict__ gCE) {
-  const int Q = G.Q;
-  const int64_t total = G.n * Q;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = i / Q;
-    const int q = (int)(i - v * Q);
-    const int64_t o = v * G.Qs + q;
-    const float gf = gEF[o], gr = gER[o];
-    if (gf != 0.f) {
-      const int row = FA[o];
-      if (row >= 0) {
-        if (gVR) atomicAdd(&gVR[(int64_t)(bsrc[row] & ~kNewVertex) * G.Qs + q], gf);
-        atomicAdd(&gCE[(int64_t)row * G.Qs + q], -gf);
+// Sparse edge terms, gathered per source in CSR order so that every sum has one owner and a fixed order (the result
+// is reproducible bit for bit):
+//   FMax argmax: the in-edge row FA[v][q] of v = dst(row) receives -gE_F[v][q], and the row's source accumulates
+//                gVR[src][q] += gE_F[v][q];
+//   RMax argmax: the minimum-cost out-edge row AO[src][q] receives -gE_R[src][q].
+// A row's FMax term is added before its RMax term (each (row, q) is written by the one lane walking that edge).  A
+// source's gVR terms are summed in FP64 along its out-edges and added once; the pieces of a split source leave FP64
+// partials in carry slots, which k_bwd_gather_fixup adds in piece order.  gVR == nullptr: the prep, whose edge terms
+// involve no VR.  Only nonzero edge gradients count (a max whose fill won routes nothing to its recorded row).
+//
+// One group of Qs/4 lanes walks one CSR run (lane j owning pattern edges 4j..4j+3), loading the (row, source) pairs,
+// then the rows' destinations, then the destinations' FA rows of a batch of edges together.
+__global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const Run* __restrict__ runs,
+                                                           const int2* __restrict__ ks, const int* __restrict__ bdst,
+                                                           int Q, int Qs, const float* __restrict__ gEF,
+                                                           const float* __restrict__ gER, const int* __restrict__ FA,
+                                                           const int* __restrict__ AO, float* __restrict__ gVR,
+                                                           float* __restrict__ gCE, double* __restrict__ carry) {
+  const int L4 = Qs / 4, G = 64 / L4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane / L4, j = lane - g * L4;
+  const int64_t r = ((int64_t)blockIdx.x * (kWalkBlock / 64) + wave) * G + g;
+  if (g >= G || r >= nruns) return;
+  const Run R = runs[r];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  int cur = ks[R.kb].y;
+  int4 ao = reinterpret_cast<const int4*>(AO)[(int64_t)cur * L4 + j];
+  float4 gr = reinterpret_cast<const float4*>(gER)[(int64_t)cur * L4 + j];
+  auto emit = [&](int src) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int q = 4 * j + c;
+      if (gVR && q < Q && acc[c] != 0.0) {
+        float* o = gVR + (int64_t)src * Qs + q;
+        *o = (float)((double)*o + acc[c]);
+      }
+      acc[c] = 0.0;
+    }
+  };
+  constexpr int kBatch = 8;
+  for (int k0 = R.kb; k0 < R.ke; k0 += kBatch) {
+    int2 e[kBatch];
+    int v[kBatch];
+    int4 fa[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) e[u] = ks[k0 + u < R.ke ? k0 + u : R.ke - 1];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = bdst[e[u].x];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) fa[u] = reinterpret_cast<const int4*>(FA)[(int64_t)v[u] * L4 + j];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (k0 + u >= R.ke) break;
+      if (e[u].y != cur) {
+        emit(cur);
+        cur = e[u].y;
+        ao = reinterpret_cast<const int4*>(AO)[(int64_t)cur * L4 + j];
+        gr = reinterpret_cast<const float4*>(gER)[(int64_t)cur * L4 + j];
+      }
+      const int row = e[u].x;
+      const int fr[4] = {fa[u].x, fa[u].y, fa[u].z, fa[u].w};
+      const int ar[4] = {ao.x, ao.y, ao.z, ao.w};
+      const float grv[4] = {gr.x, gr.y, gr.z, gr.w};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int q = 4 * j + c;
+        if (q >= Q) continue;
+        const float gf = fr[c] == row ? gEF[(int64_t)v[u] * Qs + q] : 0.f;
+        const float gq = ar[c] == row ? grv[c] : 0.f;
+        if (gf == 0.f && gq == 0.f) continue;
+        float* o = gCE + (int64_t)row * Qs + q;
+        float t = *o;
+        if (gf != 0.f) {
+          t += -gf;
+          acc[c] += (double)gf;
+        }
+        if (gq != 0.f) t += -gq;
+        *o = t;
       }
     }
-    if (gr != 0.f) {
-      const int row = AO[o];
-      if (row >= 0 && row != INT_MAX) atomicAdd(&gCE[(int64_t)row * G.Qs + q], -gr);
+  }
+  if (R.slot >= 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) carry[(int64_t)R.slot * Qs + 4 * j + c] = acc[c];
+  } else {
+    emit(cur);
+  }
+}
+
+// gVR of the split sources: one wave per split, groups of Qs/4 lanes taking its pieces round-robin, the groups'
+// FP64 sums merged in group order through LDS, then added once.
+__global__ __launch_bounds__(256) void k_bwd_gather_fixup(int nsplit, const Split* __restrict__ splits,
+                                                          const double* __restrict__ carry, int Q, int Qs,
+                                                          float* __restrict__ gVR) {
+  __shared__ double red[4][64][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int L4 = Qs / 4, G = 64 / L4;
+  const int g = lane / L4, j = lane - g * L4;
+  const int sidx = blockIdx.x * 4 + wave;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  Split S{0, 0, 0, 0};
+  if (sidx < nsplit) S = splits[sidx];
+  if (sidx < nsplit && g < G)
+    for (int c = g; c < S.count; c += G) {
+      const double* o = carry + (int64_t)(S.first + c) * Qs + 4 * j;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[u] += o[u];
+    }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) red[wave][lane][u] = s[u];
+  __syncthreads();
+  if (sidx < nsplit && g == 0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double a = 0.0;
+      for (int gg = 0; gg < G; ++gg) a += red[wave][gg * L4 + j][u];
+      const int q = 4 * j + u;
+      if (q < Q && a != 0.0) {
+        float* o = gVR + (int64_t)S.v * Qs + q;
+        *o = (float)((double)*o + a);
+      }
     }
   }
 }
@@ -546,10 +644,15 @@ void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const
   hipLaunchKernelGGL(k_bwd_in, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, S, MU, qs1, red4, gER, gVR, gVF, cD);
 }
 
-void launch_bwd_scatter(const BwdGraph& G, const float* gEF, const float* gER, const int* FA, const int* AO,
-                        const int* bsrc, float* gVR, float* gCE, hipStream_t st) {
-  if (G.Q == 0) return;
-  hipLaunchKernelGGL(k_bwd_scatter, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, gEF, gER, FA, AO, bsrc, gVR, gCE);
+void launch_bwd_gather(const BwdGraph& G, int64_t nruns, const Run* runs, int nsplit, const Split* splits,
+                       const int2* ks, const int* bdst, const float* gEF, const float* gER, const int* FA, const int* AO,
+                       float* gVR, float* gCE, double* carry, hipStream_t st) {
+  if (G.Q == 0 || nruns == 0) return;
+  hipLaunchKernelGGL(k_bwd_gather, dim3(walk_blocks(nruns, G.Qs)), dim3(kWalkBlock), 0, st, nruns, runs, ks, bdst, G.Q,
+                     G.Qs, gEF, gER, FA, AO, gVR, gCE, carry);
+  if (gVR && nsplit > 0)
+    hipLaunchKernelGGL(k_bwd_gather_fixup, dim3((unsigned)((nsplit + 3) / 4)), dim3(256), 0, st, nsplit, splits, carry,
+                       G.Q, G.Qs, gVR);
 }
 
 void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, float* gFX, float* gRX,

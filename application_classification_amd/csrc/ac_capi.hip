@@ -874,14 +874,39 @@ struct Snapshot {
   float* dst;
 };
 
+// Captured iteration pairs of one loop: one per parity of the pair's first iteration, because iteration `it`
+// reads the scalar slot that iteration it - 1 wrote (slot 1 + ((it - 1) & 1)) and writes slot 1 + (it & 1), and a
+// captured launch keeps the slot addresses it was captured with.  Each graph is kept with its executable until the
+// loop's work has finished.
+struct PairGraphs {
+  hipGraph_t g[2] = {nullptr, nullptr};
+  hipGraphExec_t x[2] = {nullptr, nullptr};
+  void release() {
+    for (int i = 0; i < 2; ++i) {
+      if (x[i]) (void)hipGraphExecDestroy(x[i]);
+      if (g[i]) (void)hipGraphDestroy(g[i]);
+      x[i] = nullptr;
+      g[i] = nullptr;
+    }
+  }
+};
+
+// A profiling tool loaded into the process (rocprofv3 names its tool library in ROCP_TOOL_LIBRARIES): graph replay
+// is then left off, so that every iteration is an ordinary launch the tool traces.
+static bool profiler_attached() {
+  const char* t = std::getenv("ROCP_TOOL_LIBRARIES");
+  return t && *t;
+}
+
 // The loop (src/main.cu:160-245) from the prep state: n_iter iterations, the snapshots (sorted by t) taken as their
 // iteration counts are reached, and, without snapshots, the final output into OUT.  With `graph`, iterations
-// 1, 2, ... are replayed in pairs (odd, even: one per parity of the scalar slots) from a captured graph wherever no
+// 1, 2, ... are replayed in pairs from captured graphs (one per parity of the pair's first iteration) wherever no
 // snapshot falls inside the pair.  Ends with event ev[2].
 static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& snaps, bool graph) {
   hipStream_t st = ctx->stream;
   ac::VertexArgs va = vertex_args(ctx);
   const QStat* qs_state = ctx->qstat.p;  // scalars finishing FMaxN / VFm as they currently are
+  const int qstride = std::max(1, ctx->Q);
   size_t si = 0;
   auto take = [&](int t) -> int {
     while (si < snaps.size() && snaps[si].t == t) {
@@ -894,34 +919,45 @@ static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& sn
   };
   int rc = take(0);
   if (rc != AC_OK) return rc;
-  hipGraphExec_t exec = nullptr;
+  if (graph && profiler_attached()) graph = false;
+  PairGraphs pg;
+  bool replayed = false;
   int it = 0;
   while (it < n_iter) {
     const bool pair = graph && it >= 1 && it + 1 < n_iter && !(si < snaps.size() && snaps[si].t == it + 1);
     if (pair) {
-      if (!exec) {
-        hipGraph_t g = nullptr;
+      const int par = it & 1;
+      if (!pg.x[par]) {
+        // The pair reads the slot iteration it - 1 wrote; it is where qs_state points now.
         HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         const QStat* q0 = qs_state;
         int rc1 = enqueue_iteration(ctx, va, it, q0);
         if (rc1 == AC_OK) rc1 = enqueue_iteration(ctx, va, it + 1, q0);
-        const hipError_t e = hipStreamEndCapture(st, &g);
+        const hipError_t e = hipStreamEndCapture(st, &pg.g[par]);
+        if (rc1 == AC_OK && e != hipSuccess)
+          rc1 = ctx->fail(AC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        if (rc1 == AC_OK) {
+          const hipError_t ei = hipGraphInstantiate(&pg.x[par], pg.g[par], nullptr, nullptr, 0);
+          if (ei != hipSuccess) {
+            pg.x[par] = nullptr;
+            rc1 = ctx->fail(AC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+          }
+        }
         if (rc1 != AC_OK) {
-          if (g) (void)hipGraphDestroy(g);
+          if (replayed) (void)hipStreamSynchronize(st);
+          pg.release();
           return rc1;
         }
-        HIPCHK(ctx, e);
-        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIPCHK(ctx, ei);
       }
-      const hipError_t el = hipGraphLaunch(exec, st);
+      const hipError_t el = hipGraphLaunch(pg.x[par], st);
       if (el != hipSuccess) {
-        (void)hipGraphExecDestroy(exec);
+        (void)hipStreamSynchronize(st);
+        pg.release();
         HIPCHK(ctx, el);
       }
-      qs_state = ctx->qstat.p + (1 + ((it + 1) & 1)) * std::max(1, ctx->Q);
-      if (ctx->Q == 0) qs_state = ctx->qstat.p;
+      replayed = true;
+      // The pair's second iteration, it + 1, wrote slot 1 + ((it + 1) & 1).
+      qs_state = ctx->Q == 0 ? ctx->qstat.p : ctx->qstat.p + (1 + ((it + 1) & 1)) * qstride;
       it += 2;
     } else {
       rc = enqueue_iteration(ctx, va, it, qs_state);
@@ -935,10 +971,11 @@ static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& sn
     rc = take(it);
     if (rc != AC_OK) break;
   }
-  if (exec) {
-    // The graph's launches stay valid after the executable is destroyed once they have been enqueued.
-    HIPCHK(ctx, hipStreamSynchronize(st));
-    (void)hipGraphExecDestroy(exec);
+  if (pg.x[0] || pg.x[1]) {
+    // Executables and graphs are destroyed only after every replay has finished.
+    const hipError_t es = hipStreamSynchronize(st);
+    pg.release();
+    HIPCHK(ctx, es);
   }
   if (rc != AC_OK) return rc;
   if (snaps.empty()) {
@@ -1251,6 +1288,18 @@ static int batch_iters(ac_ctx* ctx, const int32_t* n_iter, std::vector<int>& ite
   return AC_OK;
 }
 
+// Batched runs and the autotuner run untaped: they never feed ac_backward, and the tape buffers are sized by
+// ac_run for its own pattern, not for a slab.  The guard restores the caller's AC_OPT_TAPE.
+struct TapeOff {
+  ac_ctx* c;
+  int saved;
+  explicit TapeOff(ac_ctx* ctx) : c(ctx), saved(ctx->tape_K) {
+    c->tape_K = 0;
+    c->tape_valid = false;
+  }
+  ~TapeOff() { c->tape_K = saved; }
+};
+
 static int batch_check(ac_ctx* ctx, const char* who) {
   if (!ctx->has_data) return ctx->fail(AC_ERR_STATE, std::string(who) + ": set the data graph first");
   if (ctx->bp.empty()) return ctx->fail(AC_ERR_STATE, std::string(who) + ": set the patterns first");
@@ -1264,6 +1313,7 @@ extern "C" int ac_batch_run(ac_ctx* ctx, const int32_t* n_iter) {
   int rc = batch_check(ctx, "ac_batch_run");
   if (rc != AC_OK) return rc;
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  TapeOff untaped(ctx);
   std::vector<int> iters;
   batch_iters(ctx, n_iter, iters);
   const std::vector<std::vector<int>> slabs = pack_slabs(ctx, iters, ctx->slab_edges);
@@ -1348,6 +1398,7 @@ extern "C" int ac_batch_autotune(ac_ctx* ctx, int probe_iters, int32_t* choice_o
   if (rc != AC_OK) return rc;
   if (probe_iters < 1 || probe_iters > 64) return ctx->fail(AC_ERR_ARG, "ac_batch_autotune: probe_iters in [1, 64]");
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  TapeOff untaped(ctx);
   std::vector<int> iters;
   batch_iters(ctx, nullptr, iters);
   const int caps[3] = {64, 128, 256};
@@ -1565,8 +1616,9 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVR.p, 0, nq * sizeof(float), st));
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
       ac::launch_bwd_in(G, S0, ctx->MUd.p, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, st);
-      ac::launch_bwd_scatter(G, ctx->gEF.p, ctx->gER.p, ctx->tapeFA.p + s * nq, ctx->AO.p, ctx->bs->bsrc.p,
-                             ctx->gVR.p, ctx->gCE.p, st);
+      ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p,
+                            ctx->bs->csr_kb.p, ctx->bs->bdst.p, ctx->gEF.p, ctx->gER.p, ctx->tapeFA.p + s * nq,
+                            ctx->AO.p, ctx->gVR.p, ctx->gCE.p, ctx->carry_out.p, st);
       ac::launch_bwd_mu(G, ctx->gVR.p, ctx->gVF.p, ctx->gMU.p, ctx->gFX.p, ctx->gRX.p, st);
     }
   }
@@ -1577,7 +1629,9 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   if (Q > 0) {
     ac::launch_bwd_route(G, S0, nullptr, 0, ctx->gFX.p, ctx->gRX.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
                          ctx->red4.p, st);
-    ac::launch_bwd_scatter(G, ctx->gEF.p, ctx->gER.p, ctx->FA0.p, ctx->AO.p, ctx->bs->bsrc.p, nullptr, ctx->gCE.p, st);
+    ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p, ctx->bs->csr_kb.p,
+                          ctx->bs->bdst.p, ctx->gEF.p, ctx->gER.p, ctx->FA0.p, ctx->AO.p, nullptr, ctx->gCE.p,
+                          ctx->carry_out.p, st);
     ac::launch_bwd_prep_fill(G, ctx->red4.p, ctx->am0.p, ctx->gMU.p, st);
     std::vector<double> red4((size_t)4 * Q);
     std::vector<QStat> qs0(Q);

@@ -125,6 +125,50 @@ def test_graph_replay_is_bit_identical(small):
             np.testing.assert_array_equal(plain(d, p, 4, graph=False), plain(d, p, 4, graph=True))
 
 
+def test_batch_graph_replay_both_parities_bit_exact(small, uneven):
+    """Snapshots after 4 and 9 iterations in one slab: the snapshot at 4 forces iteration 3 to be launched alone, so
+    the pairs before it start at odd iterations (1-2) and the pairs after it at even ones (4-5, 6-7).  Each parity
+    replays its own captured pair (the scalar slots a pair reads and writes depend on its parity), so the graph run
+    is bit-identical to the launched one at both snapshots."""
+    d = small.data
+    pats = uneven[:2]
+    assert sum(p.num_edges for p in pats) <= 256
+    out = {}
+    for graph in (0, 1):
+        with Matcher(0) as m:
+            m.set_option(AC_OPT_SLAB_EDGES, 256)
+            m.set_option(AC_OPT_GRAPH, graph)
+            m.set_data(d)
+            m.batch_set_patterns(pats)
+            m.batch_run([4, 9])
+            assert m.batch_info()["slabs"] == 1
+            out[graph] = [m.batch_mu(i) for i in range(len(pats))]
+    for i in range(len(pats)):
+        say(f"pattern {i}: graph replay bit-identical = {np.array_equal(out[1][i], out[0][i])}")
+        np.testing.assert_array_equal(out[1][i], out[0][i])
+    assert_close(out[1][1], plain(d, pats[1], 9), "9 iterations vs a plain run")
+
+
+def test_batch_ignores_the_adjoint_tape(small, uneven):
+    """AC_OPT_TAPE configures ac_run's adjoint tape; a batched run on the same context runs untaped (its slabs never
+    feed ac_backward) and leaves the option as it was."""
+    from application_classification_amd.matcher import AC_OPT_TAPE
+
+    d = small.data
+    pats = uneven[:3]
+    res = []
+    for tape in (0, 4):
+        with Matcher(0) as m:
+            m.set_option(AC_OPT_TAPE, tape)
+            m.set_data(d)
+            m.batch_set_patterns(pats)
+            m.batch_run()
+            assert m.get_option(AC_OPT_TAPE) == tape
+            res.append([m.batch_mu(i) for i in range(len(pats))])
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("B", [1, 2, 4])
 def test_walk_batch_sizes_agree(small, B):
     """Every edge-walk batch size walks the same bundle layout to the same (exact) maxima."""

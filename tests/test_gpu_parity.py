@@ -452,3 +452,77 @@ def test_past_int32_cost_elements():
             else:
                 say(f"{name}: bit-identical = {np.array_equal(mu, base)}")
                 np.testing.assert_array_equal(mu, base)
+
+
+# --- the headline workload (R-MAT 22, 64-dim, planted 16 / 24) and its shape at oracle scale ------------------------
+
+def test_headline_shape_against_oracle():
+    """The headline workload's shape at a size the FP64 oracle finishes in seconds: R-MAT scale 16 (65,536 vertices,
+    about 1 M edges), 64-dim features, a planted 16-vertex / 24-edge pattern (24 pattern edges: 6 lanes per run, 10
+    runs per bundle, 4 idle lanes), so most bundles walk 24-bit cost codes.  Every log-probability within the
+    parity bar, the planted vertices recovered, and the FP32-cost and unpruned walks bit-identical."""
+    from application_classification_amd.matcher import AC_OPT_PACK_COSTS, AC_OPT_PRUNE
+
+    w = W.rmat(16, 64, 16, 24, seed=0)
+    d, p = w.data, w.pattern
+    say(f"{w.name}: {d.num_edges} edges")
+    outs = {}
+    for name, opts in [("default", {}), ("FP32 costs", {AC_OPT_PACK_COSTS: 0}), ("unpruned", {AC_OPT_PRUNE: 0})]:
+        with Matcher(0) as m:
+            for k, v in opts.items():
+                m.set_option(k, v)
+            m.set_data(d)
+            m.set_pattern(p)
+            outs[name] = m.run()
+            if name == "default":
+                nb, packed = m.cost_packing()
+                say(f"{packed} of {nb} bundles packed")
+                assert packed >= 0.75 * nb
+    ref = O.run(g64(d), g64(p))
+    assert_parity(outs["default"], ref, w.name)
+    np.testing.assert_array_equal(outs["default"].argmax(axis=0), p.vmap)
+    for name in ("FP32 costs", "unpruned"):
+        np.testing.assert_array_equal(outs[name], outs["default"])
+
+
+def test_headline_workload_rmat22_d64_p16x24():
+    """The benchmark's own workload, exactly (R-MAT scale 22, 65.2 M edges, 64-dim features, planted 16 / 24 pattern,
+    16 iterations).  The oracle would need about 79 GB and hours here, so the checks are the ones the problem fixes
+    at any size: the planted vertices are recovered, every MU column is a normalised log-distribution (column
+    log-sum-exp within 1e-5 of 0, no positive entry), and the FP32-cost walk, the pruned walk and walk batch 1 are
+    bit-identical to the default (packed, unpruned, batch 4) run the benchmark times."""
+    from scipy.special import logsumexp
+
+    from application_classification_amd.matcher import AC_OPT_PACK_COSTS, AC_OPT_PRUNE, AC_OPT_WALK_BATCH
+
+    w = W.s22(0)
+    d, p = w.data, w.pattern
+    say(f"{w.name}: dn={d.num_nodes} de={d.num_edges} pn={p.num_nodes} pe={p.num_edges}")
+    assert (d.num_nodes, p.num_nodes, p.num_edges, d.efeat.shape[1]) == (1 << 22, 16, 24, 64)
+    variants = [("default", {}), ("FP32 costs", {AC_OPT_PACK_COSTS: 0}), ("pruned", {AC_OPT_PRUNE: 1}),
+                ("walk batch 1", {AC_OPT_WALK_BATCH: 1})]
+    with Matcher(0) as m:
+        m.set_option(AC_OPT_PRUNE, 0)
+        m.set_data(d)
+        m.set_pattern(p)
+        base = None
+        for name, opts in variants:
+            saved = {k: m.get_option(k) for k in opts}
+            for k, v in opts.items():
+                m.set_option(k, v)
+            mu = m.run()
+            for k, v in saved.items():
+                m.set_option(k, v)
+            if base is None:
+                base = mu
+                nb, packed = m.cost_packing()
+                say(f"{packed} of {nb} bundles packed")
+                assert mu.shape == (d.num_nodes, p.num_nodes) and np.isfinite(mu).all()
+                assert (mu <= 0).all()
+                lse = logsumexp(mu.astype(np.float64), axis=0)
+                say(f"max |column log-sum-exp| = {np.abs(lse).max():.3e}")
+                assert np.abs(lse).max() <= 1e-5
+                np.testing.assert_array_equal(mu.argmax(axis=0), p.vmap)
+            else:
+                say(f"{name}: bit-identical = {np.array_equal(mu, base)}")
+                np.testing.assert_array_equal(mu, base)
