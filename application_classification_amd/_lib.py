@@ -27,6 +27,7 @@ AC_OPT_TAPE = 4
 AC_OPT_PRUNE = 5
 AC_OPT_CDIST_MFMA = 6
 AC_OPT_PACK_COSTS = 7
+AC_OPT_CNULL = 8
 STAGES = ["prep", "vertex_a", "reduce_a", "edge_walk", "walk_fixup", "vertex_c", "reduce_c", "output"]
 
 ERROR_NAMES = {
@@ -58,6 +59,21 @@ class HostGraph(ctypes.Structure):
     ]
 
 
+class Graph64(ctypes.Structure):
+    """The reference's Graph at its own precision (uint64 sizes and ids, FP64 features)."""
+
+    _fields_ = [
+        ("num_nodes", ctypes.c_uint64),
+        ("node_feat_dim", ctypes.c_uint64),
+        ("num_edges", ctypes.c_uint64),
+        ("edge_feat_dim", ctypes.c_uint64),
+        ("node_feats", ctypes.POINTER(ctypes.c_double)),
+        ("edge_feats", ctypes.POINTER(ctypes.c_double)),
+        ("srcs", ctypes.POINTER(ctypes.c_uint64)),
+        ("dsts", ctypes.POINTER(ctypes.c_uint64)),
+    ]
+
+
 P = ctypes.c_void_p
 I = ctypes.c_int
 I64 = ctypes.c_int64
@@ -72,6 +88,11 @@ SIGNATURES = {
     "ac_graph_load_csv": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(HostGraph)]),
     "ac_graph_save_csv": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(HostGraph)]),
     "ac_graph_free": (None, [ctypes.POINTER(HostGraph)]),
+    "ac_graph64_load_bin": (I, [ctypes.c_char_p, ctypes.POINTER(Graph64)]),
+    "ac_graph64_save_bin": (I, [ctypes.c_char_p, ctypes.POINTER(Graph64)]),
+    "ac_graph64_load_csv": (I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Graph64)]),
+    "ac_graph64_free": (None, [ctypes.POINTER(Graph64)]),
+    "ac_set_graph64": (I, [P, I, ctypes.POINTER(Graph64)]),
     "ac_gen_er": (I, [I64, I64, U64, P, P]),
     "ac_gen_rmat": (I, [I, I, D, D, D, U64, ctypes.POINTER(I64), ctypes.POINTER(P), ctypes.POINTER(P)]),
     "ac_gen_normal": (I, [P, I64, U64, U64, F]),
@@ -88,6 +109,7 @@ SIGNATURES = {
     "ac_sync": (I, [P]),
     "ac_get_mu": (I, [P, P]),
     "ac_get_match": (I, [P, F, P]),
+    "ac_get_topk": (I, [P, I, P, P]),
     "ac_get_sizes": (I, [P, P]),
     "ac_get_timing": (I, [P, ctypes.POINTER(D), ctypes.POINTER(D)]),
     "ac_get_cost_packing": (I, [P, P]),

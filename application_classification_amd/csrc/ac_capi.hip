@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -10,6 +11,8 @@
 #include <memory>
 #include <string>
 #include <vector>
+
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "../../include/ac_mi355x.h"
 #include "ac_adjoint.hpp"
@@ -26,6 +29,17 @@ using ac::Run;
 using ac::Split;
 
 namespace {
+
+// rocTX range over a scope, named like the reference's NVTX ranges (src/main.cu:107,161-244: "prep", "loop" per
+// iteration, "step1".."step5"); the fused kernels each cover several of the reference's steps, and their ranges name
+// all of them.  The ranges time the host-side enqueue (launches are asynchronous) and appear under rocprofv3
+// --marker-trace.
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 // Edges per run of an edge walk (a tuning knob, AC_RUN_EDGES).  Vertices whose edge list is longer are split into
 // pieces of this length.
@@ -60,6 +74,16 @@ int gather_stride(int Qs) {
   return s;
 }
 
+// Largest single device allocation a context makes (AC_MAX_ALLOC_BYTES, read at each allocation; unset: no cap).
+// A larger request fails as out-of-memory, the way an allocation too large for HBM does: for sharing a GPU with other
+// work, and for exercising the out-of-memory path.
+size_t max_alloc_bytes() {
+  const char* e = std::getenv("AC_MAX_ALLOC_BYTES");
+  if (!e || !*e) return SIZE_MAX;
+  const unsigned long long v = std::strtoull(e, nullptr, 10);
+  return v ? (size_t)v : SIZE_MAX;
+}
+
 // Device buffer that only grows: a smaller request reuses the allocation (n is the logical size), so runs of
 // patterns of different sizes do not free and re-allocate multi-GB buffers.
 template <typename T>
@@ -82,6 +106,7 @@ struct DevBuf {
     }
     release();
     if (count == 0) return hipSuccess;
+    if (count > max_alloc_bytes() / sizeof(T)) return hipErrorOutOfMemory;
     hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
     if (e != hipSuccess) {
       p = nullptr;
@@ -198,6 +223,11 @@ struct ac_ctx {
   bool use_graph = false;  // replay iteration pairs from a captured graph
   bool prune = true;       // skip bundles whose edges cannot raise FMax above the fill (exact)
   bool pack_costs = true;  // walks read 24-bit codes of the costs where a bundle allows them (exact)
+  // The PNNL Cnull term (off: the reference, src/main.cu:83).  cnull_groups: the pattern-edge ranges over which it is
+  // normalised, one per pattern (a batched slab holds several); d_cnull holds it per pattern edge when on.
+  bool cnull = false;
+  std::vector<int> cnull_groups;
+  DevBuf<double> d_cnull;
   bool costs_packed = false;  // the last prep wrote CEp / bpack for the current CE
   DevBuf<unsigned> CEp;    // [rows * Qs * 3 / 4] packed costs of the bundles whose base is nonzero
   DevBuf<unsigned> bpack;  // [bundles] base of a bundle's codes, 0 = FP32 rows
@@ -481,6 +511,7 @@ static int set_pattern(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int
   HIPCHK(ctx, ctx->d_pd.upload(pd_pad.data(), pd_pad.size(), st));
   HIPCHK(ctx, ctx->d_inc_off.upload(inc_off.data(), inc_off.size(), st));
   HIPCHK(ctx, ctx->d_inc.upload(inc.data(), inc.size(), st));
+  ctx->cnull_groups.assign({0, (int)nE});
   ctx->has_patt = true;
   ctx->have_result = false;
   return AC_OK;
@@ -504,6 +535,34 @@ extern "C" int ac_set_graph(ac_ctx* ctx, int which, int64_t nV, int64_t dV, cons
   HIPCHK(ctx, hipSetDevice(ctx->device));
   if (which == AC_DATA) return set_data(ctx, nV, dV, vf, nE, dE, ef, src, dst);
   return set_pattern(ctx, nV, dV, vf, nE, dE, ef, src, dst);
+}
+
+extern "C" int ac_set_graph64(ac_ctx* ctx, int which, const ac_graph64* g) {
+  if (!ctx) {
+    ac::set_global_error("ac_set_graph64: null context");
+    return AC_ERR_ARG;
+  }
+  if (!g) return ctx->fail(AC_ERR_ARG, "ac_set_graph64: null graph");
+  if (g->num_nodes < 1 || g->num_nodes > (uint64_t)INT32_MAX || g->num_edges > (uint64_t)INT32_MAX ||
+      g->node_feat_dim > 4096 || g->edge_feat_dim > 4096)
+    return ctx->fail(AC_ERR_ARG, "ac_set_graph64: sizes out of range (need 1 <= num_nodes < 2^31, edges < 2^31)");
+  const uint64_t nV = g->num_nodes, nE = g->num_edges;
+  std::vector<float> vf(nV * g->node_feat_dim), ef(nE * g->edge_feat_dim);
+  std::vector<int32_t> s(nE), d(nE);
+  ac::parallel_chunks((int64_t)vf.size(), [&](int64_t b, int64_t e, int) {
+    for (int64_t i = b; i < e; ++i) vf[i] = (float)g->node_feats[i];
+  });
+  ac::parallel_chunks((int64_t)ef.size(), [&](int64_t b, int64_t e, int) {
+    for (int64_t i = b; i < e; ++i) ef[i] = (float)g->edge_feats[i];
+  });
+  for (uint64_t e = 0; e < nE; ++e) {
+    if (g->srcs[e] >= nV || g->dsts[e] >= nV)
+      return ctx->fail(AC_ERR_ARG, "ac_set_graph64: edge " + std::to_string(e) + " has an endpoint >= num_nodes");
+    s[e] = (int32_t)g->srcs[e];
+    d[e] = (int32_t)g->dsts[e];
+  }
+  return ac_set_graph(ctx, which, (int64_t)nV, (int64_t)g->node_feat_dim, vf.data(), (int64_t)nE,
+                      (int64_t)g->edge_feat_dim, ef.data(), s.data(), d.data());
 }
 
 static int walk_batch_of(const ac_ctx* ctx) { return ctx->walk_B > 0 ? ctx->walk_B : ac::walk_batch(); }
@@ -679,8 +738,34 @@ static int prepare_run(ac_ctx* ctx) {
   if (ctx->pdV != ctx->dV || ctx->pdE != ctx->dE)
     return ctx->fail(AC_ERR_ARG, "ac_run: feature dimensions of the data and pattern graphs differ");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  return ensure_work(ctx);
+  if (ctx->cnull && ctx->tape_K > 0)
+    return ctx->fail(AC_ERR_ARG, "ac_run: AC_OPT_CNULL is not differentiated by ac_backward; use it without AC_OPT_TAPE");
+  int rc = ensure_work(ctx);
+  if (rc != AC_OK || !ctx->cnull || ctx->Q == 0) return rc;
+  // Cnull (test.py:93-95): normprob over each pattern's edges of max(||edge features||_2, max_e normprob_e(CE)).  The
+  // second term is a log-probability (<= 0) and the norm is >= 0, so the maximum is the norm, exactly; cnull[q] is the
+  // log-softmax of the pattern's edge feature norms.
+  std::vector<double> c((size_t)ctx->Q, 0.0);
+  const int dE = ctx->pdE;
+  for (size_t gi = 0; gi + 1 < ctx->cnull_groups.size(); ++gi) {
+    const int q0 = ctx->cnull_groups[gi], q1 = ctx->cnull_groups[gi + 1];
+    if (q0 >= q1) continue;
+    double m = -std::numeric_limits<double>::infinity();
+    for (int q = q0; q < q1; ++q) {
+      double ss = 0.0;
+      for (int k = 0; k < dE; ++k) ss += (double)ctx->pefeat[(size_t)q * dE + k] * (double)ctx->pefeat[(size_t)q * dE + k];
+      c[q] = std::sqrt(ss);
+      m = std::max(m, c[q]);
+    }
+    double sum = 0.0;
+    for (int q = q0; q < q1; ++q) sum += std::exp(c[q] - m);
+    for (int q = q0; q < q1; ++q) c[q] = std::log(std::exp(c[q] - m) / sum);
+  }
+  HIPCHK(ctx, ctx->d_cnull.upload(c.data(), c.size(), ctx->stream));
+  return AC_OK;
 }
+
+static const double* cnull_of(const ac_ctx* ctx) { return ctx->cnull && ctx->Q > 0 ? ctx->d_cnull.p : nullptr; }
 
 // Copies the live iteration state (FMaxN, VFm, the given scalars, pstat) into checkpoint c.
 static int checkpoint(ac_ctx* ctx, int c, const QStat* qs) {
@@ -720,6 +805,7 @@ static void launch_fixup32(ac_ctx* ctx, hipStream_t st) {
 
 // Prep (src/main.cu:107-148), between events ev[0] and ev[1].
 static int enqueue_prep(ac_ctx* ctx) {
+  Range range("prep");
   hipStream_t st = ctx->stream;
   const int64_t n = ctx->dn;
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
@@ -833,13 +919,18 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
   hipStream_t st = ctx->stream;
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
   QStat* qs_new = ctx->qstat.p + (1 + (it & 1)) * std::max(1, Q);
+  Range loop("loop");
   if (Q > 0) {
     va.first = it == 0;
     va.qs = qs_state;
-    ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
-    MARK(AC_STAGE_VERTEX_A);
-    ac::launch_reduce_q(ctx->nb_vertex + 1, Q, ctx->qpart.p, qs_new, st);
-    MARK(AC_STAGE_REDUCE_A);
+    {
+      Range r("step1+step3+step4a");  // VF, VR; VFmax, RE's normaliser, RMax; VRmax, FE's normaliser
+      ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
+      MARK(AC_STAGE_VERTEX_A);
+      ac::launch_reduce_q(ctx->nb_vertex + 1, Q, ctx->qpart.p, qs_new, st, cnull_of(ctx));
+      MARK(AC_STAGE_REDUCE_A);
+    }
+    Range r("step2+step4b+step4c");  // FE messages, normalised, scatter-max by destination
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p,
                         walk_batch_of(ctx), st, nullptr, nullptr, ctx->prune ? ctx->bmin.p : nullptr,
@@ -851,6 +942,7 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
   }
   va.first = 0;
   va.qs = qs_state;
+  Range r5("step5");  // MU sums and MU's normaliser
   ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex_c, st);
   MARK(AC_STAGE_VERTEX_C);
   ac::launch_reduce_lse(ctx->nb_vertex_c + 1, P, ctx->ppart.p, ctx->pstat.p, st);
@@ -1048,6 +1140,37 @@ extern "C" int ac_get_match(ac_ctx* ctx, float log_tau, uint8_t* host_out) {
   return AC_OK;
 }
 
+extern "C" int ac_get_topk(ac_ctx* ctx, int k, int32_t* idx_out, float* val_out) {
+  if (!ctx || !idx_out) return ctx ? ctx->fail(AC_ERR_ARG, "ac_get_topk: null output") : AC_ERR_ARG;
+  if (k < 1 || (int64_t)k > ctx->dn)
+    return ctx->fail(AC_ERR_ARG, "ac_get_topk: need 1 <= k <= number of data vertices");
+  const int64_t n = ctx->dn;
+  const int P = ctx->P;
+  std::vector<float> mu((size_t)(n * P));
+  int rc = ac_get_mu(ctx, mu.data());
+  if (rc != AC_OK) return rc;
+  // Per pattern vertex: the k best (value descending, vertex ascending on ties), by a bounded selection on the
+  // calling thread (dn * pn comparisons, no worker threads).
+  for (int64_t p = 0; p < P; ++p) {
+    auto better = [&](int a, int b) {
+      const float x = mu[(size_t)a * P + p], y = mu[(size_t)b * P + p];
+      return x > y || (x == y && a < b);
+    };
+    std::vector<int> best;
+    best.reserve((size_t)k + 1);
+    for (int64_t v = 0; v < n; ++v) {
+      if ((int)best.size() == k && !better((int)v, best.back())) continue;
+      best.insert(std::upper_bound(best.begin(), best.end(), (int)v, better), (int)v);
+      if ((int)best.size() > k) best.pop_back();
+    }
+    for (int r = 0; r < k; ++r) {
+      idx_out[p * k + r] = best[r];
+      if (val_out) val_out[p * k + r] = mu[(size_t)best[r] * P + p];
+    }
+  }
+  return AC_OK;
+}
+
 extern "C" int ac_get_sizes(const ac_ctx* ctx, int64_t* out4) {
   if (!ctx || !out4) return AC_ERR_ARG;
   out4[0] = ctx->dn;
@@ -1165,6 +1288,10 @@ extern "C" int ac_set_option(ac_ctx* ctx, int option, int value) {
     case AC_OPT_PACK_COSTS:
       ctx->pack_costs = value != 0;
       return AC_OK;
+    case AC_OPT_CNULL:
+      ctx->cnull = value != 0;
+      ctx->have_result = false;
+      return AC_OK;
     default:
       return ctx->fail(AC_ERR_ARG, "ac_set_option: unknown option " + std::to_string(option));
   }
@@ -1193,6 +1320,9 @@ extern "C" int ac_get_option(const ac_ctx* ctx, int option, int* value) {
       return AC_OK;
     case AC_OPT_PACK_COSTS:
       *value = ctx->pack_costs ? 1 : 0;
+      return AC_OK;
+    case AC_OPT_CNULL:
+      *value = ctx->cnull ? 1 : 0;
       return AC_OK;
     default:
       return AC_ERR_ARG;
@@ -1279,7 +1409,13 @@ static int set_slab_pattern(ac_ctx* ctx, const std::vector<int>& members, std::v
     }
     base += B.P;
   }
-  return ac_set_graph(ctx, AC_PATTERN, base, dV, vf.data(), (int64_t)s.size(), dE, ef.data(), s.data(), d.data());
+  const int rc = ac_set_graph(ctx, AC_PATTERN, base, dV, vf.data(), (int64_t)s.size(), dE, ef.data(), s.data(),
+                              d.data());
+  if (rc != AC_OK) return rc;
+  // Cnull is normalised over each member pattern's own edges.
+  ctx->cnull_groups.assign(1, 0);
+  for (int i : members) ctx->cnull_groups.push_back(ctx->cnull_groups.back() + ctx->bp[i].Q);
+  return AC_OK;
 }
 
 static int batch_iters(ac_ctx* ctx, const int32_t* n_iter, std::vector<int>& iters) {

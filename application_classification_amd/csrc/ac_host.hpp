@@ -1,10 +1,13 @@
 // Host-side helpers shared by the C-ABI implementation, the generators and the file readers.
 #pragma once
 
+#include <sched.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <thread>
 #include <vector>
@@ -15,14 +18,25 @@ namespace ac {
 void set_global_error(const std::string& msg);
 const char* global_error();
 
+// Worker threads for host loops: the CPUs this process may run on (its affinity mask, not the machine's count),
+// further limited by OMP_NUM_THREADS when that is set, at most 64.
 inline int host_threads() {
-  unsigned n = std::thread::hardware_concurrency();
-  if (n == 0) n = 1;
+  long n = 0;
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+  if (n <= 0) n = (long)std::thread::hardware_concurrency();
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const long v = std::atol(e);
+    if (v > 0 && v < n) n = v;
+  }
+  if (n < 1) n = 1;
   if (n > 64) n = 64;
   return (int)n;
 }
 
-// Static-chunked parallel loop over [0, n): fn(begin, end, worker_id).
+// Static-chunked parallel loop over [0, n): fn(begin, end, worker_id).  The calling thread takes chunk 0; a chunk
+// whose thread cannot be started runs on the calling thread instead, so a failed thread start costs time, not the
+// process.
 template <typename Fn>
 void parallel_chunks(int64_t n, Fn fn, int threads = 0) {
   if (threads <= 0) threads = host_threads();
@@ -34,28 +48,43 @@ void parallel_chunks(int64_t n, Fn fn, int threads = 0) {
   int64_t per = (n + threads - 1) / threads;
   std::vector<std::thread> pool;
   pool.reserve(threads);
-  for (int t = 0; t < threads; ++t) {
+  std::vector<int> inline_chunks;
+  for (int t = 1; t < threads; ++t) {
     int64_t b = (int64_t)t * per, e = std::min(n, b + per);
     if (b >= e) break;
-    pool.emplace_back([=] { fn(b, e, t); });
+    try {
+      pool.emplace_back([=] { fn(b, e, t); });
+    } catch (...) {
+      inline_chunks.push_back(t);
+    }
   }
+  fn((int64_t)0, std::min(n, per), 0);
+  for (int t : inline_chunks) fn((int64_t)t * per, std::min(n, (int64_t)t * per + per), t);
   for (auto& th : pool) th.join();
 }
 
-// Dynamic work queue over [0, n) items: fn(item).
+// Dynamic work queue over [0, n) items: fn(item).  The calling thread takes items too, so the queue drains even
+// when no thread can be started.
 template <typename Fn>
 void parallel_items(int64_t n, Fn fn, int threads = 0) {
   if (threads <= 0) threads = host_threads();
   std::atomic<int64_t> next{0};
+  auto work = [&] {
+    for (;;) {
+      const int64_t i = next.fetch_add(1);
+      if (i >= n) break;
+      fn(i);
+    }
+  };
   std::vector<std::thread> pool;
-  for (int t = 0; t < threads; ++t)
-    pool.emplace_back([&] {
-      for (;;) {
-        int64_t i = next.fetch_add(1);
-        if (i >= n) break;
-        fn(i);
-      }
-    });
+  for (int t = 1; t < threads && t < n; ++t) {
+    try {
+      pool.emplace_back(work);
+    } catch (...) {
+      break;
+    }
+  }
+  work();
   for (auto& th : pool) th.join();
 }
 

@@ -112,7 +112,8 @@ void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, co
 size_t cdist_scratch_doubles(int Kstore, int dim);
 void launch_col_lse_neg(int64_t n, int P, const float* X, Lse* part, int nblocks, hipStream_t st);
 void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStream_t st);
-void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st);
+// cnull (nullptr: off): per pattern edge, the PNNL Cnull term subtracted from both fills of the new state.
+void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st, const double* cnull = nullptr);
 // Per column p of the n rows X[n][P]: IsoCol{LSE of -X, min X and its lowest row + base}; partial buffers of
 // nparts * P entries each.
 void launch_iso_cols(int64_t n, int P, const float* X, int base, Lse* lpart, float* pv, int* pi, int nparts,

@@ -20,6 +20,7 @@
 #include <type_traits>
 
 #include "ac_internal.hpp"
+#include "ac_lse.hpp"
 
 namespace ac {
 
@@ -55,41 +56,6 @@ __device__ __forceinline__ f4v decode_costs(U3 w, unsigned base) {
   const unsigned c3 = w.z >> 8;
   return f4v{decode_cost(c0, base), decode_cost(c1, base), decode_cost(c2, base), decode_cost(c3, base)};
 }
-
-// One exp per value and no branches: the exponent is always min - max of (x, m), which rescales t when x is a new
-// maximum and is the new term otherwise.  exp(-inf) = 0 covers the empty accumulator.  min - max is formed as
-// -|x - m|: rounding to nearest is sign-symmetric, so fl(min - max) = -|fl(x - m)| exactly, and x > m exactly when
-// fl(x - m) > 0 (the difference of two distinct finite doubles never rounds to zero; m = -inf gives +inf).
-__device__ __forceinline__ void lse_add(Lse& s, double x) {
-  const bool valid = x != -kInf;
-  const double d = x - s.m;  // NaN only for x = m = -inf, which is not valid
-  const double e = (double)__expf((float)(-fabs(d)));
-  const double t = d > 0.0 ? (1.0 + s.t) * e : s.t + e;
-  s.t = valid ? t : s.t;
-  s.m = valid ? fmax(x, s.m) : s.m;
-}
-
-// lse_add for a finite x (no -inf term to skip): the same result with fewer selects.  An empty accumulator
-// (m = -inf) takes x as its maximum: exp(-inf) = 0 clears t.
-__device__ __forceinline__ void lse_add_finite(Lse& s, double x) {
-  const double d = x - s.m;
-  const double e = (double)__expf((float)(-fabs(d)));
-  s.t = d > 0.0 ? (1.0 + s.t) * e : s.t + e;
-  s.m = fmax(x, s.m);
-}
-
-__device__ __forceinline__ Lse lse_merge(Lse a, Lse b) {
-  if (b.m == -kInf) return a;
-  if (a.m == -kInf) return b;
-  if (a.m >= b.m) {
-    a.t += (1.0 + b.t) * exp(b.m - a.m);
-    return a;
-  }
-  b.t += (1.0 + a.t) * exp(a.m - b.m);
-  return b;
-}
-
-__device__ __forceinline__ Lse lse_empty() { return Lse{-kInf, 0.0}; }
 
 // max with the position of the maximum: the larger value wins, equal values keep the lower vertex (-1 = none).
 __device__ __forceinline__ void argmax_merge(double& m, int& a, double x, int b) {
@@ -476,8 +442,11 @@ void launch_reduce_lse(int nparts, int ncol, const Lse* part, Lse2* out, hipStre
   hipLaunchKernelGGL(k_reduce_lse, dim3(ncol), dim3(256), 0, st, nparts, ncol, part, out);
 }
 
+// cnull != nullptr: the iteration's fills are lowered by the PNNL Cnull term, fillF = max VR - cnull[q] and
+// fillR = max VF - cnull[q] (test.py:148-149; cnull <= 0, so the fills rise and the pruning bound below, which assumes
+// the unraised fill, stays valid).
 __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart* __restrict__ part,
-                                                  QStat* __restrict__ out) {
+                                                  const double* __restrict__ cnull, QStat* __restrict__ out) {
   __shared__ QPart red[256];
   const int q = blockIdx.x, t = threadIdx.x;
   const QPart empty{-kInf, -kInf, lse_empty(), lse_empty(), -1, -1};
@@ -514,13 +483,14 @@ __global__ __launch_bounds__(256) void k_reduce_q(int nparts, int Q, const QPart
     double cthr = kInf;
     if (fF > -kInf && fF < kInf && sl > -kInf && sl < kInf)
       cthr = -sl + 0x1p-20 * (fabs(fF) + 2.0 * fabs(sl) + 1.0);
-    out[q] = QStat{r.vrmax, r.vfmax, r.re.m, log1p(r.re.t), r.fe.m, log1p(r.fe.t), r.avr, r.avf, cthr};
+    const double fillF = cnull ? r.vrmax - cnull[q] : r.vrmax, fillR = cnull ? r.vfmax - cnull[q] : r.vfmax;
+    out[q] = QStat{fillF, fillR, r.re.m, log1p(r.re.t), r.fe.m, log1p(r.fe.t), r.avr, r.avf, cthr};
   }
 }
 
-void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st) {
+void launch_reduce_q(int nparts, int Q, const QPart* part, QStat* out, hipStream_t st, const double* cnull) {
   if (Q == 0) return;
-  hipLaunchKernelGGL(k_reduce_q, dim3(Q), dim3(256), 0, st, nparts, Q, part, out);
+  hipLaunchKernelGGL(k_reduce_q, dim3(Q), dim3(256), 0, st, nparts, Q, part, cnull, out);
 }
 
 // Isolated-vertex columns: one pass over X[n][P] giving per-block partials of LSE(-X) and of (min X, lowest row).
@@ -806,7 +776,8 @@ int walk_batch() {
 
 // kArg: also records, per (v, q), the row of the in-edge attaining the maximum (the first one in walk order on
 // ties; -1 if none) into FA[v][Qs], and per carry slot into carg, for the adjoint's max-subgradient routing.
-template <bool kWithVR, int B, bool kArg>
+// kPrune (bmin != nullptr): exact pruning, per bundle and per lane (below).
+template <bool kWithVR, int B, bool kArg, bool kPrune>
 __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const Bundle* __restrict__ bundles,
                                                          const int* __restrict__ gslot, const int* __restrict__ bsrc,
                                                          const int* __restrict__ bdst, const float* __restrict__ CE,
@@ -823,10 +794,13 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   if (g >= G || b >= nbundles) return;
   const Bundle Bd = bundles[b];
   const int slot = gslot[b * G + g];
-  if (bmin) {
-    // Exact pruning: when no edge of the bundle has a cost below its pattern edge's bound, none of its terms can
-    // raise FMax above the fill; the bundle's vertices get -inf (the fill then wins, as it would have).
-    bool live = false;
+  // Exact pruning: an edge whose cost is at or above its pattern edge's bound cannot raise FMax above the fill, so a
+  // term that is -inf instead gives the same FMax (the fill wins, as it would have).  A wave with no live lane skips
+  // its bundle and writes -inf rows; a dead lane (none of its four pattern edges has a cost below the bound in this
+  // bundle) skips its cost loads and VR gathers, and its maxima stay -inf.
+  bool live = true;
+  if (kPrune) {
+    live = false;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int q = 4 * j + u;
@@ -897,14 +871,17 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
     const int t_ = (T) < last ? (T) : last;                                                              \
     _Pragma("unroll") for (int u = 0; u < B; ++u) {                                                      \
       if constexpr (kP)                                                                                  \
-        C[u] = nt_load_u3(cpb + 3 * (co + (unsigned)((t_ * B + u) * GL4)));                              \
+        C[u] = (!kPrune || live) ? nt_load_u3(cpb + 3 * (co + (unsigned)((t_ * B + u) * GL4)))           \
+                                 : U3{0u, 0u, 0u};                                                       \
       else                                                                                               \
-        C[u] = __builtin_nontemporal_load(ceb + (co + (unsigned)((t_ * B + u) * GL4)));                  \
+        C[u] = (!kPrune || live) ? __builtin_nontemporal_load(ceb + (co + (unsigned)((t_ * B + u) * GL4))) \
+                                 : f4v{INFINITY, INFINITY, INFINITY, INFINITY};                          \
     }                                                                                                    \
     _Pragma("unroll") for (int u = 0; u < B; ++u) D[u] =                                                 \
         kFlags ? (S[u] & kNewVertex) : __builtin_nontemporal_load(dstp + (t_ * B + u) * G);              \
     _Pragma("unroll") for (int u = 0; u < B; ++u) W[u] =                                                 \
-        kWithVR ? VRv[(int64_t)(S[u] & ~kNewVertex) * VR4s + j] : f4v{0.f, 0.f, 0.f, 0.f};               \
+        (kWithVR && (!kPrune || live)) ? VRv[(int64_t)(S[u] & ~kNewVertex) * VR4s + j]                   \
+                                       : f4v{0.f, 0.f, 0.f, 0.f};                                        \
   }
 #define AC_WK_REDUCE(C, D, W, T)                                                    \
   _Pragma("unroll") for (int u = 0; u < B; ++u) {                                   \
@@ -991,25 +968,38 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
 #undef AC_WK_EMIT
 }
 
-template <bool kWithVR, bool kArg>
+template <bool kWithVR, bool kArg, bool kPrune>
 static void launch_walk_max_t(int B, int nb, hipStream_t st, int64_t nbundles, const Bundle* bundles,
                               const int* gslot, const int* bsrc, const int* bdst, const float* CE, const float* VR,
                               int VRs, int Q, int Qs, const QStat* qs, float* FMaxN, double* carry, int* FA,
                               int* carg, const float* bmin, const unsigned* CEp, const unsigned* bpack) {
   switch (B) {
     case 1:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 1, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
-                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 1, kArg, kPrune>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles,
+                         bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
       break;
     case 4:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 4, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
-                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 4, kArg, kPrune>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles,
+                         bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
       break;
     default:
-      hipLaunchKernelGGL((k_walk_max<kWithVR, 2, kArg>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, gslot,
-                         bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
+      hipLaunchKernelGGL((k_walk_max<kWithVR, 2, kArg, kPrune>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles,
+                         bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
       break;
   }
+}
+
+template <bool kWithVR, bool kArg>
+static void launch_walk_max_p(int B, int nb, hipStream_t st, int64_t nbundles, const Bundle* bundles,
+                              const int* gslot, const int* bsrc, const int* bdst, const float* CE, const float* VR,
+                              int VRs, int Q, int Qs, const QStat* qs, float* FMaxN, double* carry, int* FA,
+                              int* carg, const float* bmin, const unsigned* CEp, const unsigned* bpack) {
+  if (bmin)
+    launch_walk_max_t<kWithVR, kArg, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs,
+                                           FMaxN, carry, FA, carg, bmin, CEp, bpack);
+  else
+    launch_walk_max_t<kWithVR, kArg, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs,
+                                            FMaxN, carry, FA, carg, bmin, CEp, bpack);
 }
 
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
@@ -1020,16 +1010,16 @@ void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, 
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
   if (!bpack) CEp = nullptr;
   if (VR && FA)
-    launch_walk_max_t<true, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
+    launch_walk_max_p<true, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
                                   carry, FA, carg, bmin, CEp, bpack);
   else if (VR)
-    launch_walk_max_t<true, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
+    launch_walk_max_p<true, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
                                    carry, FA, carg, bmin, CEp, bpack);
   else if (FA)
-    launch_walk_max_t<false, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
+    launch_walk_max_p<false, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
                                    carry, FA, carg, bmin, nullptr, nullptr);
   else
-    launch_walk_max_t<false, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
+    launch_walk_max_p<false, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
                                     carry, FA, carg, bmin, nullptr, nullptr);
 }
 

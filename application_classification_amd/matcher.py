@@ -14,9 +14,10 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (AC_DATA, AC_OPT_GRAPH, AC_OPT_PACK_COSTS, AC_OPT_PRUNE, AC_OPT_SLAB_EDGES,  # noqa: F401
+from ._lib import (AC_DATA, AC_OPT_CNULL, AC_OPT_GRAPH, AC_OPT_PACK_COSTS, AC_OPT_PRUNE,  # noqa: F401
+                   AC_OPT_SLAB_EDGES,
                    AC_OPT_TAPE, AC_OPT_WALK_BATCH, AC_PATTERN,
-                   ACError, HostGraph)
+                   ACError, Graph64, HostGraph)
 
 
 @dataclass
@@ -106,6 +107,80 @@ def save_csv(vertex_csv: str, edge_csv: str, g) -> None:
     _lib.check(_lib.lib().ac_graph_save_csv(vertex_csv.encode(), edge_csv.encode(), ctypes.byref(_host_graph(g))))
 
 
+@dataclass
+class Graph64Arrays:
+    """The reference's Graph at its own precision (src/main.cu:8-19): features float64, ids uint64."""
+
+    vfeat: np.ndarray
+    efeat: np.ndarray
+    src: np.ndarray
+    dst: np.ndarray
+
+    @property
+    def num_nodes(self) -> int:
+        return int(self.vfeat.shape[0])
+
+    @property
+    def num_edges(self) -> int:
+        return int(self.src.shape[0])
+
+
+def _copy_graph64(g: Graph64) -> Graph64Arrays:
+    nV, dV, nE, dE = g.num_nodes, g.node_feat_dim, g.num_edges, g.edge_feat_dim
+
+    def arr(ptr, count, dtype):
+        if count == 0:
+            return np.zeros(0, dtype)
+        return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+    return Graph64Arrays(arr(g.node_feats, nV * dV, np.float64).reshape(nV, dV),
+                         arr(g.edge_feats, nE * dE, np.float64).reshape(nE, dE),
+                         arr(g.srcs, nE, np.uint64), arr(g.dsts, nE, np.uint64))
+
+
+def _graph64(g) -> tuple:
+    """(Graph64 struct, the arrays it points into) for any graph-like object, at FP64 / uint64."""
+    vf = np.ascontiguousarray(g.vfeat, dtype=np.float64)
+    ef = np.ascontiguousarray(g.efeat, dtype=np.float64)
+    if ef.ndim == 1:
+        ef = ef.reshape(len(g.src), -1)
+    s = np.ascontiguousarray(g.src, dtype=np.uint64)
+    d = np.ascontiguousarray(g.dst, dtype=np.uint64)
+    st = Graph64(vf.shape[0], vf.shape[1], s.shape[0], ef.shape[1] if ef.ndim == 2 else 0,
+                 vf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ef.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                 s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    return st, (vf, ef, s, d)
+
+
+def load_bin64(path: str) -> Graph64Arrays:
+    """Read a .bin at full precision (loadGraph, src/main.cu:21-47)."""
+    L = _lib.lib()
+    g = Graph64()
+    _lib.check(L.ac_graph64_load_bin(path.encode(), ctypes.byref(g)))
+    try:
+        return _copy_graph64(g)
+    finally:
+        L.ac_graph64_free(ctypes.byref(g))
+
+
+def load_csv64(vertex_csv: str, edge_csv: str) -> Graph64Arrays:
+    """Read the CSV pair as prob2bin.py:11-39 does (pandas' decimal conversion), at full precision."""
+    L = _lib.lib()
+    g = Graph64()
+    _lib.check(L.ac_graph64_load_csv(vertex_csv.encode(), edge_csv.encode(), ctypes.byref(g)))
+    try:
+        return _copy_graph64(g)
+    finally:
+        L.ac_graph64_free(ctypes.byref(g))
+
+
+def save_bin64(path: str, g) -> None:
+    """Write the .bin of prob2bin.py:46-64 from FP64 features."""
+    st, keep = _graph64(g)
+    _lib.check(_lib.lib().ac_graph64_save_bin(path.encode(), ctypes.byref(st)))
+    del keep
+
+
 class Matcher:
     """One matcher context on one GPU (ac_create / ac_set_graph / ac_run / ac_get_mu)."""
 
@@ -148,6 +223,17 @@ class Matcher:
         self._dn = g.num_nodes
         self._feat_dims = (g.vfeat.shape[1], g.efeat.shape[1] if g.efeat.ndim == 2 else 0)
 
+    def set_graph64(self, which: int, g) -> None:
+        """Hand over a graph at the reference's own precision (ac_set_graph64: FP64 features, uint64 ids)."""
+        st, keep = _graph64(g)
+        self._check(self._L.ac_set_graph64(self._h, which, ctypes.byref(st)))
+        del keep
+        if which == AC_DATA:
+            self._dn = int(st.num_nodes)
+            self._feat_dims = (int(st.node_feat_dim), int(st.edge_feat_dim))
+        else:
+            self._pn = int(st.num_nodes)
+
     def set_pattern(self, g) -> None:
         self._pn = self._set(AC_PATTERN, g).num_nodes
 
@@ -163,14 +249,25 @@ class Matcher:
         self._check(self._L.ac_sync(self._h))
 
     def mu(self) -> np.ndarray:
-        out = np.empty((self._dn, self._pn), np.float32)
+        # Sized from the library's own record of the graphs: ac_get_mu writes dn x pn floats.
+        dn, _, pn, _ = self.sizes()
+        out = np.empty((dn, pn), np.float32)
         self._check(self._L.ac_get_mu(self._h, out.ctypes.data))
         return out
 
     def match(self, log_tau: float = float(np.log(0.5))) -> np.ndarray:
-        out = np.empty((self._dn, self._pn), np.uint8)
+        dn, _, pn, _ = self.sizes()
+        out = np.empty((dn, pn), np.uint8)
         self._check(self._L.ac_get_match(self._h, log_tau, out.ctypes.data))
         return out
+
+    def topk(self, k: int):
+        """(idx, val), each pn x k: per pattern vertex the k most likely data vertices (ac_get_topk)."""
+        n, _, pn, _ = self.sizes()
+        idx = np.empty((pn, k), np.int32)
+        val = np.empty((pn, k), np.float32)
+        self._check(self._L.ac_get_topk(self._h, k, idx.ctypes.data, val.ctypes.data))
+        return idx, val
 
     def timing(self):
         """(prep_ms, loop_ms) of the last run, from HIP events on the context's stream."""
@@ -211,8 +308,9 @@ class Matcher:
     def backward(self, grad_mu):
         """dL/dMU (dn x pn) of the last taped run (AC_OPT_TAPE > 0) -> (dL/dwv, dL/dwe)."""
         g = np.ascontiguousarray(grad_mu, np.float32)
-        if g.shape != (self._dn, self._pn):
-            raise ValueError(f"grad_mu must be {self._dn} x {self._pn}")
+        dn, _, pn, _ = self.sizes()
+        if g.shape != (dn, pn):
+            raise ValueError(f"grad_mu must be {dn} x {pn}")
         dv, de = self._feat_dims
         gwv = np.zeros(dv, np.float32)
         gwe = np.zeros(de, np.float32)
@@ -251,7 +349,7 @@ class Matcher:
         self._check(self._L.ac_batch_run(self._h, None if arr is None else arr.ctypes.data))
 
     def batch_mu(self, i: int) -> np.ndarray:
-        out = np.empty((self._dn, self._bpn[i]), np.float32)
+        out = np.empty((self.sizes()[0], self._bpn[i]), np.float32)
         self._check(self._L.ac_batch_get_mu(self._h, i, out.ctypes.data))
         return out
 

@@ -27,6 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# Arithmetic of the path: values stored in FP32 (costs, maxima, VR rows) and per-term exponentials in FP32; every
+# difference, maximum and log-sum-exp merge in FP64 registers.
+DTYPE = "f32 storage, f64 accumulation"
 
 
 def log(msg):
@@ -44,28 +47,53 @@ def algorithmic_bytes(dn, de, pn, pe):
     }
 
 
-def cpu_baseline(threads):
-    """The FP64 C++ restatement of the reference (oracle/), timed on this host's cores over the reference's timed
-    region (prep + loop + transpose), on a scale-18 instance of the benchmark workload."""
+def cpu_model():
+    """The host CPU's model name (lscpu "Model name", else /proc/cpuinfo), or None."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return None
+
+
+def _cpu_leg(scale, threads, precision):
     from application_classification_amd import workloads as W
     from oracle import oracle as O
 
-    w = W.rmat(18, 64, 16, 24, seed=0)
+    w = W.rmat(scale, 64, 16, 24, seed=0)
     d, p = w.data, w.pattern
     D = O.Graph64.of(d.vfeat, d.efeat, d.src, d.dst)
     P = O.Graph64.of(p.vfeat, p.efeat, p.src, p.dst)
-    prep_s, loop_s, _ = O.timed_run(D, P, threads=threads)
+    prep_s, loop_s, _ = O.timed_run(D, P, threads=threads, precision=precision)
     de, pe, pn = d.num_edges, p.num_edges, p.num_nodes
-    return {
-        "value": de * pe * pn / loop_s,
-        "unit": "edge-pair updates/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"R-MAT scale 18 ({d.num_nodes} vertices, {de} edges), 64-dim features, planted 16/24 pattern, "
-                  f"{pn} iterations, FP64; loop {loop_s:.2f} s, prep {prep_s:.2f} s",
-        "loop_s": loop_s,
-        "prep_s": prep_s,
-    }
+    return {"value": de * pe * pn / loop_s, "threads": threads, "precision": precision,
+            "sample": f"R-MAT scale {scale} ({d.num_nodes} vertices, {de} edges), 64-dim features, planted 16/24 "
+                      f"pattern, {pn} iterations, {'FP64' if precision == 'f64' else 'FP32'}; loop {loop_s:.2f} s, prep {prep_s:.2f} s"}
+
+
+def cpu_baseline(threads, legs=True):
+    """The C++ restatement of the reference (oracle/: the same stages and materialised pattern x data buffers as
+    src/main.cu, OpenMP loops in place of Thrust), timed on this host over the reference's loop region, on bounded
+    samples of the benchmark workload's shape (64-dim features, planted 16/24 pattern).  The headline leg is FP64
+    (the reference's precision) on every host thread at R-MAT scale 18; the other legs are FP32 on every thread and
+    FP64 on one thread (at scale 16, so that it stays within seconds)."""
+    main = _cpu_leg(18, threads, "f64")
+    out = {"value": main["value"], "unit": "edge-pair updates/s", "cores": threads, "kind": "port",
+           "sample": main["sample"], "model": cpu_model()}
+    if legs:
+        out["legs"] = [_cpu_leg(18, threads, "f32"), _cpu_leg(16, 1, "f64")]
+    return out
 
 
 STAGE_OF_KERNEL = {"k_walk_max<true": "edge_walk", "k_vertex<0>": "vertex_a", "k_vertex<1>": "vertex_c",
@@ -155,10 +183,10 @@ def measured_traffic(workload, stage):
     return None
 
 
-def bench_diff(args):
+def measure_diff(steps, warmup):
     """Differentiation: R-MAT scale 18 (32-dim) with 64 planted 8-16-vertex patterns; per pattern the forward
     (prep + iterations + output) alone, then forward with the tape (checkpoint every 4 iterations) + ac_backward of
-    the planted cross-entropy.  A step is one pass over the 64 patterns."""
+    the planted cross-entropy.  A step is one pass over the 64 patterns; the figure is the best step of each side."""
     from application_classification_amd import workloads as W
     from application_classification_amd.matcher import AC_OPT_TAPE, Matcher
 
@@ -192,27 +220,15 @@ def bench_diff(args):
         m.set_option(AC_OPT_TAPE, 4 if tape else 0)
         m.set_data(g)
         m.set_weights(wv, we)
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             fwd_pass(m, tape)
-        times[tape] = min(fwd_pass(m, tape) for _ in range(args.steps))
+        times[tape] = min(fwd_pass(m, tape) for _ in range(steps))
         m.close()
         log(f"{'forward+backward' if tape else 'forward'}: {times[tape]:.1f} ms per pass over {len(pats)} patterns")
-    out = {
-        "metric": "forward+backward time / forward time (64 patterns, R-MAT 18, 32-dim, checkpoint every 4)",
-        "value": times[True] / times[False],
-        "unit": "ratio",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": times[True],
-        "forward_ms_per_step": times[False],
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "storage_dtype": "f32",
-        "gradient_dtype": "f32",
-        "data": "synthetic",
+    return {
+        "ratio": times[True] / times[False],
+        "forward_backward_ms": times[True],
+        "forward_ms": times[False],
         "config": {"workload": "rmat18-d32-64-patterns-8to16", "data_vertices": g.num_nodes,
                    "data_edges": g.num_edges, "patterns": len(pats), "checkpoint_every": 4,
                    "updates_per_pass": upd, "parallelism": "single-gpu"},
@@ -220,13 +236,36 @@ def bench_diff(args):
                    "grad_we_norm": float(np.linalg.norm(grads[1].sum(0))),
                    "all_finite": bool(np.isfinite(grads[0]).all() and np.isfinite(grads[1]).all())},
     }
+
+
+def bench_diff(args):
+    r = measure_diff(args.steps, args.warmup)
+    out = {
+        "metric": "forward+backward time / forward time (64 patterns, R-MAT 18, 32-dim, checkpoint every 4)",
+        "value": r["ratio"],
+        "unit": "ratio",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["forward_backward_ms"],
+        "forward_ms_per_step": r["forward_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE,
+        "gradient_dtype": "f32",
+        "data": "synthetic",
+        "config": r["config"],
+        "checks": r["checks"],
+    }
     print(json.dumps(out), flush=True)
 
 
-def bench_batched(args):
+def measure_batched(steps, warmup, phase="both"):
     """256 uneven planted patterns (V in [4, 32], E in [V, 2V]) against one resident R-MAT scale-20 graph (32-dim):
     the whole job (prep + iterations + output of every pattern) as 256 sequential plain runs, then as batched runs
-    (autotuned slabs, graph-replayed iterations).  A step is one pass over all 256 patterns."""
+    (autotuned slabs, graph-replayed iterations).  A step is one pass over all 256 patterns; figures are the best
+    step of each side.  phase "sequential" or "batched" times one side only (for per-side hardware counters)."""
     from application_classification_amd import workloads as W
     from application_classification_amd.matcher import AC_OPT_GRAPH, Matcher
 
@@ -247,12 +286,13 @@ def bench_batched(args):
     # Sequential plain runs: one ac_set_graph + ac_run per pattern, as the reference's driver scores one pattern.
     # The sampled patterns' results are read back before the timed passes (neither side times its copies).
     ref = {}
-    for i in sample:
-        m.set_pattern(pats[i])
-        m.run()
-        ref[i] = m.mu()
+    if phase != "batched":
+        for i in sample:
+            m.set_pattern(pats[i])
+            m.run()
+            ref[i] = m.mu()
     seq_ms = []
-    for step in range(args.steps):
+    for step in range(steps if phase != "batched" else 0):
         dev = 0.0
         t0 = time.perf_counter()
         for i, p in enumerate(pats):
@@ -262,54 +302,56 @@ def bench_batched(args):
             a, b = m.timing()
             dev += a + b
         seq_ms.append(((time.perf_counter() - t0) * 1e3, dev))
-        log(f"sequential step {step + 1}/{args.steps}: wall {seq_ms[-1][0]:.1f} ms, device {dev:.1f} ms")
+        log(f"sequential step {step + 1}/{steps}: wall {seq_ms[-1][0]:.1f} ms, device {dev:.1f} ms")
+    if phase == "sequential":
+        m.close()
+        t_seq = min(w for w, _ in seq_ms)
+        return {"value": upd / (t_seq * 1e-3), "sequential_ms_per_pass": t_seq,
+                "sequential_device_ms": min(d for _, d in seq_ms), "updates": upd}
     m.batch_set_patterns(pats)
     t = time.time()
     (cap, wb), est = m.batch_autotune(2)
     log(f"autotune: slab edges {cap}, walk batch {wb} (estimated {est:.1f} ms) in {time.time() - t:.1f} s")
     m.set_option(AC_OPT_GRAPH, 1)
-    for i in range(args.warmup):
+    for i in range(warmup):
         m.batch_run()
-        log(f"batched warmup {i + 1}/{args.warmup}: device {sum(m.batch_timing()):.1f} ms")
+        log(f"batched warmup {i + 1}/{warmup}: device {sum(m.batch_timing()):.1f} ms")
     bat_ms = []
-    for step in range(args.steps):
+    for step in range(steps):
         t0 = time.perf_counter()
         m.batch_run()
         wall = (time.perf_counter() - t0) * 1e3
         a, b = m.batch_timing()
         bat_ms.append((wall, a + b, a, b))
-        log(f"batched step {step + 1}/{args.steps}: wall {wall:.1f} ms, device {a + b:.1f} ms (prep {a:.1f}, "
+        log(f"batched step {step + 1}/{steps}: wall {wall:.1f} ms, device {a + b:.1f} ms (prep {a:.1f}, "
             f"loop {b:.1f})")
     info = m.batch_info()
     worst = 0.0
     bad = 0
-    for i in sample:
+    for i in (sample if phase != "batched" else []):
         nb, ratio = allclose_report(m.batch_mu(i), ref[i], 1e-5, 1e-8)
         bad += nb
         worst = max(worst, ratio)
     m.close()
+    if phase == "batched":
+        t_bat = min(w for w, *_ in bat_ms)
+        return {"value": upd / (t_bat * 1e-3), "ms_per_pass": t_bat, "batched_device_ms": min(d for _, d, *_ in bat_ms),
+                "updates": upd, "slabs": info["slabs"], "slab_edges": cap, "walk_batch": wb,
+                "updates_computed": info["updates_computed"]}
     t_seq = min(w for w, _ in seq_ms)
     t_bat = min(w for w, *_ in bat_ms)
-    out = {
-        "metric": "batched edge-pair updates/s (256 patterns, whole job) vs 256 sequential plain runs",
+    dev_seq = min(d for _, d in seq_ms)
+    dev_bat = min(d for _, d, *_ in bat_ms)
+    return {
         "value": upd / (t_bat * 1e-3),
-        "unit": "edge-pair updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": t_bat,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
         "vs_sequential_plain": t_seq / t_bat,
-        "sequential_ms_per_step": t_seq,
-        "sequential_device_ms": min(d for _, d in seq_ms),
-        "batched_device_ms": min(d for _, d, *_ in bat_ms),
+        "device_vs_sequential_plain": dev_seq / dev_bat,
+        "ms_per_pass": t_bat,
+        "sequential_ms_per_pass": t_seq,
+        "sequential_device_ms": dev_seq,
+        "batched_device_ms": dev_bat,
         "batched_prep_ms": min(a for *_, a, _ in bat_ms),
         "batched_loop_ms": min(b for *_, b in bat_ms),
-        "dtype": "f64",
-        "storage_dtype": "f32",
-        "data": "synthetic",
         "config": {"workload": "rmat20-d32-256-uneven-patterns", "data_vertices": g.num_nodes,
                    "data_edges": g.num_edges, "patterns": len(pats),
                    "sum_pattern_edges": int(sum(p.num_edges for p in pats)), "slab_edges": cap,
@@ -318,6 +360,29 @@ def bench_batched(args):
         "checks": {"sampled_patterns": len(sample), "violations_vs_sequential": int(bad),
                    "max_err_over_tol_vs_sequential": worst},
     }
+
+
+def bench_batched(args):
+    r = measure_batched(args.steps, args.warmup, args.phase)
+    if args.phase != "both":
+        print(json.dumps(dict(r, phase=args.phase, metric="batched workload, one side only", steps=args.steps,
+                              warmup=args.warmup)), flush=True)
+        return
+    out = {
+        "metric": "batched edge-pair updates/s (256 patterns, whole job) vs 256 sequential plain runs",
+        "value": r["value"],
+        "unit": "edge-pair updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_pass"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE,
+        "data": "synthetic",
+    }
+    out.update({k: v for k, v in r.items() if k not in ("value", "ms_per_pass")})
     print(json.dumps(out), flush=True)
 
 
@@ -328,6 +393,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="s22", choices=["s22", "s18", "small", "batched", "diff"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--phase", default="both", choices=["both", "sequential", "batched"],
+                    help="--config batched: time both sides (default) or one (for per-side hardware counters)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched and forward+backward measurements that the default line carries")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-pruned", action="store_true",
                     help="skip the comparison runs, pruned and with FP32 costs (profiled commands then hold the "
@@ -375,6 +444,7 @@ def main():
     w = W.CONFIGS[args.config](0)
     d, p = w.data, w.pattern
     dn, de, pn, pe = d.num_nodes, d.num_edges, p.num_nodes, p.num_edges
+    name, dim = w.name, int(d.vfeat.shape[1])
     log(f"workload {w.name}: dn={dn} de={de} pn={pn} pe={pe} generated in {time.time() - t:.1f} s")
 
     from application_classification_amd.matcher import AC_OPT_PACK_COSTS, AC_OPT_PRUNE
@@ -437,7 +507,7 @@ def main():
         avg_ms = stages[dom] / pn
         achieved = B[dom] / (avg_ms * 1e-3)
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(w.name, dom),
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(name, dom),
                     "alg_bytes_per_launch": B[dom], "avg_launch_ms": avg_ms}
         log(f"stages (ms, summed over {pn} iterations): {json.dumps({k: round(v, 4) for k, v in stages.items()})}")
 
@@ -484,13 +554,34 @@ def main():
         log(f"FP32 costs: loop {u_loop:.3f} ms, identical: {fp32_costs['bit_identical']}")
         m.set_option(AC_OPT_PACK_COSTS, 1)
 
+    m.close()
+    del m, mu, w, d, p  # the R-MAT 22 graphs leave the GPU and the host before the other measurements
+
+    # The metric's other two parts: batched against sequential plain runs, and forward+backward against forward
+    # (one warmup pass and one timed pass each; --config batched / diff time them with --steps / --warmup).
+    batched = fwd_bwd = None
+    if not args.no_extras:
+        log("batched: 256 uneven patterns on R-MAT 20 ...")
+        b = measure_batched(1, 1)
+        batched = {"vs_sequential_plain": b["vs_sequential_plain"],
+                   "device_vs_sequential_plain": b["device_vs_sequential_plain"], "value": b["value"],
+                   "ms_per_pass": b["ms_per_pass"], "sequential_ms_per_pass": b["sequential_ms_per_pass"],
+                   "slabs": b["config"]["slabs"], "slab_edges": b["config"]["slab_edges"],
+                   "max_err_over_tol_vs_sequential": b["checks"]["max_err_over_tol_vs_sequential"],
+                   "sampled_patterns": b["checks"]["sampled_patterns"]}
+        log(f"batched: {json.dumps(batched)}")
+        log("forward+backward: 64 patterns on R-MAT 18 ...")
+        f = measure_diff(1, 1)
+        fwd_bwd = {"ratio": f["ratio"], "forward_backward_ms": f["forward_backward_ms"], "forward_ms": f["forward_ms"],
+                   "patterns": f["config"]["patterns"], "checkpoint_every": 4, "grads_finite": f["checks"]["all_finite"]}
+        log(f"forward+backward: {json.dumps(fwd_bwd)}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"cpu baseline: FP64 restatement on {threads} threads ...")
+        log(f"cpu baseline: C++ restatement, FP64 and FP32 on {threads} threads, FP64 on 1 thread ...")
         cpu = cpu_baseline(threads)
-        log(f"cpu baseline: {cpu['value']:.3e} updates/s ({cpu['sample']})")
-        cpu = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        log(f"cpu baseline: {json.dumps(cpu)}")
 
     out = {
         "metric": "edge-pair updates/s per iteration + % HBM roofline",
@@ -505,12 +596,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "storage_dtype": "f32",
+        "dtype": DTYPE,
         "data": "synthetic",
-        "config": {"workload": w.name, "data_vertices": dn, "data_edges": de, "pattern_vertices": pn,
-                   "pattern_edges": pe, "feature_dim": int(d.vfeat.shape[1]), "iterations": pn,
-                   "parallelism": "single-gpu"},
+        "config": {"workload": name, "data_vertices": dn, "data_edges": de, "pattern_vertices": pn,
+                   "pattern_edges": pe, "feature_dim": dim, "iterations": pn, "parallelism": "single-gpu"},
         "hbm_roofline_frac_iteration": iter_frac,
         "alg_bytes_per_iteration": B["iteration"],
         "roofline": roofline,
@@ -518,10 +607,11 @@ def main():
         "fp32_costs": fp32_costs,
         "pruned": pruned,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "fwd_bwd": fwd_bwd,
         "stage_ms": stages,
         "checks": checks,
     }
-    m.close()
     print(json.dumps(out), flush=True)
 
 

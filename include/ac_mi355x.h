@@ -63,6 +63,28 @@ int ac_graph_load_csv(const char* vertex_csv, const char* edge_csv, ac_host_grap
 int ac_graph_save_csv(const char* vertex_csv, const char* edge_csv, const ac_host_graph* g);
 void ac_graph_free(ac_host_graph* g);
 
+/* The reference's Graph at its own precision (src/main.cu:8-19 with Int = uint64_t, Real = double,
+ * src/helpers.hxx:3-4): what loadGraph reads and prob2bin.py writes, nothing rounded. */
+typedef struct ac_graph64 {
+  uint64_t num_nodes;
+  uint64_t node_feat_dim;
+  uint64_t num_edges;
+  uint64_t edge_feat_dim;
+  double* node_feats;
+  double* edge_feats;
+  uint64_t* srcs;
+  uint64_t* dsts;
+} ac_graph64;
+/* Replaces loadGraph's file read (src/main.cu:21-47): the .bin as stored, validated (file length, ids < nV). */
+int ac_graph64_load_bin(const char* path, ac_graph64* out);
+/* Writes the .bin layout of prob2bin.py:46-64 (uint64 header, f64 features, uint64 srcs, uint64 dsts). */
+int ac_graph64_save_bin(const char* path, const ac_graph64* g);
+/* Reads the CSV pair as prob2bin.py:11-39 does, converting each decimal to the double pandas.read_csv produces by
+ * default (which is not always the correctly rounded one), so ac_graph64_save_bin then writes the bytes
+ * prob2bin.py writes. */
+int ac_graph64_load_csv(const char* vertex_csv, const char* edge_csv, ac_graph64* out);
+void ac_graph64_free(ac_graph64* g);
+
 /* -------------------------------------------------------------------------------------------------------------
  * Synthetic inputs (no counterpart in the reference, whose datasets are absent; these make its configurations).
  * All generators are deterministic functions of their seed and use every host core.
@@ -105,6 +127,9 @@ const char* ac_global_error(void);
 int ac_set_graph(ac_ctx* ctx, int which, int64_t num_nodes, int64_t node_feat_dim, const float* node_feats,
                  int64_t num_edges, int64_t edge_feat_dim, const float* edge_feats, const int32_t* srcs,
                  const int32_t* dsts);
+/* The same from the reference's own Graph (src/main.cu:49-62 uploads it unchanged): ids are checked and stored as
+ * int32, features are rounded to FP32 for the device (the matcher's storage precision). */
+int ac_set_graph64(ac_ctx* ctx, int which, const ac_graph64* g);
 
 /* Runs prep (src/main.cu:107-148) and n_iter iterations of the loop (src/main.cu:160-245); n_iter < 0 means the
  * number of pattern vertices, as src/main.cu:160.  Synchronous. */
@@ -119,6 +144,10 @@ int ac_get_mu(ac_ctx* ctx, float* host_out);
  * the first data vertex attaining the column maximum of MU[.][p], or 2 if additionally MU[v][p] >= log_tau;
  * 0 otherwise. */
 int ac_get_match(ac_ctx* ctx, float log_tau, uint8_t* host_out);
+/* Top-k data vertices per pattern vertex (no counterpart in the reference, which prints MU only): for each p, the k
+ * data vertices of largest MU[v][p], most likely first, ties by lower v: idx_out[p * k + r] (and, if val_out is not
+ * NULL, val_out[p * k + r] = MU[v][p]).  1 <= k <= dn. */
+int ac_get_topk(ac_ctx* ctx, int k, int32_t* idx_out, float* val_out);
 /* Sizes {dn, de, pn, pe}. */
 int ac_get_sizes(const ac_ctx* ctx, int64_t* out4);
 /* Device time of the last run, from HIP events: prep (src/main.cu:107-148) and loop (src/main.cu:160-245). */
@@ -160,6 +189,11 @@ int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms);
 #define AC_OPT_PACK_COSTS 7  /* 1 (default): the iteration's edge walk reads the edge costs of a bundle as 24-bit codes
                               * (bit pattern minus a per-bundle exponent base) when all of them lie within two
                               * binades, losslessly, so results are bit-identical; 0: FP32 costs throughout        */
+#define AC_OPT_CNULL 8        /* 1: the PNNL Cnull term, which the reference leaves out (src/main.cu:83, test.py:97):
+                              * every iteration's fills become VRmax - cnull (FMax) and VFmax - cnull (RMax), with
+                              * cnull the log-softmax over the pattern's edges of max(||edge features||_2,
+                              * max_e normprob(CE)) (test.py:93-95,148-149); 0 (default): left out, as the reference.
+                              * Not differentiated: runs with AC_OPT_TAPE > 0 reject it                            */
 int ac_set_option(ac_ctx* ctx, int option, int value);
 int ac_get_option(const ac_ctx* ctx, int option, int* value);
 

@@ -14,6 +14,9 @@
 //  * MU accumulates in ascending pattern-edge order, the FMax term before the RMax term of each edge; the
 //    reference uses atomicAdd in block-schedule order (src/ac.hxx:164-170).
 //  * every reduction is blocked in a fixed order, so results do not depend on the thread count.
+//  * the PNNL Cnull term is off, as in src/main.cu:83 ("Ignoring for now") and test.py:97 (cnull = 0).  OR_CNULL
+//    applies it as test.py:93-95 and :148-149 state it: cnull = normprob_q(max(||pattern edge features||_2,
+//    max_e normprob_e(CE))), and every iteration's fills become VRmax - cnull (FMax) and VFmax - cnull (RMax).
 //
 // Parity status of each function (what pins it) is stated beside it and checked by tests/test_oracle_pins.py.
 #include <omp.h>
@@ -30,6 +33,7 @@ extern "C" {
 
 #define OR_COMPAT_ZERO_TAIL 1u
 #define OR_LITERAL_LSM 2u
+#define OR_CNULL 4u
 
 typedef struct or_graph {
   int64_t nV, dV, nE, dE;
@@ -186,6 +190,23 @@ int run(const or_graph* D, const or_graph* P, int n_iter, unsigned flags, double
   row_log_softmax<R>(pe, de, CE.data(), literal);
   row_log_softmax<R>(pe, de, RE.data(), literal);
   row_log_softmax<R>(pe, de, FE.data(), literal);
+  // Cnull (test.py:93-95, with ce = normprob(ce) of test.py:84): per pattern edge the larger of its feature vector's
+  // norm and the largest normalised edge cost, then normprob (test.py:55-57) over the pattern edges.
+  std::vector<R> cnull(pe, R(0));
+  if (flags & OR_CNULL) {
+    std::vector<double> a(pe);
+    for (int64_t q = 0; q < pe; ++q) {
+      double ss = 0.0;
+      for (int64_t k = 0; k < P->dE; ++k) ss += P->efeat[q * P->dE + k] * P->efeat[q * P->dE + k];
+      const double mx = de ? (double)row_max(CE.data() + q * de, de) : -std::numeric_limits<double>::infinity();
+      a[q] = std::max(std::sqrt(ss), mx);
+    }
+    double m = -std::numeric_limits<double>::infinity(), sum = 0.0;
+    for (double x : a) m = std::max(m, x);
+    for (double x : a) sum += std::exp(x - m);
+    for (int64_t q = 0; q < pe; ++q) cnull[q] = (R)std::log(std::exp(a[q] - m) / sum);
+  }
+  std::vector<R> fill(pe);
   // init_VX (src/main.cu:122-133)
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < pe * dn; ++i) {
@@ -216,13 +237,17 @@ int run(const or_graph* D, const or_graph* P, int n_iter, unsigned flags, double
       RE[i] = VF[dn * q + D->src[e]] - CE[i];
     }
     // step3: RowMax2(VF), RowSoftmax2(RE), EdgeMaxReduce2_t by source (src/main.cu:197-211)
+    // (with OR_CNULL the fill is VFmax - cnull, test.py:149)
     row_max_all<R>(pe, dn, VF.data(), VFmax.data());
     row_log_softmax<R>(pe, de, RE.data(), literal);
-    edge_max_reduce<R>(de, dn, pe, VFmax.data(), RE.data(), RMax.data(), D->src);
-    // step4: the same for VR / FE by destination (src/main.cu:216-224)
+    for (int64_t q = 0; q < pe; ++q) fill[q] = VFmax[q] - cnull[q];
+    edge_max_reduce<R>(de, dn, pe, fill.data(), RE.data(), RMax.data(), D->src);
+    // step4: the same for VR / FE by destination (src/main.cu:216-224; with OR_CNULL the fill is VRmax - cnull,
+    // test.py:148)
     row_max_all<R>(pe, dn, VR.data(), VRmax.data());
     row_log_softmax<R>(pe, de, FE.data(), literal);
-    edge_max_reduce<R>(de, dn, pe, VRmax.data(), FE.data(), FMax.data(), D->dst);
+    for (int64_t q = 0; q < pe; ++q) fill[q] = VRmax[q] - cnull[q];
+    edge_max_reduce<R>(de, dn, pe, fill.data(), FE.data(), FMax.data(), D->dst);
     // step5: ComputeMU2_t + RowSoftmax2(MU) (src/main.cu:229-241)
     compute_mu<R>(dn, pe, dn, pn, CV.data(), FMax.data(), RMax.data(), P->src, P->dst, MU.data());
     row_log_softmax<R>(pn, dn, MU.data(), literal);

@@ -21,6 +21,7 @@ SOURCE = os.path.join(HERE, "ac_oracle.cpp")
 
 OR_COMPAT_ZERO_TAIL = 1
 OR_LITERAL_LSM = 2
+OR_CNULL = 4
 
 
 def build(verbose: bool = False) -> str:
@@ -144,13 +145,15 @@ def run_weighted(data: Graph64, patt: Graph64, wv=None, we=None, n_iter: int = -
     return mu
 
 
-def timed_run(data: Graph64, patt: Graph64, n_iter: int = -1, threads: int = 0):
-    """Runs the restatement as the timed CPU baseline; returns (prep_seconds, loop_seconds)."""
+def timed_run(data: Graph64, patt: Graph64, n_iter: int = -1, threads: int = 0, precision: str = "f64"):
+    """Runs the restatement as the timed CPU baseline (Real = double for "f64", float for "f32"); returns
+    (prep_seconds, loop_seconds, MU)."""
     dn, pn = data.vfeat.shape[0], patt.vfeat.shape[0]
     mu = np.empty((dn, pn), dtype=np.float64)
     el = (ctypes.c_double * 2)()
     D, Pg = data.c(), patt.c()
-    rc = lib().or_run_f64(ctypes.byref(D), ctypes.byref(Pg), n_iter, 0, threads, mu.ctypes.data, None, None, el)
+    fn = lib().or_run_f64 if precision == "f64" else lib().or_run_f32
+    rc = fn(ctypes.byref(D), ctypes.byref(Pg), n_iter, 0, threads, mu.ctypes.data, None, None, el)
     if rc != 0:
         raise ValueError("oracle rejected the inputs")
     return el[0], el[1], mu

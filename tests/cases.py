@@ -62,3 +62,28 @@ def tiny_hand(seed: int = 11) -> Case:
 def rmat_case(scale: int, dim: int, n_pv: int, n_pe: int, seed: int = 0) -> Case:
     w = W.rmat(scale, dim, n_pv, n_pe, seed)
     return Case(w.name, w.data, w.pattern)
+
+
+def georgiy64(seed: int = 0) -> Case:
+    """The georgiy case at the reference's own precision: the sample CSVs read as prob2bin.py reads them (FP64,
+    pandas' decimal conversion, through ac_graph64_load_csv) and the data edges' features drawn in FP64 with the
+    structure georgiy() gives them; nothing is rounded to FP32.  Arrays: features float64, ids int64."""
+    from application_classification_amd import matcher
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as tmp:
+        no_edges = os.path.join(tmp, "none.csv")
+        open(no_edges, "w").write("0 2\n")
+        dv = matcher.load_csv64(os.path.join(REF_DATA, "georgiyData.Vertex.csv"), no_edges)
+    pg = matcher.load_csv64(os.path.join(REF_DATA, "georgiyPattern.Vertex.csv"),
+                            os.path.join(REF_DATA, "georgiyPattern.Edges.csv"))
+    src, dst = W.er_edges(dv.num_nodes, 20000, seed)
+    r = np.random.default_rng(seed)
+    ef = np.empty((len(src), 17), np.float64)
+    ef[:, :10] = r.uniform(0.0, 10.0, size=(len(src), 10))
+    ef[:, 10] = 1.0
+    ef[:, 11::2] = dv.vfeat[src][:, -3:]
+    ef[:, 12::2] = dv.vfeat[dst][:, -3:]
+    data = W.Graph(dv.vfeat, ef, src.astype(np.int64), dst.astype(np.int64))
+    patt = W.Graph(pg.vfeat, pg.efeat, pg.src.astype(np.int64), pg.dst.astype(np.int64))
+    return Case("georgiy64", data, patt)

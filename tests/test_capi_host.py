@@ -165,3 +165,103 @@ def test_cli_usage():
     assert os.access(exe, os.X_OK)
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+# --- the reference's Graph at full precision (ac_graph64) and prob2bin.py's conversion -----------------------------
+
+def prob2bin_bytes(vertex_csv, edge_csv):
+    """The bytes prob2bin.py writes for one CSV pair, by its own steps with pandas (prob2bin.py:11-12 read, :28-29
+    id check, :33-39 column split, :46-64 layout) -- pandas is the independent parser the C++ reader is checked
+    against."""
+    import pandas as pd
+
+    def read_csv(p):
+        return pd.read_csv(p, skiprows=1, sep=" ", header=None)
+
+    v, e = read_csv(vertex_csv), read_csv(edge_csv)
+    assert (v[0] == v.index).all()
+    vt = v.values[:, 1:]
+    et = e[list(range(2, e.shape[1]))].values
+    ee = e[[0, 1]].values
+    parts = [np.hstack([vt.shape, et.shape]).astype(np.uint64), vt.ravel().astype(np.float64),
+             et.ravel().astype(np.float64), ee[:, 0].ravel().astype(np.uint64), ee[:, 1].ravel().astype(np.uint64)]
+    return b"".join(bytes(bytearray(x)) for x in parts)
+
+
+def _write_edge_csv(path, n_vertices, n_edges, dim, seed):
+    """A data-edge CSV in the reference's text form (header "N C", CRLF, values as Python's shortest repr), with
+    magnitudes from 1e-30 to 1e30, negatives, integers and exponent forms."""
+    r = np.random.default_rng(seed)
+    src = r.integers(0, n_vertices, n_edges)
+    dst = r.integers(0, n_vertices, n_edges)
+    vals = r.normal(size=(n_edges, dim)) * 10.0 ** r.integers(-30, 30, size=(n_edges, dim))
+    vals[:, 0] = r.uniform(0, 10, n_edges)
+    vals[:, 1] = np.round(vals[:, 1] * 1e3)  # integer-valued floats
+    lines = [f"{n_edges} {dim + 2}"]
+    for i in range(n_edges):
+        lines.append(" ".join([str(src[i]), str(dst[i])] + [repr(float(x)) for x in vals[i]]))
+    open(path, "w", newline="").write("\r\n".join(lines) + "\r\n")
+
+
+def test_csv_to_bin_is_byte_identical_to_prob2bin(tmp_path):
+    exe = os.path.join(ROOT, "application_classification_amd", "ac_main")
+    ref = os.path.join(cases.REF_DATA, "georgiyPattern")
+    e_csv = str(tmp_path / "dataE.csv")
+    _write_edge_csv(e_csv, 1000, 3000, 17, seed=3)
+    pairs = [(ref + ".Vertex.csv", ref + ".Edges.csv"),
+             (os.path.join(cases.REF_DATA, "georgiyData.Vertex.csv"), e_csv)]
+    for i, (v, e) in enumerate(pairs):
+        expect = prob2bin_bytes(v, e)
+        out = str(tmp_path / f"g{i}.bin")
+        matcher.save_bin64(out, matcher.load_csv64(v, e))
+        assert open(out, "rb").read() == expect
+        out2 = str(tmp_path / f"cli{i}.bin")
+        r = subprocess.run([exe, "--to-bin", v, e, out2], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        assert open(out2, "rb").read() == expect
+        # pandas' converter is not the correctly rounded one; this file differs from strtod somewhere
+        toks = [float(t) for line in open(v).read().splitlines()[1:] for t in line.split()]
+        g = matcher.load_csv64(v, e)
+        assert (np.array(toks).reshape(g.num_nodes, -1)[:, 1:] != g.vfeat).any() or i == 1
+
+
+def test_bin64_round_trip_and_validation(tmp_path):
+    ref = os.path.join(cases.REF_DATA, "georgiyPattern")
+    g = matcher.load_csv64(ref + ".Vertex.csv", ref + ".Edges.csv")
+    assert g.vfeat.dtype == np.float64 and g.src.dtype == np.uint64
+    path = str(tmp_path / "p.bin")
+    matcher.save_bin64(path, g)
+    h = matcher.load_bin64(path)
+    for a, b in ((g.vfeat, h.vfeat), (g.efeat, h.efeat), (g.src, h.src), (g.dst, h.dst)):
+        np.testing.assert_array_equal(a, b)
+    # the FP32 reader is the FP64 one rounded
+    f = matcher.load_bin(path)
+    np.testing.assert_array_equal(f.vfeat, g.vfeat.astype(np.float32))
+    np.testing.assert_array_equal(f.efeat, g.efeat.astype(np.float32))
+    c = matcher.load_csv(ref + ".Vertex.csv", ref + ".Edges.csv")
+    np.testing.assert_array_equal(c.efeat, g.efeat.astype(np.float32))
+    raw = open(path, "rb").read()
+    open(path, "wb").write(raw[:-1])
+    with pytest.raises(_lib.ACError) as e:
+        matcher.load_bin64(path)
+    assert e.value.code == _lib.AC_ERR_IO
+    bad = bytearray(raw)
+    bad[-8:] = np.array([10], np.uint64).tobytes()  # last destination = num_nodes
+    open(path, "wb").write(bytes(bad))
+    with pytest.raises(_lib.ACError) as e:
+        matcher.load_bin64(path)
+    assert e.value.code == _lib.AC_ERR_IO
+
+
+@pytest.mark.parametrize("tok", ["0.1", "-2.5e-3", "1E5", "+3.25", ".5", "5.", "123456789012345678901234",
+                                 "0.000000000000000000001234567890123456789", "1e-320", "-0.0", "7.0e+02",
+                                 "9.999999999999999e22", "2.2250738585072014e-308"])
+def test_decimal_conversion_matches_pandas(tmp_path, tok):
+    import pandas as pd
+
+    v, e = str(tmp_path / "v.csv"), str(tmp_path / "e.csv")
+    open(v, "w").write(f"1 2\n0 {tok}\n")
+    open(e, "w").write("0 2\n")
+    g = matcher.load_csv64(v, e)
+    want = pd.read_csv(v, skiprows=1, sep=" ", header=None).values[0, 1]
+    assert np.float64(want).tobytes() == np.float64(g.vfeat[0, 0]).tobytes(), (tok, want, g.vfeat[0, 0])

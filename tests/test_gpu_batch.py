@@ -214,3 +214,25 @@ def test_batch_errors(small, uneven):
         with pytest.raises(ACError) as e:
             m.batch_mu(0)
         assert e.value.code == AC_ERR_STATE
+
+
+def test_batch_cnull_normalised_per_pattern(small, uneven):
+    """With AC_OPT_CNULL each pattern of a slab gets the Cnull of its own edges (the log-softmax runs over one
+    pattern's edges, not over the slab), so batched results equal plain runs with the option set."""
+    from application_classification_amd.matcher import AC_OPT_CNULL
+
+    d = W.Graph(small.data.vfeat, small.data.efeat * 2.0, small.data.src, small.data.dst)
+    pats = [W.Graph(p.vfeat, p.efeat * 2.0, p.src, p.dst) for p in uneven[:5]]
+    with Matcher(0) as m:
+        m.set_option(AC_OPT_CNULL, 1)
+        m.set_option(AC_OPT_SLAB_EDGES, 256)
+        m.set_data(d)
+        m.batch_set_patterns(pats)
+        m.batch_run()
+        assert m.batch_info()["slabs"] < len(pats)
+        for i, p in enumerate(pats):
+            with Matcher(0) as m1:
+                m1.set_option(AC_OPT_CNULL, 1)
+                m1.set_data(d)
+                m1.set_pattern(p)
+                assert_close(m.batch_mu(i), m1.run(), f"Cnull pattern {i}")

@@ -259,21 +259,49 @@ def test_stage_row_max_and_edge_max_reduce_bit_exact():
     np.testing.assert_array_equal(rm.cpu().numpy(), xe.max(axis=1))
 
 
-def test_stage_row_log_softmax():
-    r = np.random.default_rng(2)
-    x = (r.normal(size=(6, 5000)) * 20).astype(np.float32)
-    x[3, 17] = 60.0  # one dominant entry: its log-probability is tiny and must keep relative accuracy
-    d = dev(x)
-    ac.RowSoftmax2(6, 5000, d)
+@pytest.mark.parametrize("n_row,n_col", [(24, 2_000_001), (3, 300_000), (70_000, 3), (2, 1)])
+def test_stage_row_max_chunked(n_row, n_col):
+    """Rows split over many blocks (each folding its chunk maximum in with one exact atomic), rows of one or three
+    elements, all-negative rows and a row whose maximum is zero (a -0.0 / +0.0 pair)."""
+    r = np.random.default_rng(n_row)
+    x = r.normal(size=(n_row, n_col)).astype(np.float32)
+    x[0] = -np.abs(x[0]) - 1.0
+    if n_row > 1 and n_col > 1:
+        x[1] = -np.abs(x[1]) - 1.0
+        x[1, -1] = 0.0
+        x[1, 0] = -0.0
+    out = torch.empty(n_row, dtype=torch.float32, device="cuda")
+    ac.RowMax2(n_row, n_col, dev(x), out)
     torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    np.testing.assert_array_equal(got, x.max(axis=1))  # -0.0 == +0.0: the sign of a zero maximum is unspecified
+
+
+@pytest.mark.parametrize("n_row,n_col", [(6, 5000), (24, 2_000_001), (3, 300_000), (70_000, 3), (1, 1)])
+def test_stage_row_log_softmax(n_row, n_col):
+    """Single-chunk rows, rows over hundreds of blocks (partials merged in a fixed order), more rows than the
+    partial budget (one chunk each), and the one-element row (log-probability 0)."""
+    r = np.random.default_rng(2)
+    x = (r.normal(size=(n_row, n_col)) * 20).astype(np.float32)
+    if n_col > 17:
+        x[n_row // 2, 17] = 200.0  # one dominant entry: its log-probability is tiny and must keep relative accuracy
+    d = dev(x)
+    ac.RowSoftmax2(n_row, n_col, d)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
     ref = O.row_log_softmax(x.astype(np.float64))
-    bad, ratio = allclose_report(d.cpu().numpy(), ref)
+    bad, ratio = allclose_report(got, ref)
+    say(f"row log-softmax {n_row} x {n_col}: max err/tol {ratio:.4f}")
     assert bad == 0, ratio
+    d2 = dev(x)
+    ac.RowSoftmax2(n_row, n_col, d2)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d2.cpu().numpy(), got)  # fixed merge order: reproducible
 
 
-def test_stage_compute_mu_and_transpose():
+@pytest.mark.parametrize("pn,pe,dn", [(5, 9, 1000), (16, 24, 300_001)])
+def test_stage_compute_mu_and_transpose(pn, pe, dn):
     r = np.random.default_rng(3)
-    pn, pe, dn = 5, 9, 1000
     cv = r.normal(size=(pn, dn)).astype(np.float32)
     fm = r.normal(size=(pe, dn)).astype(np.float32)
     rm = r.normal(size=(pe, dn)).astype(np.float32)
@@ -526,3 +554,79 @@ def test_headline_workload_rmat22_d64_p16x24():
             else:
                 say(f"{name}: bit-identical = {np.array_equal(mu, base)}")
                 np.testing.assert_array_equal(mu, base)
+
+
+# --- the PNNL Cnull term (AC_OPT_CNULL; off by default, as in the reference) ---------------------------------------
+
+def _scaled_edges(g, s):
+    return W.Graph(g.vfeat, (g.efeat * s).astype(np.float32), g.src, g.dst)
+
+
+@pytest.mark.parametrize("case", ["small_x2", "rmat12"])
+def test_cnull_against_oracle(small, case):
+    """AC_OPT_CNULL against the oracle's OR_CNULL (pinned to test.py:93-95,148-149 by tests/test_oracle_pins.py), with
+    pruning on and off (bit-identical: the raised fill only tightens the bound), and the option's default leaving the
+    reference's result unchanged.  small_x2 doubles the edge features so that the raised fills win at some vertices
+    and the term visibly changes MU; on rmat12 (vertices without in- or out-edges) the fills win at every vertex, so
+    the raised fill is a uniform shift that the normalisation removes, and parity is what is checked."""
+    from application_classification_amd.matcher import AC_OPT_CNULL, AC_OPT_PRUNE
+
+    if case == "small_x2":
+        d, p = _scaled_edges(small.data, 2.0), _scaled_edges(small.pattern, 2.0)
+    else:
+        c = cases.rmat_case(12, 16, 8, 12)
+        d, p = c.data, c.pattern
+    out = {}
+    for cn, prune in ((0, 1), (1, 1), (1, 0)):
+        with Matcher(0) as m:
+            m.set_option(AC_OPT_PRUNE, prune)
+            if cn:
+                m.set_option(AC_OPT_CNULL, 1)
+                assert m.get_option(AC_OPT_CNULL) == 1
+            m.set_data(d)
+            m.set_pattern(p)
+            out[(cn, prune)] = m.run()
+    ref = O.run(g64(d), g64(p), flags=O.OR_CNULL)
+    assert_parity(out[(1, 1)], ref, f"Cnull {case}")
+    np.testing.assert_array_equal(out[(1, 0)], out[(1, 1)])
+    if case == "small_x2":
+        assert np.abs(out[(1, 1)] - out[(0, 1)]).max() > 1e-3  # the term takes effect
+    assert_parity(out[(0, 1)], O.run(g64(d), g64(p)), f"no Cnull {case}")
+
+
+def test_cnull_rejected_with_the_adjoint_tape(small):
+    from application_classification_amd.matcher import AC_OPT_CNULL, AC_OPT_TAPE
+
+    with Matcher(0) as m:
+        m.set_option(AC_OPT_CNULL, 1)
+        m.set_option(AC_OPT_TAPE, 4)
+        m.set_data(small.data)
+        m.set_pattern(small.pattern)
+        with pytest.raises(ACError) as e:
+            m.run()
+        assert e.value.code == AC_ERR_ARG
+
+
+def test_out_of_memory_is_reported(small, monkeypatch):
+    """A device allocation the context cannot make is reported as AC_ERR_OOM with a message, not a crash: forced from
+    the host by capping single allocations (AC_MAX_ALLOC_BYTES), first at upload (the data graph's features), then at
+    the first run (the work buffers).  With the cap lifted the same context runs and gives the uncapped result."""
+    from application_classification_amd._lib import AC_ERR_OOM
+
+    d, p = small.data, small.pattern
+    ref = gpu_mu(d, p)
+    with Matcher(0) as m:
+        monkeypatch.setenv("AC_MAX_ALLOC_BYTES", "65536")
+        with pytest.raises(ACError) as e:
+            m.set_data(d)
+        assert e.value.code == AC_ERR_OOM and "upload" in str(e.value)
+        monkeypatch.delenv("AC_MAX_ALLOC_BYTES")
+        m.set_data(d)
+        m.set_pattern(p)
+        monkeypatch.setenv("AC_MAX_ALLOC_BYTES", str(256 * 1024))
+        with pytest.raises(ACError) as e:
+            m.run()
+        assert e.value.code == AC_ERR_OOM
+        say(f"out of memory reported: {e.value}")
+        monkeypatch.delenv("AC_MAX_ALLOC_BYTES")
+        np.testing.assert_array_equal(m.run(), ref)

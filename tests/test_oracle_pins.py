@@ -131,6 +131,34 @@ def test_oracle_agrees_with_reference_cpu_path_restatement(small):
     np.testing.assert_allclose(mu, ref, rtol=1e-9, atol=1e-12)
 
 
+def test_cnull_term_matches_reference_cpu_path_restatement(small):
+    """OR_CNULL: the PNNL Cnull term as test.py:93-95 / :148-149 state it (commented out there), against the
+    vectorised restatement of test.py with the same lines enabled.  The edge features are doubled so that the raised
+    fills win at some vertices (on the unscaled graph every vertex's edge terms stay above them and the term changes
+    nothing); every vertex keeps in- and out-edges, where src/main.cu and test.py agree.  The prep state (zero
+    iterations) is untouched by the term."""
+    d, p = small.data, small.pattern
+    ef, pf = d.efeat.astype(float) * 2, p.efeat.astype(float) * 2
+    G, Pg = O.Graph64.of(d.vfeat, ef, d.src, d.dst), O.Graph64.of(p.vfeat, pf, p.src, p.dst)
+    args = (d.vfeat.astype(float), ef, d.src, d.dst, p.vfeat.astype(float), pf, p.src, p.dst)
+    for n_iter in (1, 3, None):
+        mu = O.run(G, Pg, n_iter=-1 if n_iter is None else n_iter, flags=O.OR_CNULL)
+        ref = testpy_numpy.run(*args, n_iter=n_iter, cnull=True)
+        np.testing.assert_allclose(mu, ref, rtol=1e-9, atol=1e-12)
+    assert np.abs(O.run(G, Pg, n_iter=2, flags=O.OR_CNULL) - O.run(G, Pg, n_iter=2)).max() > 1.0
+    np.testing.assert_array_equal(O.run(G, Pg, n_iter=0, flags=O.OR_CNULL), O.run(G, Pg, n_iter=0))
+
+
+def test_cnull_is_the_log_softmax_of_pattern_edge_norms():
+    """max_e normprob(ce) <= 0 <= ||features||, so the Cnull of test.py:93-95 is the log-softmax over pattern edges of
+    the feature norms: running the oracle with that vector as the fill offset (via the numpy restatement) agrees."""
+    r = np.random.default_rng(5)
+    pe = r.normal(size=(7, 4))
+    ce = log_softmax(r.normal(size=(50, 7)) * 3, axis=0)
+    literal = testpy_numpy.normprob(np.maximum(np.sqrt((pe ** 2).sum(axis=-1)), ce.max(axis=0)))
+    np.testing.assert_array_equal(literal, testpy_numpy.normprob(np.sqrt((pe ** 2).sum(axis=-1))))
+
+
 def test_oracle_prep_differs_from_test_py_on_zero_degree_vertices():
     """Where a vertex has no in-edge (out-edge), src/main.cu fills FMax (RMax) with VRmax (VFmax), src/ac.hxx:124-126,
     while test.py leaves 0 (test.py:100-116).  The oracle follows src/main.cu.  The fill only shows when the edge
