@@ -127,6 +127,7 @@ SIGNATURES = {
     "ac_batch_autotune": (I, [P, I, P, P]),
     "ac_cdist": (I, [I64, I64, I64, P, P, P, P]),
     "ac_row_log_softmax": (I, [I64, I64, P, P]),
+    "ac_row_log_softmax_static": (I, [I64, I64, P, P]),
     "ac_row_max": (I, [I64, I64, P, P, P]),
     "ac_edge_max_reduce": (I, [I64, I64, I64, P, P, P, P, P]),
     "ac_compute_mu": (I, [I64, I64, I64, I64, P, P, P, P, P, P, P]),

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/ac_mi355x.h"
@@ -59,12 +60,12 @@ __global__ void k_stage_cdist(int64_t n_a, int64_t n_b, int64_t dim, const float
 //   k_lsm_rows  : one block per row merges the row's partials in a fixed order into fin[row] = (m, log1p(t));
 //   k_lsm_apply : block (row, chunk) writes its chunk normalised.
 // The chunking depends only on (n_row, n_col), so the merge order, and with it every result, is the same from call to
-// call.
+// call and for the allocating and the static-scratch entry points.
 
 constexpr int kLsmBlock = 256;
 constexpr int64_t kLsmChunkMin = 8192;  // elements per chunk, at least (32 per thread)
-// Scratch of one call, in ac::Lse entries: n_row * nchunk partials, then n_row finals; the chunking keeps it within
-// this many entries (1 MiB) whenever there are fewer rows than that.
+// Scratch of one call, in ac::Lse entries: n_row * nchunk partials, then n_row finals.  The static scratch of
+// ac_row_log_softmax_static holds this many (1 MiB), and both entry points size their chunking to fit it.
 constexpr int64_t kLsmScratch = 1 << 16;
 
 int64_t lsm_chunks(int64_t n_row, int64_t n_col) {
@@ -143,6 +144,37 @@ int enqueue_row_lsm(int64_t n_row, int64_t n_col, float* x, ac::Lse* scratch, hi
   hipLaunchKernelGGL(k_lsm_rows, dim3((unsigned)n_row), dim3(kLsmBlock), 0, st, nchunk, scratch, fin);
   hipLaunchKernelGGL(k_lsm_apply, dim3((unsigned)nblk), dim3(kLsmBlock), 0, st, n_col, nchunk, x, fin);
   return launch_status(who);
+}
+
+// Scratch of ac_row_log_softmax_static: one copy per GPU, held until the process ends.  Host code reaches it only
+// through the address hipGetSymbolAddress returns on each device; the array's name itself is a host-side stand-in.
+static __device__ __align__(256) ac::Lse scratch[kLsmScratch];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+ac::Lse* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch's address on the current device, looked up on the first call there (with that device current).
+int static_scratch(ac::Lse** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    ac::set_global_error(std::string("ac_row_log_softmax_static: hipGetDevice: ") + hipGetErrorString(e));
+    return AC_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) return bad("ac_row_log_softmax_static: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      ac::set_global_error(std::string("ac_row_log_softmax_static: hipGetSymbolAddress: ") + hipGetErrorString(e));
+      return AC_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<ac::Lse*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return AC_OK;
 }
 
 // Float max as integer atomics: non-negative floats order like signed ints, negative floats order inversely to
@@ -260,6 +292,17 @@ extern "C" int ac_row_log_softmax(int64_t n_row, int64_t n_col, float* x, void* 
     return AC_ERR_HIP;
   }
   return AC_OK;
+}
+
+extern "C" int ac_row_log_softmax_static(int64_t n_row, int64_t n_col, float* x, void* stream) {
+  if (n_row < 0 || n_col < 0 || (n_row * n_col > 0 && !x)) return bad("ac_row_log_softmax_static: bad arguments");
+  if (n_row > kLsmScratch / 2)
+    return bad("ac_row_log_softmax_static: more than 32768 rows (its fixed scratch holds 65536 partials)");
+  if (n_row * n_col == 0) return AC_OK;
+  ac::Lse* scr = nullptr;
+  const int rc = static_scratch(&scr);
+  if (rc != AC_OK) return rc;
+  return enqueue_row_lsm(n_row, n_col, x, scr, as_stream(stream), "ac_row_log_softmax_static");
 }
 
 extern "C" int ac_row_max(int64_t n_row, int64_t n_col, const float* in, float* out, void* stream) {

@@ -250,6 +250,12 @@ int ac_cdist(int64_t n_a, int64_t n_b, int64_t dim, const float* feats_a, const 
 /* ac::RowSoftmax2 (src/ac.hxx:77-113): in-place log-softmax of each row of an n_row x n_col matrix, computed
  * max-stabilised: x - max - log1p(sum_{k != argmax} exp(x_k - max)). */
 int ac_row_log_softmax(int64_t n_row, int64_t n_col, float* x, void* stream);
+/* No counterpart in the reference: ac_row_log_softmax for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The same kernels and results (bit for bit), with
+ * the partials in a fixed device-resident scratch of 1 MiB per GPU instead of a stream-ordered allocation; it calls
+ * no allocating routine.  At most 32768 rows (AC_ERR_ARG beyond).  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other streams or threads) race and must be ordered by the caller. */
+int ac_row_log_softmax_static(int64_t n_row, int64_t n_col, float* x, void* stream);
 /* ac::RowMax2 (src/ac.hxx:58-75): out[r] = max_c in[r * n_col + c]. */
 int ac_row_max(int64_t n_row, int64_t n_col, const float* in, float* out, void* stream);
 /* ac::EdgeMaxReduce2_t (src/ac.hxx:115-149): xmax[r * n_col_out + v] = max(vymax[r],
