@@ -21,6 +21,7 @@
 
 #include "ac_internal.hpp"
 #include "ac_lse.hpp"
+#include "ac_sqrt_rn.hpp"
 
 namespace ac {
 
@@ -141,26 +142,47 @@ __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int
 // accuracy (relative error of d^2 below 1e-11 elsewhere).
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-template <int D4, bool kIdx>
-__global__ __launch_bounds__(256) void k_cdist_mfma(int64_t nrows, const int* __restrict__ idx,
+// LDS position of pattern operand e = c * 4 D4 + d (pattern column c = 16 pt + i, feature d = D4 kk + s2): the
+// layout [pt][s2][kk][i], so that product step s2 of tile pt reads 64 consecutive doubles across the wave (lane
+// 16 kk + i), one conflict-free LDS access.  In the layout of patm itself the 64 lanes' operands lie 128 or 512 bytes
+// apart, all in one bank pair, and each access is serialised.
+__device__ __forceinline__ int lds_operand(int e, int D4) {
+  const int c = e / (4 * D4), d = e - c * (4 * D4);
+  const int pt = c >> 4, i = c & 15, kk = d / D4, s2 = d - kk * D4;
+  return ((pt * D4 + s2) * 4 + kk) * 16 + i;
+}
+
+// Correctly rounded square root of x >= 0 in FP32 (x >= 2^-104): the hardware root corrected by sqrt_rn_fix, eight
+// VALU instructions against the library's general sequence.
+__device__ __forceinline__ float sqrt_rn(float x) { return sqrt_rn_fix(x, __builtin_amdgcn_sqrtf(x)); }
+
+// Up to 64 features the kernel fits three waves per SIMD (at most 168 registers, no spills), which keeps the matrix
+// core busy while other waves run their epilogues and loads; wider rows keep the whole register file.  kW: weighted
+// metric (a template parameter, so the unweighted loop carries no per-feature select).
+template <int D4, bool kIdx, bool kW>
+__global__ __launch_bounds__(256, D4 <= 16 ? 3 : 1) void k_cdist_mfma(int64_t nrows, const int* __restrict__ idx,
                                                     const float* __restrict__ feats, int dim,
                                                     const double* __restrict__ patm, const double* __restrict__ pp,
                                                     const double* __restrict__ w, const double* __restrict__ pat,
                                                     int K, int Kstore, float* __restrict__ out, int out_stride) {
-  // LDS: [npt * 16][4 D4] pattern operands (the block's copy of patm), [4 D4] weights, [npt * 16] pattern norms.
+  // LDS: npt * 16 * 4 D4 pattern operands (the block's copy of patm, laid out by lds_operand), [4 D4] weights,
+  // [npt * 16] pattern norms.
   extern __shared__ double bsh[];
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
   const int64_t nw = (int64_t)gridDim.x * 4, step = nw * 16;
   const int npt = (Kstore + 15) / 16;
   double* wsh = bsh + npt * 16 * 4 * D4;
   double* psh = wsh + 4 * D4;
-  for (int e = threadIdx.x; e < npt * 16 * 4 * D4; e += 256) bsh[e] = patm[e];
-  if (w)
+  for (int e = threadIdx.x; e < npt * 16 * 4 * D4; e += 256) bsh[lds_operand(e, D4)] = patm[e];
+  if (kW)
     for (int d = threadIdx.x; d < 4 * D4; d += 256) wsh[d] = d < dim ? w[d] : 0.0;
   for (int c = threadIdx.x; c < npt * 16; c += 256) psh[c] = pp[c];
   __syncthreads();
-  // Whole 16-float chunks load as float4s when the row width is exactly 4 D4 (e.g. 32 or 64 features).
+  // Whole 16-float chunks load as float4s when the row width is exactly 4 D4 (e.g. 32 or 64 features); this lane's
+  // chunk then has no feature past dim.  Otherwise the features past dim are zeroed at conversion (lim of them
+  // exist).  Rows that do not exist are not zeroed: their loads read row 0 and their outputs are +inf regardless.
   const bool vec = dim == 4 * D4 && (D4 % 4) == 0;
+  const int lim = dim - kk * D4;
   // Feature row of lane row i in the tile at r (-1: past the end or a padding row), in two steps so that the id load
   // stays in flight: row_raw loads idx at a valid (clamped) row, row_of interprets the loaded value later.
   auto row_raw = [&](int64_t r) -> int {
@@ -173,7 +195,7 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(int64_t nrows, const int* __
   };
   // This lane's chunk [kk D4, (kk + 1) D4) of row src as raw floats.  Every load has a valid address (row 0 for
   // src < 0, the last feature past dim) and no lane's register is written twice, so the loads stay in flight until
-  // the values are converted; convert() zeroes what lies past dim or belongs to no row.
+  // the values are converted.
   auto load_x = [&](int64_t src, float* x) {
     const float* xs = feats + (src >= 0 ? src : 0) * dim;
     if (vec) {
@@ -192,9 +214,24 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(int64_t nrows, const int* __
       }
     }
   };
-  auto convert = [&](int64_t src, const float* x, double* a) {
+  auto convert = [&](const float* x, double* a) {
+    if (vec) {
 #pragma unroll
-    for (int s2 = 0; s2 < D4; ++s2) a[s2] = (src >= 0 && kk * D4 + s2 < dim) ? (double)x[s2] : 0.0;
+      for (int s2 = 0; s2 < D4; ++s2) a[s2] = (double)x[s2];
+    } else {
+#pragma unroll
+      for (int s2 = 0; s2 < D4; ++s2) a[s2] = s2 < lim ? (double)x[s2] : 0.0;
+    }
+  };
+  // d^2 of feature row src against pattern column c, directly in FP64 (the cancellation fallback).
+  auto direct = [&](int64_t src, int c) -> double {
+    const float* xs = feats + src * dim;
+    double d2 = 0.0;
+    for (int d = 0; d < dim; ++d) {
+      const double e = (double)xs[d] - pat[((int64_t)(c / 8) * dim + d) * 8 + c % 8];
+      d2 = fma(kW ? w[d] * e : e, e, d2);
+    }
+    return d2;
   };
   // The next tile's features are in flight (raw floats) during this tile's products and are converted at the end of
   // the iteration; its row ids are loaded a tile earlier still, so that its gather issues at once.
@@ -206,7 +243,7 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(int64_t nrows, const int* __
     src = row_of(r0, row_raw(r0));
     srcn = row_of(r0 + step, row_raw(r0 + step));
     load_x(src, xr);
-    convert(src, xr, a);
+    convert(xr, a);
   }
   for (; r0 < nrows; r0 += step) {
     const bool more = r0 + step < nrows;
@@ -214,57 +251,66 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(int64_t nrows, const int* __
     const int rawnn = row_raw(r0 + 2 * step);
     double xx = 0.0;
 #pragma unroll
-    for (int s2 = 0; s2 < D4; ++s2) xx = fma(w ? wsh[kk * D4 + s2] * a[s2] : a[s2], a[s2], xx);
+    for (int s2 = 0; s2 < D4; ++s2) xx = fma(kW ? wsh[kk * D4 + s2] * a[s2] : a[s2], a[s2], xx);
     xx += __shfl_xor(xx, 16, 64);
     xx += __shfl_xor(xx, 32, 64);  // lanes i, i + 16, ...: |x|^2 of row i
+    // This lane's four output rows 4 r + kk: their |x|^2, feature rows, and where their outputs go.
     double xxr[4];
-    int64_t srcr[4];
+    int srcr[4];
+    float* orow[4];
+    bool rok[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       xxr[r] = __shfl(xx, 4 * r + kk, 64);
-      srcr[r] = __shfl(src, 4 * r + kk, 64);
+      srcr[r] = __shfl((int)src, 4 * r + kk, 64);
+      const int64_t rr = r0 + 4 * r + kk;
+      rok[r] = rr < nrows;
+      orow[r] = out + (rok[r] ? rr : 0) * out_stride;
     }
     // Pattern tiles in pairs: two independent accumulator chains interleave on the matrix core.
     for (int pt = 0; pt < npt; pt += 2) {
       const bool two = pt + 1 < npt;
-      const double* bp0 = bsh + (pt * 16 + i) * (4 * D4) + kk * D4;
-      const double* bp1 = bsh + ((two ? pt + 1 : pt) * 16 + i) * (4 * D4) + kk * D4;
+      const double* bp0 = bsh + (pt * D4 * 4 + kk) * 16 + i;
+      const double* bp1 = bsh + ((two ? pt + 1 : pt) * D4 * 4 + kk) * 16 + i;
       d4v acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int s2 = 0; s2 < D4; ++s2) {
-        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], bp0[s2], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], bp1[s2], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], bp0[s2 * 64], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], bp1[s2 * 64], acc1, 0, 0, 0);
       }
-      for (int h = 0; h < (two ? 2 : 1); ++h) {
-        const int c = (pt + h) * 16 + i;
-        const d4v acc = h ? acc1 : acc0;
+      // Epilogue, branch-free but for the rare cancellation: d^2 in FP64, its square root in FP32 (sqrt_rn of the
+      // rounded d^2, within 0.75 ulp of the distance, since FP64 VALU work here competes with the products for the
+      // same FP64 rate).  Column c1 exists only with a second tile (two).
+      const int c0 = pt * 16 + i, c1 = c0 + 16;
+      const double p0 = psh[c0], p1 = psh[two ? c1 : c0];
+      double d2[2][4];
+      bool fix = false;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double ref0 = xxr[r] + p0, ref1 = xxr[r] + p1;
+        d2[0][r] = ref0 - 2.0 * acc0[r];
+        d2[1][r] = ref1 - 2.0 * acc1[r];
+        const bool live = srcr[r] >= 0;
+        fix |= live && c0 < K && d2[0][r] < 0x1p-10 * ref0;
+        fix |= live && two && c1 < K && d2[1][r] < 0x1p-10 * ref1;
+      }
+      if (__any(fix)) {  // cancellation: recompute those pairs directly
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t rr = r0 + 4 * r + kk;
-          if (rr >= nrows || c >= Kstore) continue;
-          float v;
-          if (srcr[r] < 0) {
-            v = INFINITY;
-          } else if (c >= K) {
-            v = 0.f;
-          } else {
-            const double ref = xxr[r] + psh[c];
-            double d2 = ref - 2.0 * acc[r];
-            if (d2 < 0x1p-10 * ref) {  // cancellation: recompute directly
-              const float* xs = feats + srcr[r] * dim;
-              d2 = 0.0;
-              for (int d = 0; d < dim; ++d) {
-                const double e = (double)xs[d] - pat[((int64_t)(c / 8) * dim + d) * 8 + c % 8];
-                d2 = fma(w ? w[d] * e : e, e, d2);
-              }
-            }
-            v = (float)sqrt(d2);
-          }
-          out[rr * out_stride + c] = v;
+          const bool live = srcr[r] >= 0;
+          if (live && c0 < K && d2[0][r] < 0x1p-10 * (xxr[r] + p0)) d2[0][r] = direct(srcr[r], c0);
+          if (live && two && c1 < K && d2[1][r] < 0x1p-10 * (xxr[r] + p1)) d2[1][r] = direct(srcr[r], c1);
         }
       }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float v0 = srcr[r] < 0 ? INFINITY : c0 < K ? sqrt_rn((float)d2[0][r]) : 0.f;
+        const float v1 = srcr[r] < 0 ? INFINITY : c1 < K ? sqrt_rn((float)d2[1][r]) : 0.f;
+        if (rok[r] && c0 < Kstore) orow[r][c0] = v0;
+        if (rok[r] && two && c1 < Kstore) orow[r][c1] = v1;
+      }
     }
-    if (more) convert(srcn, xr, a);
+    if (more) convert(xr, a);
     // An empty register barrier: the loaded id is first read here, so its load stays in flight until now.
     int raw = rawnn;
     if (kIdx) asm volatile("" : "+v"(raw));
@@ -341,12 +387,18 @@ void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, co
     const size_t lds = ((size_t)(Kp16 + 1) * 4 * D4 + Kp16) * sizeof(double);  // operands, weights, norms
 #define AC_CD_MFMA(N)                                                                                            \
   case N:                                                                                                        \
-    if (idx)                                                                                                     \
-      hipLaunchKernelGGL((k_cdist_mfma<N, true>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats, dim, patm, \
-                         pp, w, pat, K, Kstore, out, out_stride);                                                \
-    else                                                                                                         \
-      hipLaunchKernelGGL((k_cdist_mfma<N, false>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats, dim,      \
+    if (idx && w)                                                                                                \
+      hipLaunchKernelGGL((k_cdist_mfma<N, true, true>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats, dim, \
                          patm, pp, w, pat, K, Kstore, out, out_stride);                                          \
+    else if (idx)                                                                                                \
+      hipLaunchKernelGGL((k_cdist_mfma<N, true, false>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats,     \
+                         dim, patm, pp, w, pat, K, Kstore, out, out_stride);                                     \
+    else if (w)                                                                                                  \
+      hipLaunchKernelGGL((k_cdist_mfma<N, false, true>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats,     \
+                         dim, patm, pp, w, pat, K, Kstore, out, out_stride);                                     \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_cdist_mfma<N, false, false>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats,    \
+                         dim, patm, pp, w, pat, K, Kstore, out, out_stride);                                     \
     return;
     switch (D4) {
       AC_CD_MFMA(4)

@@ -265,3 +265,17 @@ def test_decimal_conversion_matches_pandas(tmp_path, tok):
     g = matcher.load_csv64(v, e)
     want = pd.read_csv(v, skiprows=1, sep=" ", header=None).values[0, 1]
     assert np.float64(want).tobytes() == np.float64(g.vfeat[0, 0]).tobytes(), (tok, want, g.vfeat[0, 0])
+
+
+def test_distance_square_root_correction_is_correctly_rounded(tmp_path):
+    """The distance kernels' FP32 square root (the hardware root corrected by ac::sqrt_rn_fix) is correctly rounded:
+    tests/sqrt_rn_check.hip runs the same correction step on the host over every 101st float from 2^-104 up, from
+    the correctly rounded root and both its neighbours."""
+    import os
+    exe = str(tmp_path / "sqrt_rn_check")
+    src = os.path.join(os.path.dirname(__file__), "sqrt_rn_check.hip")
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-x", "hip", "--offload-arch=gfx950", src, "-o", exe],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([exe, "101"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and " 0 wrong" in r.stdout, r.stdout + r.stderr
