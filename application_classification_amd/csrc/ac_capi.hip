@@ -857,7 +857,8 @@ static int enqueue_prep(ac_ctx* ctx) {
     const bool tape = ctx->tape_K > 0;
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, nullptr, ctx->VRs, Q, Qs, qs_prep, ctx->FMaxN.p, ctx->carry_in.p,
-                        walk_batch_of(ctx), st, tape ? ctx->FA0.p : nullptr, tape ? ctx->carg_in.p : nullptr);
+                        walk_batch_of(ctx), st, tape ? ctx->FA0.p : nullptr, tape ? ctx->carg_in.p : nullptr,
+                        nullptr, ctx->costs_packed ? ctx->CEp.p : nullptr, ctx->costs_packed ? ctx->bpack.p : nullptr);
     if (tape)
       ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_prep, ctx->FMaxN.p, st,
                                 ctx->carg_in.p, ctx->FA0.p);

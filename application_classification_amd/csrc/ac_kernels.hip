@@ -908,10 +908,10 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   const int* dstp = bdst + Bd.base + g;
   const f4v* VRv = reinterpret_cast<const f4v*>(VR);
   const int last = Bd.len / B - 1;
-  // Packed costs (walks with VR, CEp != nullptr): the bundle's base, 0 when it keeps its FP32 rows.  Decoding is
-  // exact, so maxima and recorded argmax rows are those of the FP32 walk.
-  const unsigned pbase = kWithVR && CEp ? bpack[b] : 0u;
-  const unsigned* cpb = kWithVR && CEp ? CEp + Bd.base * L4 * 3 : nullptr;
+  // Packed costs (CEp != nullptr): the bundle's base, 0 when it keeps its FP32 rows.  Decoding is exact, so maxima
+  // and recorded argmax rows are those of the FP32 walk.
+  const unsigned pbase = CEp ? bpack[b] : 0u;
+  const unsigned* cpb = CEp ? CEp + Bd.base * L4 * 3 : nullptr;
 #define AC_WK_SRC(S, T)                                                          \
   if (kWithVR) {                                                                 \
     const int t_ = (T) < last ? (T) : last;                                      \
@@ -969,8 +969,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
     }                                                                               \
   }
   // The pipelined walk over the bundle, for FP32 cost rows (kP false) or packed ones (decoded when reduced, so the
-  // loads stay in flight as raw dwords).  kP is written in terms of kWithVR so that it is value-dependent: the
-  // discarded `if constexpr` branches are then not instantiated.
+  // loads stay in flight as raw dwords).  kP is written in terms of the template parameters so that it is
+  // value-dependent: the discarded `if constexpr` branches are then not instantiated.
 #define AC_WK_PIPELINE(KP)                                                   \
   {                                                                          \
     constexpr bool kP = (KP);                                                \
@@ -994,8 +994,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
       if (t + 2 > last) break;                                               \
     }                                                                        \
   }
-  if (kWithVR && pbase)
-    AC_WK_PIPELINE(kWithVR)
+  if (pbase)
+    AC_WK_PIPELINE(kWithVR || B > 0)
   else
     AC_WK_PIPELINE(false && kWithVR)
 #undef AC_WK_PIPELINE
@@ -1069,10 +1069,10 @@ void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, 
                                    carry, FA, carg, bmin, CEp, bpack);
   else if (FA)
     launch_walk_max_p<false, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                   carry, FA, carg, bmin, nullptr, nullptr);
+                                   carry, FA, carg, bmin, CEp, bpack);
   else
     launch_walk_max_p<false, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                    carry, FA, carg, bmin, nullptr, nullptr);
+                                    carry, FA, carg, bmin, CEp, bpack);
 }
 
 // The base of a bundle's 24-bit cost codes from its least and largest cost bit patterns over columns q < Q, +inf
