@@ -164,7 +164,7 @@ def planted_patterns(data: Graph, sizes, seeds, sigma: float = 0.01):
 
 def uneven_patterns(data: Graph, count: int, seed: int, v_range=(4, 32)) -> list:
     """`count` planted patterns with V uniform in v_range and E uniform in [V, 2V] (the batched workload's shape).
-    A pattern whose edge count cannot be planted is drawn again with fewer edges and a new seed."""
+    A pattern whose edge count cannot be planted is drawn again with a new seed and fewer edges, never fewer than V."""
     rng = np.random.default_rng(seed)
     sizes = []
     for _ in range(count):
@@ -176,7 +176,7 @@ def uneven_patterns(data: Graph, count: int, seed: int, v_range=(4, 32)) -> list
         miss = [i for i, p in enumerate(out) if p is None]
         if not miss:
             break
-        sz = [(sizes[i][0], max(sizes[i][0] - 1, sizes[i][1] - attempt * max(1, sizes[i][0] // 4))) for i in miss]
+        sz = [(sizes[i][0], max(sizes[i][0], sizes[i][1] - attempt * max(1, sizes[i][0] // 4))) for i in miss]
         got = planted_patterns(data, sz, [seeds[i] + attempt for i in miss])
         for i, p in zip(miss, got):
             out[i] = p

@@ -171,6 +171,106 @@ def summarize_profile(src, workload, out_dir):
     print("\n".join(lines))
 
 
+def summarize_batched_profile(src, out_dir):
+    """Per-pass kernel time and HBM traffic of the batched workload, sequential plain runs against batched runs, from
+    rocprofv3 outputs under src: for each phase P in (seq, bat) and step count S in (1, 2), P<S>/trace (--kernel-trace
+    --stats), P<S>/fetch and P<S>/write (--pmc FETCH_SIZE / WRITE_SIZE) and P<S>.json (the bench's line); cfetch/ and
+    cwrite/ as for summarize_profile.  One pass is the difference of the two-step and one-step runs, so graph
+    generation, uploads and setup cancel.  Writes profiles/batched_rmat20-d32-256-uneven_traffic.{md,json}."""
+    import collections
+    import csv
+    import glob
+
+    def table(sub, pattern):
+        f = sorted(glob.glob(os.path.join(src, sub, "**", pattern), recursive=True))
+        return list(csv.DictReader(open(f[0]))) if f else []
+
+    def counter_total(sub, counter):
+        tot = collections.defaultdict(float)
+        for r in table(sub, "*_counter_collection.csv"):
+            if r.get("Counter_Name") == counter:
+                tot[_short(r["Kernel_Name"])] += float(r["Counter_Value"]) * 1024.0
+        return tot
+
+    def stats(sub):
+        return {_short(r["Name"]): (int(r["Calls"]), float(r["TotalDurationNs"]) / 1e6)
+                for r in table(sub, "*_kernel_stats.csv")}
+
+    def copy_scale(sub, counter):
+        c = counter_total(sub, counter)
+        k = next((k for k in c if "copyBuffer" in k), None)
+        return 3 * (1 << 30) / c[k] if k else None  # three 1 GiB copies
+
+    fs, ws = copy_scale("cfetch", "FETCH_SIZE"), copy_scale("cwrite", "WRITE_SIZE")
+    out = {"workload": "rmat20-d32-256-uneven-patterns", "fetch_scale": fs, "write_scale": ws, "phases": {}}
+    lines = ["# Batched workload: one pass, sequential plain runs against batched runs", "",
+             "R-MAT scale 20 (32-dim features) with 256 uneven planted patterns (V in [4, 32], E in [V, 2V]); one pass "
+             "scores all 256 patterns.  Every figure is the difference of a two-step and a one-step run, so it covers "
+             "exactly one pass (generation, uploads and setup cancel).  Under the profiler batched runs launch their "
+             "iterations directly instead of replaying graphs (same kernels, same order).", "",
+             "Commands (run from the repository root on one MI355X; P = sequential or batched, S = 1 and 2, "
+             "SLAB / WB = the autotuned slab width and walk batch of a plain `--config batched` run):", "",
+             "    rocprofv3 --kernel-trace --stats --output-format csv -d <dir>/<p><S>/trace -o t -- python -u bench.py "
+             "--config batched --phase P --steps S --warmup 0 --slab-edges SLAB --walk-batch WB",
+             "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/<p><S>/fetch -o f -- (the same bench command)",
+             "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir>/<p><S>/write -o w -- (the same bench command)",
+             "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/cfetch -o copy -- python -u bench.py "
+             "--copy-calibration",
+             "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir>/cwrite -o copy -- python -u bench.py "
+             "--copy-calibration",
+             "    python bench.py --summarize-batched <dir>", "",
+             f"Counter calibration on a plain 1 GiB device copy: FETCH_SIZE x {fs:.4f}, WRITE_SIZE x {ws:.4f}." if fs
+             and ws else "Counter calibration: missing.", ""]
+    for ph, label in (("seq", "sequential plain runs"), ("bat", "batched runs")):
+        js = {}
+        for st in (1, 2):
+            path = os.path.join(src, f"{ph}{st}.json")
+            if os.path.exists(path):
+                txt = [ln for ln in open(path).read().splitlines() if ln.startswith("{")]
+                js[st] = json.loads(txt[-1]) if txt else {}
+        upd = (js.get(2) or js.get(1) or {}).get("updates")
+        s1, s2 = stats(f"{ph}1/trace"), stats(f"{ph}2/trace")
+        f1, f2 = counter_total(f"{ph}1/fetch", "FETCH_SIZE"), counter_total(f"{ph}2/fetch", "FETCH_SIZE")
+        w1, w2 = counter_total(f"{ph}1/write", "WRITE_SIZE"), counter_total(f"{ph}2/write", "WRITE_SIZE")
+        kern = {}
+        for k in set(s1) | set(s2):
+            calls = s2.get(k, (0, 0.0))[0] - s1.get(k, (0, 0.0))[0]
+            ms = s2.get(k, (0, 0.0))[1] - s1.get(k, (0, 0.0))[1]
+            if calls <= 0:
+                continue
+            rd = (f2.get(k, 0.0) - f1.get(k, 0.0)) * (fs or float("nan"))
+            wr = (w2.get(k, 0.0) - w1.get(k, 0.0)) * (ws or float("nan"))
+            kern[k] = {"calls": calls, "ms": ms, "read_bytes": rd, "write_bytes": wr}
+        dev_ms = sum(v["ms"] for v in kern.values())
+        rd = sum(v["read_bytes"] for v in kern.values())
+        wr = sum(v["write_bytes"] for v in kern.values())
+        ph_out = {"device_ms": dev_ms, "read_bytes": rd, "write_bytes": wr, "updates": upd,
+                  "bytes_per_update": (rd + wr) / upd if upd else None,
+                  "achieved_GBps": (rd + wr) / (dev_ms * 1e-3) / 1e9 if dev_ms else None, "kernels": kern}
+        out["phases"][ph] = ph_out
+        lines += [f"## {label}", "",
+                  f"Kernel time {dev_ms:.1f} ms per pass; HBM {rd / 1e9:.2f} GB read + {wr / 1e9:.2f} GB written"
+                  + (f" = {(rd + wr) / upd:.3f} bytes per edge-pair update ({upd:.3e} updates)" if upd else "")
+                  + (f"; {ph_out['achieved_GBps']:.0f} GB/s over kernel time." if dev_ms else "."), "",
+                  "| kernel | calls | total ms | % | HBM read GB | HBM write GB |", "|---|---:|---:|---:|---:|---:|"]
+        for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"]):
+            if v["ms"] < 0.005 * dev_ms:
+                continue
+            lines.append(f"| {k} | {v['calls']} | {v['ms']:.2f} | {100 * v['ms'] / dev_ms:.1f} | "
+                         f"{v['read_bytes'] / 1e9:.3f} | {v['write_bytes'] / 1e9:.3f} |")
+        lines.append("")
+    sq, bt = out["phases"].get("seq"), out["phases"].get("bat")
+    if sq and bt and sq["device_ms"] and bt["device_ms"]:
+        lines.append(f"Sequential / batched: kernel time x{sq['device_ms'] / bt['device_ms']:.2f}, HBM bytes "
+                     f"x{(sq['read_bytes'] + sq['write_bytes']) / (bt['read_bytes'] + bt['write_bytes']):.2f}.")
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, "batched_rmat20-d32-256-uneven_traffic.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(out_dir, "batched_rmat20-d32-256-uneven_traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("\n".join(lines))
+
+
 def measured_traffic(workload, stage):
     """HBM bytes (read + write) per launch of a stage's kernel from the committed profile summary, or None."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
@@ -261,11 +361,12 @@ def bench_diff(args):
     print(json.dumps(out), flush=True)
 
 
-def measure_batched(steps, warmup, phase="both"):
+def measure_batched(steps, warmup, phase="both", tuned=None):
     """256 uneven planted patterns (V in [4, 32], E in [V, 2V]) against one resident R-MAT scale-20 graph (32-dim):
     the whole job (prep + iterations + output of every pattern) as 256 sequential plain runs, then as batched runs
     (autotuned slabs, graph-replayed iterations).  A step is one pass over all 256 patterns; figures are the best
-    step of each side.  phase "sequential" or "batched" times one side only (for per-side hardware counters)."""
+    step of each side.  phase "sequential" or "batched" times one side only (for per-side hardware counters); tuned
+    = (slab_edges, walk_batch) skips the autotuner."""
     from application_classification_amd import workloads as W
     from application_classification_amd.matcher import AC_OPT_GRAPH, Matcher
 
@@ -286,7 +387,7 @@ def measure_batched(steps, warmup, phase="both"):
     # Sequential plain runs: one ac_set_graph + ac_run per pattern, as the reference's driver scores one pattern.
     # The sampled patterns' results are read back before the timed passes (neither side times its copies).
     ref = {}
-    if phase != "batched":
+    if phase == "both":
         for i in sample:
             m.set_pattern(pats[i])
             m.run()
@@ -309,9 +410,17 @@ def measure_batched(steps, warmup, phase="both"):
         return {"value": upd / (t_seq * 1e-3), "sequential_ms_per_pass": t_seq,
                 "sequential_device_ms": min(d for _, d in seq_ms), "updates": upd}
     m.batch_set_patterns(pats)
-    t = time.time()
-    (cap, wb), est = m.batch_autotune(2)
-    log(f"autotune: slab edges {cap}, walk batch {wb} (estimated {est:.1f} ms) in {time.time() - t:.1f} s")
+    if tuned:
+        from application_classification_amd.matcher import AC_OPT_SLAB_EDGES, AC_OPT_WALK_BATCH
+
+        cap, wb = tuned
+        m.set_option(AC_OPT_SLAB_EDGES, cap)
+        m.set_option(AC_OPT_WALK_BATCH, wb)
+        log(f"slab edges {cap}, walk batch {wb} (given)")
+    else:
+        t = time.time()
+        (cap, wb), est = m.batch_autotune(2)
+        log(f"autotune: slab edges {cap}, walk batch {wb} (estimated {est:.1f} ms) in {time.time() - t:.1f} s")
     m.set_option(AC_OPT_GRAPH, 1)
     for i in range(warmup):
         m.batch_run()
@@ -363,7 +472,8 @@ def measure_batched(steps, warmup, phase="both"):
 
 
 def bench_batched(args):
-    r = measure_batched(args.steps, args.warmup, args.phase)
+    tuned = (args.slab_edges, args.walk_batch) if args.slab_edges else None
+    r = measure_batched(args.steps, args.warmup, args.phase, tuned)
     if args.phase != "both":
         print(json.dumps(dict(r, phase=args.phase, metric="batched workload, one side only", steps=args.steps,
                               warmup=args.warmup)), flush=True)
@@ -395,6 +505,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--phase", default="both", choices=["both", "sequential", "batched"],
                     help="--config batched: time both sides (default) or one (for per-side hardware counters)")
+    ap.add_argument("--slab-edges", type=int, default=0,
+                    help="--config batched: slab width in pattern edges (with --walk-batch; 0 = autotune)")
+    ap.add_argument("--walk-batch", type=int, default=0, help="--config batched: walk batch (with --slab-edges)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched and forward+backward measurements that the default line carries")
     ap.add_argument("--no-profile", action="store_true")
@@ -403,6 +516,10 @@ def main():
                          "headline launches only)")
     ap.add_argument("--summarize-profile", metavar="DIR",
                     help="summarise rocprofv3 outputs under DIR into profiles/ and exit")
+    ap.add_argument("--summarize-batched", metavar="DIR",
+                    help="summarise the batched workload's per-pass rocprofv3 outputs under DIR into profiles/ and "
+                         "exit")
+    ap.add_argument("--out", metavar="DIR", help="--summarize-batched: write the summary here instead of profiles/")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (scales rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE to bytes) and exit")
@@ -412,6 +529,9 @@ def main():
 
         name = {"s22": "rmat22-d64-p16x24", "s18": "rmat18-d32-p12x16", "small": "small-er1000"}[args.config]
         summarize_profile(args.summarize_profile, name, os.path.join(ROOT, "profiles"))
+        return
+    if args.summarize_batched:
+        summarize_batched_profile(args.summarize_batched, args.out or os.path.join(ROOT, "profiles"))
         return
     if args.copy_calibration:
         import torch

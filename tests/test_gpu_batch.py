@@ -111,6 +111,34 @@ def test_batch_rmat_planted(uneven):
             np.testing.assert_array_equal(mu.argmax(axis=0), p.vmap)
 
 
+def test_batched_workload_rmat20_256_patterns():
+    """The batched benchmark's workload exactly: 256 patterns (V in [4, 32], E in [V, 2V]) against R-MAT scale 20
+    with 32-dim features, autotuned slabs, graph-replayed iterations; every 32nd pattern against a sequential plain
+    run on the same resident data, and every pattern's planted vertices recovered."""
+    from application_classification_amd.matcher import AC_OPT_GRAPH
+
+    g, pats = W.batched(0)
+    assert len(pats) == 256 and all(p.num_nodes <= p.num_edges <= 2 * p.num_nodes for p in pats)
+    say(f"batched workload: {g.num_edges} edges, {sum(p.num_edges for p in pats)} pattern edges")
+    with Matcher(0) as m:
+        m.set_data(g)
+        ref = {}
+        for i in range(0, len(pats), 32):
+            m.set_pattern(pats[i])
+            ref[i] = m.run()
+        m.batch_set_patterns(pats)
+        (cap, wb), _ = m.batch_autotune(1)
+        m.set_option(AC_OPT_GRAPH, 1)
+        say(f"autotuned: slab edges {cap}, walk batch {wb}")
+        m.batch_run()
+        info = m.batch_info()
+        say(f"{info}")
+        for i, r in ref.items():
+            assert_close(m.batch_mu(i), r, f"pattern {i} ({pats[i].num_nodes}/{pats[i].num_edges})")
+        missed = [i for i, p in enumerate(pats) if not np.array_equal(m.batch_mu(i).argmax(axis=0), p.vmap)]
+        assert not missed, f"planted vertices not recovered for patterns {missed[:10]}"
+
+
 def test_graph_replay_is_bit_identical(small):
     """Iteration pairs replayed from a captured graph give exactly the launched result, run after run (each run
     replays the pair several times, with other work and synchronisations between runs)."""
