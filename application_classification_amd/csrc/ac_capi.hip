@@ -569,8 +569,9 @@ static int walk_batch_of(const ac_ctx* ctx) { return ctx->walk_B > 0 ? ctx->walk
 
 // Bundle lengths are padded to a multiple of the largest walk batch, so one layout serves every batch size.
 constexpr int kMaxWalkBatch = 4;
-// Bundle layouts kept per data graph (one per distinct G among recent patterns).
-constexpr size_t kBundleCache = 8;
+// Bundle layouts kept per data graph (one per distinct G among recent patterns).  Patterns of up to 64 edges have
+// at most 12 distinct G (G = 64 / (Qs / 4)), so a mix of such widths run one after another never rebuilds a layout.
+constexpr size_t kBundleCache = 16;
 
 // Selects the bundle layout for the current pattern width, building it if it is not cached: the
 // in-edge runs, longest first (ties by first source, so that the bundles walked at the same time gather from nearby

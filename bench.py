@@ -259,6 +259,19 @@ def summarize_batched_profile(src, out_dir):
             lines.append(f"| {k} | {v['calls']} | {v['ms']:.2f} | {100 * v['ms'] / dev_ms:.1f} | "
                          f"{v['read_bytes'] / 1e9:.3f} | {v['write_bytes'] / 1e9:.3f} |")
         lines.append("")
+    plain = os.path.join(src, "plain.json")
+    if os.path.exists(plain):  # the unprofiled `--config batched` run the configuration came from
+        txt = [ln for ln in open(plain).read().splitlines() if ln.startswith("{")]
+        pj = json.loads(txt[-1]) if txt else {}
+        out["plain_run"] = {k: pj.get(k) for k in ("sequential_ms_per_pass", "ms_per_step", "sequential_device_ms",
+                                                    "batched_device_ms", "vs_sequential_plain",
+                                                    "device_vs_sequential_plain", "checks")}
+        lines += ["## Unprofiled run (graph replay on)", "",
+                  f"Sequential {pj.get('sequential_ms_per_pass', 0):.1f} ms wall / "
+                  f"{pj.get('sequential_device_ms', 0):.1f} ms device per pass; batched "
+                  f"{pj.get('ms_per_step', 0):.1f} ms wall / {pj.get('batched_device_ms', 0):.1f} ms device: "
+                  f"x{pj.get('vs_sequential_plain', 0):.2f} wall clock, x{pj.get('device_vs_sequential_plain', 0):.2f} "
+                  f"device time; sampled patterns against sequential: {pj.get('checks')}.", ""]
     sq, bt = out["phases"].get("seq"), out["phases"].get("bat")
     if sq and bt and sq["device_ms"] and bt["device_ms"]:
         lines.append(f"Sequential / batched: kernel time x{sq['device_ms'] / bt['device_ms']:.2f}, HBM bytes "
