@@ -690,17 +690,21 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_src(int64_t nruns, const Ru
     const Run R = runs[r];
     Lse s[4] = {lse_empty(), lse_empty(), lse_empty(), lse_empty()};
     int cur = ks[R.kb].y;
+    // A finished source vertex.  It runs at vertex changes, which differ between the wave's runs, so each costs the
+    // whole wave: its log1p and the merge into the block partial take FP32 transcendentals (lsO is stored in FP32;
+    // the merge's exponential has the precision of lse_add's terms).
     auto emit = [&](int v) {
       float4 mn, ls, vf;
       mn.x = (float)(-s[0].m), mn.y = (float)(-s[1].m), mn.z = (float)(-s[2].m), mn.w = (float)(-s[3].m);
-      ls.x = (float)log1p(s[0].t), ls.y = (float)log1p(s[1].t), ls.z = (float)log1p(s[2].t), ls.w = (float)log1p(s[3].t);
+      ls.x = log1pf((float)s[0].t), ls.y = log1pf((float)s[1].t), ls.z = log1pf((float)s[2].t);
+      ls.w = log1pf((float)s[3].t);
       vf.x = (float)s[0].m, vf.y = (float)s[1].m, vf.z = (float)s[2].m, vf.w = (float)s[3].m;
       reinterpret_cast<float4*>(minCEo)[(int64_t)v * L4 + j] = mn;
       reinterpret_cast<float4*>(lsO)[(int64_t)v * L4 + j] = ls;
       reinterpret_cast<float4*>(VFm)[(int64_t)v * L4 + j] = vf;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        glob[c] = lse_merge(glob[c], s[c]);
+        glob[c] = lse_merge_fast(glob[c], s[c]);
         s[c] = lse_empty();
       }
     };
@@ -733,7 +737,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_src(int64_t nruns, const Ru
       for (int c = 0; c < 4; ++c) {
         carry[((int64_t)R.slot * Qs + 4 * j + c) * 2 + 0] = s[c].m;
         carry[((int64_t)R.slot * Qs + 4 * j + c) * 2 + 1] = s[c].t;
-        glob[c] = lse_merge(glob[c], s[c]);
+        glob[c] = lse_merge_fast(glob[c], s[c]);
       }
     } else {
       emit(cur);
@@ -1419,6 +1423,10 @@ static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
 int vertex_tile(int P, int Q) {
   const int Qs = std::max(4, (Q + 3) / 4 * 4);
   int vt = kVTMax;
+  if (const char* e = std::getenv("AC_VERTEX_TILE")) {  // tuning override: rows per tile, 8, 16, 32 or 64
+    const int v = std::atoi(e);
+    if (v == 8 || v == 16 || v == 32 || v == 64) vt = v;
+  }
   while (vt > 4 && (vertex_lds_bytes(kModeA, vt, P, Q, Qs) > 65536 || vt * Qs > kTileFloats ||
                     vt * P > kTileFloats))
     vt /= 2;

@@ -40,6 +40,19 @@ __device__ __forceinline__ Lse lse_merge(Lse a, Lse b) {
   return b;
 }
 
+// lse_merge with the exponential of the two maxima's difference in FP32 (__expf), as lse_add forms each term: for merges
+// in a kernel's inner loop, where the FP64 exp would dominate.  The merged t keeps a relative error of about 1e-7.
+__device__ __forceinline__ Lse lse_merge_fast(Lse a, Lse b) {
+  if (b.m == -__builtin_huge_val()) return a;
+  if (a.m == -__builtin_huge_val()) return b;
+  if (a.m >= b.m) {
+    a.t += (1.0 + b.t) * (double)__expf((float)(b.m - a.m));
+    return a;
+  }
+  b.t += (1.0 + a.t) * (double)__expf((float)(a.m - b.m));
+  return b;
+}
+
 __device__ __forceinline__ Lse lse_empty() { return Lse{-__builtin_huge_val(), 0.0}; }
 
 }  // namespace ac

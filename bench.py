@@ -284,6 +284,45 @@ def summarize_batched_profile(src, out_dir):
     print("\n".join(lines))
 
 
+def summarize_diff_profile(src, out_dir):
+    """Writes profiles/diff_rmat18-d32-64-patterns.{json,md} from one GPU call: src/diff.json (the line of
+    `bench.py --config diff`) and src/trace/ (rocprofv3 --kernel-trace --stats of `bench.py --config diff --steps 1
+    --warmup 1`); the .md's figures are the .json's."""
+    import csv
+    import glob
+
+    txt = [ln for ln in open(os.path.join(src, "diff.json")).read().splitlines() if ln.startswith("{")]
+    d = json.loads(txt[-1])
+    f = sorted(glob.glob(os.path.join(src, "trace", "**", "*_kernel_stats.csv"), recursive=True))
+    rows = list(csv.DictReader(open(f[0]))) if f else []
+    tot = sum(float(r["TotalDurationNs"]) for r in rows) / 1e6
+    c = d.get("config", {})
+    lines = ["# Adjoint: 64 planted 8-16-vertex patterns on R-MAT scale 18 (32-dim), learnable cdist weights", "",
+             "Commands (run from the repository root on one MI355X):", "",
+             f"    python -u bench.py --config diff --steps {d['steps']} --warmup {d['warmup']} > <dir>/diff.json",
+             "    rocprofv3 --kernel-trace --stats --output-format csv -d <dir>/trace -o diff -- "
+             "python -u bench.py --config diff --steps 1 --warmup 1",
+             "    python bench.py --summarize-diff <dir>", "",
+             "| pass over 64 patterns | ms |", "|---|---|",
+             f"| forward (ac_run, weighted cdist) | {d['forward_ms_per_step']:.1f} |",
+             f"| forward with tape (checkpoint every {c.get('checkpoint_every')}) + ac_backward of the planted "
+             f"cross-entropy | {d['ms_per_step']:.1f} |", "",
+             f"forward+backward / forward = **{d['value']:.2f}** (best of {d['steps']} steps after {d['warmup']} "
+             f"warm-up). Checks: {d.get('checks')}.", ""]
+    if rows:
+        lines += [f"Kernel time of the profiled command (one warm-up and one timed step of each pass): {tot:.1f} ms; "
+                  "the sixteen largest:", "", "| kernel | calls | total ms | % |", "|---|---:|---:|---:|"]
+        for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:16]:
+            lines.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.1f} | "
+                         f"{float(r['Percentage']):.1f} |")
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, "diff_rmat18-d32-64-patterns.md"), "w") as fo:
+        fo.write("\n".join(lines) + "\n")
+    with open(os.path.join(out_dir, "diff_rmat18-d32-64-patterns.json"), "w") as fo:
+        json.dump(d, fo, indent=1)
+    print("\n".join(lines))
+
+
 def measured_traffic(workload, stage):
     """HBM bytes (read + write) per launch of a stage's kernel from the committed profile summary, or None."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
@@ -532,6 +571,9 @@ def main():
     ap.add_argument("--summarize-batched", metavar="DIR",
                     help="summarise the batched workload's per-pass rocprofv3 outputs under DIR into profiles/ and "
                          "exit")
+    ap.add_argument("--summarize-diff", metavar="DIR",
+                    help="write the adjoint profile (json and md) into profiles/ from DIR/diff.json and DIR/trace, "
+                         "and exit")
     ap.add_argument("--out", metavar="DIR", help="--summarize-batched: write the summary here instead of profiles/")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (scales rocprofv3 FETCH_SIZE / "
@@ -542,6 +584,9 @@ def main():
 
         name = {"s22": "rmat22-d64-p16x24", "s18": "rmat18-d32-p12x16", "small": "small-er1000"}[args.config]
         summarize_profile(args.summarize_profile, name, os.path.join(ROOT, "profiles"))
+        return
+    if args.summarize_diff:
+        summarize_diff_profile(args.summarize_diff, args.out or os.path.join(ROOT, "profiles"))
         return
     if args.summarize_batched:
         summarize_batched_profile(args.summarize_batched, args.out or os.path.join(ROOT, "profiles"))
