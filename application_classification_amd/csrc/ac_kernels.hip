@@ -1410,7 +1410,7 @@ struct IncRec {
 };
 
 static size_t vertex_lds_bytes(int mode, int VT, int P, int Q, int Qs) {
-  size_t b = (size_t)(4 * Q + 2 * P) * sizeof(double);                       // tables
+  size_t b = (size_t)(2 * P) * sizeof(double);                               // MU normalisers
   b += (size_t)(2 * Q + 1) * sizeof(IncRec);                                 // incidence records (+ one pad)
   if (mode != kModeC) b += (size_t)(P + 1) * VT * sizeof(double);            // MU
   if (mode == kModeC) b += (size_t)P * sizeof(Lse);                          // column accumulators
@@ -1510,8 +1510,7 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   extern __shared__ double lds[];
   const int Q = a.Q, P = a.P, Qs = a.Qs, VT = a.VT;
   const int ldq = Qs + 1, ldp = P + 1;
-  double* tab = lds;                                   // [4][Q]: fillF, fillR, mRE, lRE
-  double* pmu = tab + 4 * Q;                           // [2][P]: m, l of LSE_MU
+  double* pmu = lds;                                   // [2][P]: m, l of LSE_MU
   IncRec* recs = reinterpret_cast<IncRec*>(pmu + 2 * P);  // [2Q + 1] incidence records, inc order
   double* MUs = reinterpret_cast<double*>(recs + 2 * Q + 1);  // [VT][P + 1] (modes A, OUT, MU)
   const int ldm = P + 1;
@@ -1534,13 +1533,6 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
       r = (code & 1) ? IncRec{x.fillR, x.mRE, x.lRE, VT * ldq + qq, 0} : IncRec{x.fillF, 0.0, 0.0, qq, 0};
     }
     recs[k] = r;
-  }
-  for (int i = t; i < Q; i += NT) {
-    const QStat x = a.qs[i];
-    tab[i] = x.fillF;
-    tab[Q + i] = x.fillR;
-    tab[2 * Q + i] = x.mRE;
-    tab[3 * Q + i] = x.lRE;
   }
   for (int i = t; i < P; i += NT) {
     if (kMode != kModeC) {
@@ -1565,6 +1557,13 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
   // Read once: loads inside the tile loop would follow the loop's stores (possible aliases) and wait for every
   // outstanding load, the next tile's prefetch included.
   const int q_ps = kA && qthread ? a.ps[q] : 0, q_pd = kA && qthread ? a.pd[q] : 0;
+  // The thread's pattern edge's fills and RE normaliser, in registers for the whole pass: an LDS table of them would
+  // take pass A (VT = 64, 24 pattern edges) just past the 40 KB that lets four blocks share a CU.
+  double fF = 0.0, fR = 0.0, mR = 0.0, lR = 0.0;
+  if (kA && qthread) {
+    const QStat x = a.qs[q];
+    fF = x.fillF, fR = x.fillR, mR = x.mRE, lR = x.lRE;
+  }
   double vfmax = -kInf, vrmax = -kInf;
   int avf = -1, avr = -1;
   Lse lre = lse_empty(), lfe = lse_empty();
@@ -1703,7 +1702,6 @@ __global__ __launch_bounds__(NT) void k_vertex(VertexArgs a) {
     if (kA) {
       __syncthreads();
       if (qthread) {
-        const double fF = tab[q], fR = tab[Q + q], mR = tab[2 * Q + q], lR = tab[3 * Q + q];
         for (int r = rq; r < nv; r += RQ) {
           const int64_t i = (v0 + r) * Qs + q;
           const float mo = Ms[r * ldq + q];
