@@ -139,11 +139,11 @@ def summarize_profile(src, workload, out_dir):
     lines = [f"# Kernel profile: {workload}", "",
              "Commands (run from the repository root on one MI355X):", "",
              "    rocprofv3 --kernel-trace --stats --output-format csv -d <dir>/trace -o s22 -- "
-             "python -u bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-pruned",
+             "python -u bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-pruned --no-extras",
              "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/fetch -o s22 -- "
-             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile --no-pruned",
+             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile --no-pruned --no-extras",
              "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir>/write -o s22 -- "
-             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile --no-pruned",
+             "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile --no-pruned --no-extras",
              "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/cfetch -o copy -- python -u bench.py --copy-calibration",
              "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir>/cwrite -o copy -- python -u bench.py --copy-calibration",
              f"    python bench.py --summarize-profile <dir>", "",
