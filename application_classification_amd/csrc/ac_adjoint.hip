@@ -210,9 +210,10 @@ __global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const do
 // One group of Qs/4 lanes walks one CSR run (lane j owning pattern edges 4j..4j+3), loading the (row, source) pairs,
 // then the rows' destinations, then the destinations' FA rows of a batch of edges together.
 __global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const Run* __restrict__ runs,
-                                                           const int2* __restrict__ ks, const int* __restrict__ bdst,
+                                                           const int2* __restrict__ ks, const int* __restrict__ kdst,
                                                            int Q, int Qs, const float* __restrict__ gEF,
-                                                           const float* __restrict__ gER, const int* __restrict__ FA,
+                                                           const float* __restrict__ gER,
+                                                           const unsigned* __restrict__ argbits,
                                                            const int* __restrict__ AO, float* __restrict__ gVR,
                                                            float* __restrict__ gCE, double* __restrict__ carry) {
   const int L4 = Qs / 4, G = 64 / L4;
@@ -236,17 +237,20 @@ __global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const 
       acc[c] = 0.0;
     }
   };
+  // Lane j's four columns in the rows' argmax masks: word (4j) / 32, bits (4j) % 32 .. + 3.
+  const int W = (Qs + 31) / 32, aw = (4 * j) >> 5, ab = (4 * j) & 31;
   constexpr int kBatch = 8;
   for (int k0 = R.kb; k0 < R.ke; k0 += kBatch) {
     int2 e[kBatch];
     int v[kBatch];
-    int4 fa[kBatch];
+    unsigned fa[kBatch];
+    // The pairs and the destinations (kdst, per CSR position) load together; the rows' argmax masks next.
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) e[u] = ks[k0 + u < R.ke ? k0 + u : R.ke - 1];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) v[u] = bdst[e[u].x];
+    for (int u = 0; u < kBatch; ++u) v[u] = kdst[k0 + u < R.ke ? k0 + u : R.ke - 1];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) fa[u] = reinterpret_cast<const int4*>(FA)[(int64_t)v[u] * L4 + j];
+    for (int u = 0; u < kBatch; ++u) fa[u] = (argbits[(int64_t)e[u].x * W + aw] >> ab) & 0xFu;
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= R.ke) break;
@@ -257,14 +261,13 @@ __global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const 
         gr = reinterpret_cast<const float4*>(gER)[(int64_t)cur * L4 + j];
       }
       const int row = e[u].x;
-      const int fr[4] = {fa[u].x, fa[u].y, fa[u].z, fa[u].w};
       const int ar[4] = {ao.x, ao.y, ao.z, ao.w};
       const float grv[4] = {gr.x, gr.y, gr.z, gr.w};
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int q = 4 * j + c;
         if (q >= Q) continue;
-        const float gf = fr[c] == row ? gEF[(int64_t)v[u] * Qs + q] : 0.f;
+        const float gf = (fa[u] >> c) & 1u ? gEF[(int64_t)v[u] * Qs + q] : 0.f;
         const float gq = ar[c] == row ? grv[c] : 0.f;
         if (gf == 0.f && gq == 0.f) continue;
         float* o = gCE + (int64_t)row * Qs + q;
@@ -644,12 +647,44 @@ void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const
   hipLaunchKernelGGL(k_bwd_in, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, S, MU, qs1, red4, gER, gVR, gVF, cD);
 }
 
+// argbits[row][q / 32] bit q % 32: row is the FMax argmax of column q of its destination (FA[v][q] == row) and that
+// edge term carries gradient (gEF[v][q] != 0), the only case in which the gather uses it.  FA rows are written only
+// where the walk recorded a maximum (vertices with in-edges, bundles not pruned); a nonzero gEF implies one, and the
+// row is bounds-checked besides.  Each (v, q) sets one bit, with an atomic OR: exact and independent of order.  The
+// words must be zero beforehand.
+__global__ void k_argbits(int64_t n, int Q, int Qs, const int* __restrict__ FA, const float* __restrict__ gEF,
+                          int64_t brows, unsigned* __restrict__ argbits) {
+  const int W = (Qs + 31) / 32;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * Qs; i += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(i % Qs);
+    if (q >= Q || gEF[i] == 0.f) continue;
+    const int fa = FA[i];
+    if (fa >= 0 && fa < brows) atomicOr(argbits + (int64_t)fa * W + (q >> 5), 1u << (q & 31));
+  }
+}
+
+__global__ void k_csr_dst(int64_t m, const int2* __restrict__ ks, const int* __restrict__ bdst, int* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = bdst[ks[i].x];
+}
+
+void launch_csr_dst(int64_t m, const int2* ks, const int* bdst, int* out, hipStream_t st) {
+  if (m == 0) return;
+  const int blocks = (int)std::min<int64_t>((m + 255) / 256, 1 << 16);
+  hipLaunchKernelGGL(k_csr_dst, dim3(blocks), dim3(256), 0, st, m, ks, bdst, out);
+}
+
 void launch_bwd_gather(const BwdGraph& G, int64_t nruns, const Run* runs, int nsplit, const Split* splits,
-                       const int2* ks, const int* bdst, const float* gEF, const float* gER, const int* FA, const int* AO,
-                       float* gVR, float* gCE, double* carry, hipStream_t st) {
+                       const int2* ks, const int* kdst, const float* gEF, const float* gER, const int* FA, const int* AO,
+                       float* gVR, float* gCE, double* carry, unsigned* argbits, int64_t brows, hipStream_t st) {
   if (G.Q == 0 || nruns == 0) return;
-  hipLaunchKernelGGL(k_bwd_gather, dim3(walk_blocks(nruns, G.Qs)), dim3(kWalkBlock), 0, st, nruns, runs, ks, bdst, G.Q,
-                     G.Qs, gEF, gER, FA, AO, gVR, gCE, carry);
+  const int W = (G.Qs + 31) / 32;
+  launch_fill_int(reinterpret_cast<int*>(argbits), brows * W, 0, st);
+  const int64_t nq = G.n * G.Qs;
+  hipLaunchKernelGGL(k_argbits, dim3((unsigned)std::min<int64_t>((nq + 255) / 256, 1 << 16)), dim3(256), 0, st, G.n,
+                     G.Q, G.Qs, FA, gEF, brows, argbits);
+  hipLaunchKernelGGL(k_bwd_gather, dim3(walk_blocks(nruns, G.Qs)), dim3(kWalkBlock), 0, st, nruns, runs, ks, kdst, G.Q,
+                     G.Qs, gEF, gER, argbits, AO, gVR, gCE, carry);
   if (gVR && nsplit > 0)
     hipLaunchKernelGGL(k_bwd_gather_fixup, dim3((unsigned)((nsplit + 3) / 4)), dim3(256), 0, st, nsplit, splits, carry,
                        G.Q, G.Qs, gVR);

@@ -215,6 +215,7 @@ struct ac_ctx {
     DevBuf<Bundle> bundles;
     DevBuf<int> gslot, bsrc, bdst, beid;
     DevBuf<int2> csr_kb;
+    DevBuf<int> csr_dst;  // the destination at each CSR position (built on the first backward pass)
     int64_t nbundles = 0, brows = 0;
   };
   std::vector<std::unique_ptr<BundleSet>> bsets;
@@ -258,6 +259,7 @@ struct ac_ctx {
   DevBuf<Lse2> ckP, sgP;
   // Adjoint work buffers.
   DevBuf<float> gMUin, gMU, gCV, gFX, gRX, gEF, gER, gVR, gVF, gCE, apv;
+  DevBuf<unsigned> argbits;  // per bundle row, the pattern edges it is the FMax argmax of (backward scratch)
   DevBuf<double> cD, colsum, bpart, red4, cq, opart, osum, MUd;
   DevBuf<int> api;
 
@@ -850,16 +852,16 @@ static int enqueue_prep(ac_ctx* ctx) {
     ac::launch_bundle_min(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Q, Qs, ctx->bmin.p,
                           ctx->pack_costs ? ctx->bpack.p : nullptr, st);
     ctx->costs_packed = false;
-    if (ctx->pack_costs) {
-      HIPCHK(ctx, ctx->CEp.alloc((size_t)std::max<int64_t>(1, ctx->bs->brows * (Qs / 4) * 3)));
-      ac::launch_pack_costs(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Q, Qs, ctx->bpack.p, ctx->CEp.p, st);
-      ctx->costs_packed = true;
-    }
+    if (ctx->pack_costs) HIPCHK(ctx, ctx->CEp.alloc((size_t)std::max<int64_t>(1, ctx->bs->brows * (Qs / 4) * 3)));
+    // The prep's max over in-edges of -CE reads every cost row once, so it also writes the 24-bit codes of the
+    // bundles that pack (one pass over CE instead of a separate packing pass).
     const bool tape = ctx->tape_K > 0;
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, nullptr, ctx->VRs, Q, Qs, qs_prep, ctx->FMaxN.p, ctx->carry_in.p,
                         walk_batch_of(ctx), st, tape ? ctx->FA0.p : nullptr, tape ? ctx->carg_in.p : nullptr,
-                        nullptr, ctx->costs_packed ? ctx->CEp.p : nullptr, ctx->costs_packed ? ctx->bpack.p : nullptr);
+                        nullptr, nullptr, ctx->pack_costs ? ctx->bpack.p : nullptr,
+                        ctx->pack_costs ? ctx->CEp.p : nullptr);
+    ctx->costs_packed = ctx->pack_costs;
     if (tape)
       ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_prep, ctx->FMaxN.p, st,
                                 ctx->carg_in.p, ctx->FA0.p);
@@ -1699,6 +1701,11 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, ctx->gCV.alloc(np));
   for (DevBuf<float>* b : {&ctx->gFX, &ctx->gRX, &ctx->gEF, &ctx->gER, &ctx->gVR, &ctx->gVF}) HIPCHK(ctx, b->alloc(nq));
   HIPCHK(ctx, ctx->gCE.alloc((size_t)std::max<int64_t>(1, brows) * Qs));
+  HIPCHK(ctx, ctx->argbits.alloc((size_t)std::max<int64_t>(1, brows) * ((Qs + 31) / 32)));
+  if (ctx->bs->csr_dst.n != (size_t)std::max<int64_t>(1, ctx->de)) {  // once per bundle layout
+    HIPCHK(ctx, ctx->bs->csr_dst.alloc((size_t)std::max<int64_t>(1, ctx->de)));
+    ac::launch_csr_dst(ctx->de, ctx->bs->csr_kb.p, ctx->bs->bdst.p, ctx->bs->csr_dst.p, st);
+  }
   HIPCHK(ctx, ctx->cD.alloc(nq));
   HIPCHK(ctx, ctx->MUd.alloc(np));
   HIPCHK(ctx, ctx->colsum.alloc((size_t)P));
@@ -1755,8 +1762,8 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
       ac::launch_bwd_in(G, S0, ctx->MUd.p, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, st);
       ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p,
-                            ctx->bs->csr_kb.p, ctx->bs->bdst.p, ctx->gEF.p, ctx->gER.p, ctx->tapeFA.p + s * nq,
-                            ctx->AO.p, ctx->gVR.p, ctx->gCE.p, ctx->carry_out.p, st);
+                            ctx->bs->csr_kb.p, ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gER.p, ctx->tapeFA.p + s * nq,
+                            ctx->AO.p, ctx->gVR.p, ctx->gCE.p, ctx->carry_out.p, ctx->argbits.p, brows, st);
       ac::launch_bwd_mu(G, ctx->gVR.p, ctx->gVF.p, ctx->gMU.p, ctx->gFX.p, ctx->gRX.p, st);
     }
   }
@@ -1768,8 +1775,8 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
     ac::launch_bwd_route(G, S0, nullptr, 0, ctx->gFX.p, ctx->gRX.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
                          ctx->red4.p, st);
     ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p, ctx->bs->csr_kb.p,
-                          ctx->bs->bdst.p, ctx->gEF.p, ctx->gER.p, ctx->FA0.p, ctx->AO.p, nullptr, ctx->gCE.p,
-                          ctx->carry_out.p, st);
+                          ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gER.p, ctx->FA0.p, ctx->AO.p, nullptr, ctx->gCE.p,
+                          ctx->carry_out.p, ctx->argbits.p, brows, st);
     ac::launch_bwd_prep_fill(G, ctx->red4.p, ctx->am0.p, ctx->gMU.p, st);
     std::vector<double> red4((size_t)4 * Q);
     std::vector<QStat> qs0(Q);

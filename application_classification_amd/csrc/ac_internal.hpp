@@ -131,16 +131,13 @@ void launch_walk_src_fixup(int nsplit, const Split* splits, const double* carry,
 void launch_build_bundles(int64_t nbundles, const Bundle* bundles, const int* brun, const Run* runs, const int2* sd,
                           const int* eid, int G, int* bsrc, int* bdst, int* beid, int* rowmap, hipStream_t st);
 void launch_remap_ks(int64_t m, const int2* ks, const int* rowmap, int2* out, hipStream_t st);
-// CEp / bpack (launch_bundle_min, launch_pack_costs): walks with VR read the packed rows of the bundles whose base is
-// nonzero.
+// CEp / bpack (launch_bundle_min): the walk reads the packed rows of the bundles whose base is nonzero.  CEp_out /
+// bpack (the prep walk, no pruning): the walk reads the FP32 rows and writes those bundles' packed rows to CEp_out.
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
                      double* carry, int B, hipStream_t st, int* FA = nullptr, int* carg = nullptr,
-                     const float* bmin = nullptr, const unsigned* CEp = nullptr, const unsigned* bpack = nullptr);
-// Lossless 24-bit coding of the bundle-ordered costs into CEp[rows * Qs * 3 / 4] (three dwords per four costs), for
-// the bundles whose base bpack[b] (written by launch_bundle_min) is nonzero.
-void launch_pack_costs(int64_t nbundles, const Bundle* bundles, const float* CE, int Q, int Qs, const unsigned* bpack,
-                       unsigned* CEp, hipStream_t st);
+                     const float* bmin = nullptr, const unsigned* CEp = nullptr, const unsigned* bpack = nullptr,
+                     unsigned* CEp_out = nullptr);
 // Plain-path fixup (FP32 normalised pieces): chunks of at most kFixChunk pieces, then the vertices of several
 // chunks from their partials (part: one [Qs] row per partial).
 constexpr int kFixChunk = 16;

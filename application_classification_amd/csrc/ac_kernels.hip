@@ -842,7 +842,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
                                                          double* __restrict__ carry, int* __restrict__ FA,
                                                          int* __restrict__ carg, const float* __restrict__ bmin,
                                                          const unsigned* __restrict__ CEp,
-                                                         const unsigned* __restrict__ bpack) {
+                                                         const unsigned* __restrict__ bpack,
+                                                         unsigned* __restrict__ CEpo) {
   const int L4 = Qs / 4, G = 64 / L4, VR4s = VRs / 4;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = lane / L4, j = lane - g * L4;
@@ -916,6 +917,11 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
   // and recorded argmax rows are those of the FP32 walk.
   const unsigned pbase = CEp ? bpack[b] : 0u;
   const unsigned* cpb = CEp ? CEp + Bd.base * L4 * 3 : nullptr;
+  // Packing on the way (the prep walk, CEpo != nullptr, no pruning): every row is reduced exactly once from its FP32
+  // costs, and is also written out as 24-bit codes when the bundle's base is nonzero (pack_base: each cost of a column
+  // q < Q other than +inf codes as its bit pattern minus the base, +inf and columns q >= Q as 0).
+  const unsigned obase = CEpo && !kPrune ? bpack[b] : 0u;
+  unsigned* const cpo = obase ? CEpo + Bd.base * L4 * 3 : nullptr;
 #define AC_WK_SRC(S, T)                                                          \
   if (kWithVR) {                                                                 \
     const int t_ = (T) < last ? (T) : last;                                      \
@@ -955,6 +961,17 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_max(int64_t nbundles, const
       c_ = decode_costs(C[u], pbase);                                               \
     else                                                                            \
       c_ = C[u];                                                                    \
+    if (!kP && cpo) {                                                               \
+      unsigned k_[4];                                                               \
+      _Pragma("unroll") for (int w_ = 0; w_ < 4; ++w_) {                            \
+        const unsigned x_ = __float_as_uint(c_[w_]);                                \
+        k_[w_] = (4 * j + w_ < Q && x_ != 0x7F800000u) ? x_ - obase : 0u;           \
+      }                                                                             \
+      unsigned* o_ = cpo + 3 * (co + (unsigned)(((T) * B + u) * GL4));             \
+      o_[0] = k_[0] | (k_[1] << 24);                                                \
+      o_[1] = (k_[1] >> 8) | (k_[2] << 16);                                         \
+      o_[2] = (k_[2] >> 16) | (k_[3] << 8);                                         \
+    }                                                                               \
     const double d0 = (double)W[u].x - (double)c_.x;                                \
     const double d1 = (double)W[u].y - (double)c_.y;                                \
     const double d2 = (double)W[u].z - (double)c_.z;                                \
@@ -1028,19 +1045,23 @@ template <bool kWithVR, bool kArg, bool kPrune>
 static void launch_walk_max_t(int B, int nb, hipStream_t st, int64_t nbundles, const Bundle* bundles,
                               const int* gslot, const int* bsrc, const int* bdst, const float* CE, const float* VR,
                               int VRs, int Q, int Qs, const QStat* qs, float* FMaxN, double* carry, int* FA,
-                              int* carg, const float* bmin, const unsigned* CEp, const unsigned* bpack) {
+                              int* carg, const float* bmin, const unsigned* CEp, const unsigned* bpack,
+                              unsigned* CEpo) {
   switch (B) {
     case 1:
       hipLaunchKernelGGL((k_walk_max<kWithVR, 1, kArg, kPrune>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles,
-                         bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
+                         bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack,
+                         CEpo);
       break;
     case 4:
       hipLaunchKernelGGL((k_walk_max<kWithVR, 4, kArg, kPrune>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles,
-                         bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
+                         bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack,
+                         CEpo);
       break;
     default:
       hipLaunchKernelGGL((k_walk_max<kWithVR, 2, kArg, kPrune>), dim3(nb), dim3(kWalkBlock), 0, st, nbundles,
-                         bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack);
+                         bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN, carry, FA, carg, bmin, CEp, bpack,
+                         CEpo);
       break;
   }
 }
@@ -1049,34 +1070,36 @@ template <bool kWithVR, bool kArg>
 static void launch_walk_max_p(int B, int nb, hipStream_t st, int64_t nbundles, const Bundle* bundles,
                               const int* gslot, const int* bsrc, const int* bdst, const float* CE, const float* VR,
                               int VRs, int Q, int Qs, const QStat* qs, float* FMaxN, double* carry, int* FA,
-                              int* carg, const float* bmin, const unsigned* CEp, const unsigned* bpack) {
+                              int* carg, const float* bmin, const unsigned* CEp, const unsigned* bpack,
+                              unsigned* CEpo) {
   if (bmin)
     launch_walk_max_t<kWithVR, kArg, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs,
-                                           FMaxN, carry, FA, carg, bmin, CEp, bpack);
+                                           FMaxN, carry, FA, carg, bmin, CEp, bpack, CEpo);
   else
     launch_walk_max_t<kWithVR, kArg, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs,
-                                            FMaxN, carry, FA, carg, bmin, CEp, bpack);
+                                            FMaxN, carry, FA, carg, bmin, CEp, bpack, CEpo);
 }
 
 void launch_walk_max(int64_t nbundles, const Bundle* bundles, const int* gslot, const int* bsrc, const int* bdst,
                      const float* CE, const float* VR, int VRs, int Q, int Qs, const QStat* qs, float* FMaxN,
                      double* carry, int B, hipStream_t st, int* FA, int* carg, const float* bmin,
-                     const unsigned* CEp, const unsigned* bpack) {
+                     const unsigned* CEp, const unsigned* bpack, unsigned* CEp_out) {
   if (nbundles == 0) return;
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
-  if (!bpack) CEp = nullptr;
+  if (!bpack) CEp = CEp_out = nullptr;
+  if (CEp_out) CEp = nullptr;  // packing on the way: the walk reads the FP32 rows
   if (VR && FA)
     launch_walk_max_p<true, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                  carry, FA, carg, bmin, CEp, bpack);
+                                  carry, FA, carg, bmin, CEp, bpack, CEp_out);
   else if (VR)
     launch_walk_max_p<true, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                   carry, FA, carg, bmin, CEp, bpack);
+                                   carry, FA, carg, bmin, CEp, bpack, CEp_out);
   else if (FA)
     launch_walk_max_p<false, true>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                   carry, FA, carg, bmin, CEp, bpack);
+                                   carry, FA, carg, bmin, CEp, bpack, CEp_out);
   else
     launch_walk_max_p<false, false>(B, nb, st, nbundles, bundles, gslot, bsrc, bdst, CE, VR, VRs, Q, Qs, qs, FMaxN,
-                                    carry, FA, carg, bmin, CEp, bpack);
+                                    carry, FA, carg, bmin, CEp, bpack, CEp_out);
 }
 
 // The base of a bundle's 24-bit cost codes from its least and largest cost bit patterns over columns q < Q, +inf
@@ -1148,45 +1171,6 @@ void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE,
   if (nbundles == 0) return;
   const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
   hipLaunchKernelGGL(k_bundle_min, dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, CE, Q, Qs, bmin, bpack);
-}
-
-// Lossless 24-bit coding of a bundle's costs for the plain walk (one wave per bundle, lanes as in the walk), with
-// the base k_bundle_min wrote to bpack[b] (pack_base; 0: the bundle keeps its FP32 rows and the walk reads CE).
-// Each cost of a column q < Q other than +inf codes as its bit pattern minus the base, +inf and columns q >= Q as 0.
-__global__ __launch_bounds__(kWalkBlock) void k_pack_costs(int64_t nbundles, const Bundle* __restrict__ bundles,
-                                                           const float* __restrict__ CE, int Q, int Qs,
-                                                           const unsigned* __restrict__ bpack,
-                                                           unsigned* __restrict__ CEp) {
-  const int L4 = Qs / 4, G = 64 / L4;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int g = lane / L4, j = lane - g * L4;
-  const int64_t b = (int64_t)blockIdx.x * (kWalkBlock / 64) + wave;
-  if (b >= nbundles) return;  // wave-uniform
-  const unsigned base = bpack[b];
-  if (base == 0u || g >= G) return;
-  const Bundle Bd = bundles[b];
-  const f4v* ce = reinterpret_cast<const f4v*>(CE) + (Bd.base + g) * L4 + j;
-  unsigned* cp = CEp + ((Bd.base + g) * L4 + j) * 3;
-  for (int s = 0; s < Bd.len; ++s) {
-    const f4v c = __builtin_nontemporal_load(ce + (int64_t)s * G * L4);
-    unsigned k[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const unsigned x = __float_as_uint(c[u]);
-      k[u] = (4 * j + u < Q && x != 0x7F800000u) ? x - base : 0u;
-    }
-    unsigned* o = cp + (int64_t)s * G * L4 * 3;
-    o[0] = k[0] | (k[1] << 24);
-    o[1] = (k[1] >> 8) | (k[2] << 16);
-    o[2] = (k[2] >> 16) | (k[3] << 8);
-  }
-}
-
-void launch_pack_costs(int64_t nbundles, const Bundle* bundles, const float* CE, int Q, int Qs, const unsigned* bpack,
-                       unsigned* CEp, hipStream_t st) {
-  if (nbundles == 0) return;
-  const int nb = (int)((nbundles + kWalkBlock / 64 - 1) / (kWalkBlock / 64));
-  hipLaunchKernelGGL(k_pack_costs, dim3(nb), dim3(kWalkBlock), 0, st, nbundles, bundles, CE, Q, Qs, bpack, CEp);
 }
 
 // Bundle layout from the CSC arrays: one block per bundle writes its rows (step-major, run-minor).  rowmap[k] is
