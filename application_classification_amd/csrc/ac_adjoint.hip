@@ -197,24 +197,23 @@ __global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const do
   }
 }
 
-// Sparse edge terms, gathered per source in CSR order so that every sum has one owner and a fixed order (the result
-// is reproducible bit for bit):
+// Sparse edge terms, with every sum owned by one lane in a fixed order (the result is reproducible bit for bit):
 //   FMax argmax: the in-edge row FA[v][q] of v = dst(row) receives -gE_F[v][q], and the row's source accumulates
-//                gVR[src][q] += gE_F[v][q];
-//   RMax argmax: the minimum-cost out-edge row AO[src][q] receives -gE_R[src][q].
-// A row's FMax term is added before its RMax term (each (row, q) is written by the one lane walking that edge).  A
-// source's gVR terms are summed in FP64 along its out-edges and added once; the pieces of a split source leave FP64
+//                gVR[src][q] += gE_F[v][q] (k_bwd_gather, per source in CSR order);
+//   RMax argmax: the minimum-cost out-edge row AO[src][q] receives -gE_R[src][q] (k_bwd_rterms, afterwards, so that a
+//                row's FMax term comes first).
+// A source's gVR terms are summed in FP64 along its out-edges and added once; the pieces of a split source leave FP64
 // partials in carry slots, which k_bwd_gather_fixup adds in piece order.  gVR == nullptr: the prep, whose edge terms
 // involve no VR.  Only nonzero edge gradients count (a max whose fill won routes nothing to its recorded row).
 //
-// One group of Qs/4 lanes walks one CSR run (lane j owning pattern edges 4j..4j+3), loading the (row, source) pairs,
-// then the rows' destinations, then the destinations' FA rows of a batch of edges together.
+// One group of Qs/4 lanes walks one CSR run (lane j owning pattern edges 4j..4j+3).  Whether a row is an FMax argmax
+// comes from its argmax mask (k_argbits), so an edge costs a 4-byte load; the RMax terms, which would need each
+// source's argmin rows the moment the walk reaches the source, are left to the separate pass.
 __global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const Run* __restrict__ runs,
                                                            const int2* __restrict__ ks, const int* __restrict__ kdst,
                                                            int Q, int Qs, const float* __restrict__ gEF,
-                                                           const float* __restrict__ gER,
                                                            const unsigned* __restrict__ argbits,
-                                                           const int* __restrict__ AO, float* __restrict__ gVR,
+                                                           float* __restrict__ gVR,
                                                            float* __restrict__ gCE, double* __restrict__ carry) {
   const int L4 = Qs / 4, G = 64 / L4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -224,8 +223,6 @@ __global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const 
   const Run R = runs[r];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   int cur = ks[R.kb].y;
-  int4 ao = reinterpret_cast<const int4*>(AO)[(int64_t)cur * L4 + j];
-  float4 gr = reinterpret_cast<const float4*>(gER)[(int64_t)cur * L4 + j];
   auto emit = [&](int src) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -239,47 +236,51 @@ __global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const 
   };
   // Lane j's four columns in the rows' argmax masks: word (4j) / 32, bits (4j) % 32 .. + 3.
   const int W = (Qs + 31) / 32, aw = (4 * j) >> 5, ab = (4 * j) & 31;
+  // Steps in batches of kBatch over two register sets used in turn: the pairs (ks), destinations (kdst) and argmax
+  // masks of a batch are three streams over the run's CSR positions, and batch t + 1 is in flight while batch t is
+  // processed.  Loads past the run's end are clamped to its last position (valid addresses) and never processed.
   constexpr int kBatch = 8;
-  for (int k0 = R.kb; k0 < R.ke; k0 += kBatch) {
-    int2 e[kBatch];
-    int v[kBatch];
-    unsigned fa[kBatch];
-    // The pairs and the destinations (kdst, per CSR position) load together; the rows' argmax masks next.
+  auto load_batch = [&](int k0, int2* e, int* v, unsigned* fa) __attribute__((always_inline)) {
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) e[u] = ks[k0 + u < R.ke ? k0 + u : R.ke - 1];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) v[u] = kdst[k0 + u < R.ke ? k0 + u : R.ke - 1];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) fa[u] = (argbits[(int64_t)e[u].x * W + aw] >> ab) & 0xFu;
+    for (int u = 0; u < kBatch; ++u) {
+      const int k = k0 + u < R.ke ? k0 + u : R.ke - 1;
+      e[u] = ks[k];
+      v[u] = kdst[k];
+      fa[u] = (argbits[(int64_t)k * W + aw] >> ab) & 0xFu;
+    }
+  };
+  auto process = [&](int k0, const int2* e, const int* v, const unsigned* fa) __attribute__((always_inline)) {
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if (k0 + u >= R.ke) break;
       if (e[u].y != cur) {
         emit(cur);
         cur = e[u].y;
-        ao = reinterpret_cast<const int4*>(AO)[(int64_t)cur * L4 + j];
-        gr = reinterpret_cast<const float4*>(gER)[(int64_t)cur * L4 + j];
       }
       const int row = e[u].x;
-      const int ar[4] = {ao.x, ao.y, ao.z, ao.w};
-      const float grv[4] = {gr.x, gr.y, gr.z, gr.w};
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int q = 4 * j + c;
-        if (q >= Q) continue;
-        const float gf = (fa[u] >> c) & 1u ? gEF[(int64_t)v[u] * Qs + q] : 0.f;
-        const float gq = ar[c] == row ? grv[c] : 0.f;
-        if (gf == 0.f && gq == 0.f) continue;
+        if (q >= Q || !((fa[u] >> c) & 1u)) continue;
+        const float gf = gEF[(int64_t)v[u] * Qs + q];
+        if (gf == 0.f) continue;
         float* o = gCE + (int64_t)row * Qs + q;
-        float t = *o;
-        if (gf != 0.f) {
-          t += -gf;
-          acc[c] += (double)gf;
-        }
-        if (gq != 0.f) t += -gq;
-        *o = t;
+        *o += -gf;
+        acc[c] += (double)gf;
       }
     }
+  };
+  int2 eA[kBatch], eB[kBatch];
+  int vA[kBatch], vB[kBatch];
+  unsigned fA[kBatch], fB[kBatch];
+  load_batch(R.kb, eA, vA, fA);
+  for (int k0 = R.kb;; k0 += 2 * kBatch) {
+    load_batch(k0 + kBatch, eB, vB, fB);  // batch t + 1 in flight while batch t is processed
+    process(k0, eA, vA, fA);
+    if (k0 + kBatch >= R.ke) break;
+    load_batch(k0 + 2 * kBatch, eA, vA, fA);
+    process(k0 + kBatch, eB, vB, fB);
+    if (k0 + 2 * kBatch >= R.ke) break;
   }
   if (R.slot >= 0) {
 #pragma unroll
@@ -647,47 +648,74 @@ void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const
   hipLaunchKernelGGL(k_bwd_in, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, S, MU, qs1, red4, gER, gVR, gVF, cD);
 }
 
-// argbits[row][q / 32] bit q % 32: row is the FMax argmax of column q of its destination (FA[v][q] == row) and that
-// edge term carries gradient (gEF[v][q] != 0), the only case in which the gather uses it.  FA rows are written only
+// argbits[k][q / 32] bit q % 32, for CSR position k = rowpos[row]: the edge's row is the FMax argmax of column q of
+// its destination (FA[v][q] == row) and that edge term carries gradient (gEF[v][q] != 0), the only case in which the
+// gather uses it.  Indexed by CSR position, the masks are read by the gather as a stream.  FA rows are written only
 // where the walk recorded a maximum (vertices with in-edges, bundles not pruned); a nonzero gEF implies one, and the
-// row is bounds-checked besides.  Each (v, q) sets one bit, with an atomic OR: exact and independent of order.  The
-// words must be zero beforehand.
+// row and its position are bounds-checked besides.  Each (v, q) sets one bit, with an atomic OR: exact and
+// independent of order.  The words must be zero beforehand.
 __global__ void k_argbits(int64_t n, int Q, int Qs, const int* __restrict__ FA, const float* __restrict__ gEF,
-                          int64_t brows, unsigned* __restrict__ argbits) {
+                          const int* __restrict__ rowpos, int64_t brows, int64_t m, unsigned* __restrict__ argbits) {
   const int W = (Qs + 31) / 32;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * Qs; i += (int64_t)gridDim.x * blockDim.x) {
     const int q = (int)(i % Qs);
     if (q >= Q || gEF[i] == 0.f) continue;
     const int fa = FA[i];
-    if (fa >= 0 && fa < brows) atomicOr(argbits + (int64_t)fa * W + (q >> 5), 1u << (q & 31));
+    if (fa < 0 || fa >= brows) continue;
+    const int k = rowpos[fa];
+    if (k >= 0 && k < m) atomicOr(argbits + (int64_t)k * W + (q >> 5), 1u << (q & 31));
   }
 }
 
-__global__ void k_csr_dst(int64_t m, const int2* __restrict__ ks, const int* __restrict__ bdst, int* __restrict__ out) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = bdst[ks[i].x];
+// The RMax edge terms: gCE[AO[src][q]][q] -= gER[src][q].  Each (src, q) owns its argmin row's column q (a row has one
+// source), so the pass needs no atomics; it runs after the FMax terms, so each (row, q) gets -gF then -gR, in that
+// order.  AO is +INT32_MAX where a source has no out-edge; rows are bounds-checked.
+__global__ void k_bwd_rterms(int64_t n, int Q, int Qs, const float* __restrict__ gER, const int* __restrict__ AO,
+                             int64_t brows, float* __restrict__ gCE) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * Qs; i += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(i % Qs);
+    if (q >= Q) continue;
+    const float g = gER[i];
+    if (g == 0.f) continue;
+    const int row = AO[i];
+    if (row >= 0 && row < brows) gCE[(int64_t)row * Qs + q] += -g;
+  }
 }
 
-void launch_csr_dst(int64_t m, const int2* ks, const int* bdst, int* out, hipStream_t st) {
+__global__ void k_csr_dst(int64_t m, const int2* __restrict__ ks, const int* __restrict__ bdst, int* __restrict__ out,
+                          int* __restrict__ rowpos) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    const int row = ks[i].x;
+    out[i] = bdst[row];
+    rowpos[row] = (int)i;
+  }
+}
+
+void launch_csr_dst(int64_t m, const int2* ks, const int* bdst, int* out, int* rowpos, int64_t brows,
+                    hipStream_t st) {
+  launch_fill_int(rowpos, brows, -1, st);  // padding rows have no CSR position
   if (m == 0) return;
   const int blocks = (int)std::min<int64_t>((m + 255) / 256, 1 << 16);
-  hipLaunchKernelGGL(k_csr_dst, dim3(blocks), dim3(256), 0, st, m, ks, bdst, out);
+  hipLaunchKernelGGL(k_csr_dst, dim3(blocks), dim3(256), 0, st, m, ks, bdst, out, rowpos);
 }
 
 void launch_bwd_gather(const BwdGraph& G, int64_t nruns, const Run* runs, int nsplit, const Split* splits,
                        const int2* ks, const int* kdst, const float* gEF, const float* gER, const int* FA, const int* AO,
-                       float* gVR, float* gCE, double* carry, unsigned* argbits, int64_t brows, hipStream_t st) {
+                       float* gVR, float* gCE, double* carry, unsigned* argbits, const int* rowpos, int64_t brows,
+                       int64_t m, hipStream_t st) {
   if (G.Q == 0 || nruns == 0) return;
   const int W = (G.Qs + 31) / 32;
-  launch_fill_int(reinterpret_cast<int*>(argbits), brows * W, 0, st);
+  launch_fill_int(reinterpret_cast<int*>(argbits), m * W, 0, st);
   const int64_t nq = G.n * G.Qs;
   hipLaunchKernelGGL(k_argbits, dim3((unsigned)std::min<int64_t>((nq + 255) / 256, 1 << 16)), dim3(256), 0, st, G.n,
-                     G.Q, G.Qs, FA, gEF, brows, argbits);
+                     G.Q, G.Qs, FA, gEF, rowpos, brows, m, argbits);
   hipLaunchKernelGGL(k_bwd_gather, dim3(walk_blocks(nruns, G.Qs)), dim3(kWalkBlock), 0, st, nruns, runs, ks, kdst, G.Q,
-                     G.Qs, gEF, gER, argbits, AO, gVR, gCE, carry);
+                     G.Qs, gEF, argbits, gVR, gCE, carry);
   if (gVR && nsplit > 0)
     hipLaunchKernelGGL(k_bwd_gather_fixup, dim3((unsigned)((nsplit + 3) / 4)), dim3(256), 0, st, nsplit, splits, carry,
                        G.Q, G.Qs, gVR);
+  hipLaunchKernelGGL(k_bwd_rterms, dim3((unsigned)std::min<int64_t>((nq + 255) / 256, 1 << 16)), dim3(256), 0, st,
+                     G.n, G.Q, G.Qs, gER, AO, brows, gCE);
 }
 
 void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, float* gFX, float* gRX,

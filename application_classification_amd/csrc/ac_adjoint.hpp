@@ -39,15 +39,19 @@ void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int
                       const float* gRX, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st);
 void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const QStat* qs1, const double* red4,
                    const float* gER, float* gVR, float* gVF, double* cD, hipStream_t st);
-// Sparse edge terms of one iteration (or of the prep, gVR == nullptr), gathered over the CSR runs `runs` (pairs ks =
+// Sparse edge terms of one iteration (or of the prep, gVR == nullptr): the FMax terms gathered over the CSR runs
+// `runs`, then the RMax terms from AO / gER (pairs ks =
 // {bundle row, source}, kdst = the destination at each CSR position); carry: one [Qs] FP64 row per carry slot of the
-// runs; argbits: brows * ceil(Qs / 32) words of scratch for the rows' argmax masks (built from FA first).
-// Deterministic.
+// runs; argbits: m * ceil(Qs / 32) words of scratch for the argmax masks of the m CSR positions (built from FA
+// first, through rowpos: the CSR position of each of the brows bundle rows).  Deterministic.
 void launch_bwd_gather(const BwdGraph& G, int64_t nruns, const Run* runs, int nsplit, const Split* splits,
                        const int2* ks, const int* kdst, const float* gEF, const float* gER, const int* FA, const int* AO,
-                       float* gVR, float* gCE, double* carry, unsigned* argbits, int64_t brows, hipStream_t st);
-// out[k] = bdst[ks[k].x]: the destination of the edge at each CSR position.
-void launch_csr_dst(int64_t m, const int2* ks, const int* bdst, int* out, hipStream_t st);
+                       float* gVR, float* gCE, double* carry, unsigned* argbits, const int* rowpos, int64_t brows,
+                       int64_t m, hipStream_t st);
+// out[k] = bdst[ks[k].x]: the destination of the edge at each of the m CSR positions; rowpos[row] = the CSR position
+// of each bundle row (-1: padding).
+void launch_csr_dst(int64_t m, const int2* ks, const int* bdst, int* out, int* rowpos, int64_t brows,
+                    hipStream_t st);
 void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, float* gFX, float* gRX,
                    hipStream_t st);
 void launch_bwd_prep_fill(const BwdGraph& G, const double* red4, const int* am0, float* gMU, hipStream_t st);

@@ -216,6 +216,7 @@ struct ac_ctx {
     DevBuf<int> gslot, bsrc, bdst, beid;
     DevBuf<int2> csr_kb;
     DevBuf<int> csr_dst;  // the destination at each CSR position (built on the first backward pass)
+    DevBuf<int> rowpos;   // the CSR position of each bundle row, -1 for padding (idem)
     int64_t nbundles = 0, brows = 0;
   };
   std::vector<std::unique_ptr<BundleSet>> bsets;
@@ -1701,10 +1702,11 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, ctx->gCV.alloc(np));
   for (DevBuf<float>* b : {&ctx->gFX, &ctx->gRX, &ctx->gEF, &ctx->gER, &ctx->gVR, &ctx->gVF}) HIPCHK(ctx, b->alloc(nq));
   HIPCHK(ctx, ctx->gCE.alloc((size_t)std::max<int64_t>(1, brows) * Qs));
-  HIPCHK(ctx, ctx->argbits.alloc((size_t)std::max<int64_t>(1, brows) * ((Qs + 31) / 32)));
+  HIPCHK(ctx, ctx->argbits.alloc((size_t)std::max<int64_t>(1, ctx->de) * ((Qs + 31) / 32)));
   if (ctx->bs->csr_dst.n != (size_t)std::max<int64_t>(1, ctx->de)) {  // once per bundle layout
     HIPCHK(ctx, ctx->bs->csr_dst.alloc((size_t)std::max<int64_t>(1, ctx->de)));
-    ac::launch_csr_dst(ctx->de, ctx->bs->csr_kb.p, ctx->bs->bdst.p, ctx->bs->csr_dst.p, st);
+    HIPCHK(ctx, ctx->bs->rowpos.alloc((size_t)std::max<int64_t>(1, brows)));
+    ac::launch_csr_dst(ctx->de, ctx->bs->csr_kb.p, ctx->bs->bdst.p, ctx->bs->csr_dst.p, ctx->bs->rowpos.p, brows, st);
   }
   HIPCHK(ctx, ctx->cD.alloc(nq));
   HIPCHK(ctx, ctx->MUd.alloc(np));
@@ -1763,7 +1765,8 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
       ac::launch_bwd_in(G, S0, ctx->MUd.p, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, st);
       ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p,
                             ctx->bs->csr_kb.p, ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gER.p, ctx->tapeFA.p + s * nq,
-                            ctx->AO.p, ctx->gVR.p, ctx->gCE.p, ctx->carry_out.p, ctx->argbits.p, brows, st);
+                            ctx->AO.p, ctx->gVR.p, ctx->gCE.p, ctx->carry_out.p, ctx->argbits.p, ctx->bs->rowpos.p,
+                            brows, ctx->de, st);
       ac::launch_bwd_mu(G, ctx->gVR.p, ctx->gVF.p, ctx->gMU.p, ctx->gFX.p, ctx->gRX.p, st);
     }
   }
@@ -1776,7 +1779,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
                          ctx->red4.p, st);
     ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p, ctx->bs->csr_kb.p,
                           ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gER.p, ctx->FA0.p, ctx->AO.p, nullptr, ctx->gCE.p,
-                          ctx->carry_out.p, ctx->argbits.p, brows, st);
+                          ctx->carry_out.p, ctx->argbits.p, ctx->bs->rowpos.p, brows, ctx->de, st);
     ac::launch_bwd_prep_fill(G, ctx->red4.p, ctx->am0.p, ctx->gMU.p, st);
     std::vector<double> red4((size_t)4 * Q);
     std::vector<QStat> qs0(Q);
