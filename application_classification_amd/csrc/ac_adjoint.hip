@@ -120,11 +120,13 @@ __global__ __launch_bounds__(256) void k_bwd_out(int64_t total, int P, const dou
   }
 }
 
-// Max-subgradient routing of FMax / RMax of state S (thread per (v, q), q fixed per thread).  gF = gFX + gZ[pd]
-// (with_gz), gR = gRX + gZ[ps].  The edge term takes the gradient where it is strictly above the fill.  Per-block
-// partials part[b][4][Q]: sum gE_F, sum gE_R, fill gradient of fillF, of fillR.
+// Max-subgradient routing of FMax / RMax of state S (thread per (v, q), q fixed per thread).  gF = -gVF + gZ[pd]
+// (with_gz), gR = -gVR + gZ[ps], where gVF / gVR are the next state's (the step processed before this one; zero at the
+// first): the gradients of this state's FMax / RMax through the next step's inputs.  The edge term takes the
+// gradient where it is strictly above the fill.  Per-block partials part[b][4][Q]: sum gE_F, sum gE_R, fill gradient
+// of fillF, of fillR.
 __global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const float* __restrict__ gZ, int with_gz,
-                                                   const float* __restrict__ gFX, const float* __restrict__ gRX,
+                                                   const float* __restrict__ gVF, const float* __restrict__ gVR,
                                                    float* __restrict__ gEF, float* __restrict__ gER,
                                                    double* __restrict__ part) {
   __shared__ double red[256][4];
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const
     const double fillF = S.qs[q].fillF, fillR = S.qs[q].fillR;
     for (int64_t v = (int64_t)blockIdx.x * RQ + t / Q; v < G.n; v += (int64_t)gridDim.x * RQ) {
       const int64_t i = v * G.Qs + q;
-      float gF = gFX[i], gR = gRX[i];
+      float gF = -gVF[i], gR = -gVR[i];
       if (with_gz) {
         gF += gZ[v * G.P + pdq];
         gR += gZ[v * G.P + psq];
@@ -326,11 +328,10 @@ __global__ __launch_bounds__(256) void k_bwd_gather_fixup(int nsplit, const Spli
   }
 }
 
-// gMU[v][p] = sum over the incidences of p (ascending q) of gVF (q ends at p) or gVR (q starts at p); then
-// gFX = -gVF, gRX = -gVR (the gradients of the input state's FMax / RMax).
+// gMU[v][p] = sum over the incidences of p (ascending q) of gVF (q ends at p) or gVR (q starts at p).  The input
+// state's FMax / RMax gradients are -gVF / -gVR, which k_bwd_route of the next step back reads directly.
 __global__ __launch_bounds__(256) void k_bwd_mu(BwdGraph G, const float* __restrict__ gVR, const float* __restrict__ gVF,
-                                                float* __restrict__ gMU, float* __restrict__ gFX,
-                                                float* __restrict__ gRX) {
+                                                float* __restrict__ gMU) {
   const int64_t total = G.n * G.P;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = i / G.P;
@@ -342,11 +343,6 @@ __global__ __launch_bounds__(256) void k_bwd_mu(BwdGraph G, const float* __restr
       a += (code & 1) ? gVR[o] : gVF[o];
     }
     gMU[i] = a;
-  }
-  const int64_t tq = G.n * G.Qs;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tq; i += (int64_t)gridDim.x * blockDim.x) {
-    gFX[i] = -gVF[i];
-    gRX[i] = -gVR[i];
   }
 }
 
@@ -635,10 +631,10 @@ void launch_bwd_out(const BwdGraph& G, const double* MU, const double* colsum, f
   hipLaunchKernelGGL(k_bwd_out, dim3(grid_for(G.n * G.P)), dim3(256), 0, st, G.n * G.P, G.P, MU, colsum, gMU, gCV);
 }
 
-void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gFX,
-                      const float* gRX, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st) {
+void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gVF,
+                      const float* gVR, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st) {
   if (G.Q == 0) return;
-  hipLaunchKernelGGL(k_bwd_route, dim3(nparts), dim3(256), 0, st, G, S, gZ, with_gz, gFX, gRX, gEF, gER, part);
+  hipLaunchKernelGGL(k_bwd_route, dim3(nparts), dim3(256), 0, st, G, S, gZ, with_gz, gVF, gVR, gEF, gER, part);
   hipLaunchKernelGGL(k_sum_parts, dim3(sum_blocks(4 * G.Q)), dim3(256), 0, st, nparts, 4 * G.Q, part, red4);
 }
 
@@ -718,10 +714,8 @@ void launch_bwd_gather(const BwdGraph& G, int64_t nruns, const Run* runs, int ns
                      G.n, G.Q, G.Qs, gER, AO, brows, gCE);
 }
 
-void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, float* gFX, float* gRX,
-                   hipStream_t st) {
-  hipLaunchKernelGGL(k_bwd_mu, dim3(grid_for(G.n * std::max(G.P, G.Qs))), dim3(256), 0, st, G, gVR, gVF, gMU, gFX,
-                     gRX);
+void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, hipStream_t st) {
+  hipLaunchKernelGGL(k_bwd_mu, dim3(grid_for(G.n * G.P)), dim3(256), 0, st, G, gVR, gVF, gMU);
 }
 
 void launch_bwd_prep_fill(const BwdGraph& G, const double* red4, const int* am0, float* gMU, hipStream_t st) {

@@ -35,8 +35,10 @@ void launch_fill_int(int* p, int64_t count, int value, hipStream_t st);
 void launch_sum_parts(int nparts, int C, const double* part, double* out, hipStream_t st);
 void launch_colsum(int64_t n, int C, const float* X, double* part, int nparts, double* out, hipStream_t st);
 void launch_bwd_out(const BwdGraph& G, const double* MU, const double* colsum, float* gMU, float* gCV, hipStream_t st);
-void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gFX,
-                      const float* gRX, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st);
+// gVF / gVR: those of the next state (the previous step back; zeros at the first), whose negations are this state's
+// FMax / RMax gradients.
+void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gVF,
+                      const float* gVR, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st);
 void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const QStat* qs1, const double* red4,
                    const float* gER, float* gVR, float* gVF, double* cD, hipStream_t st);
 // Sparse edge terms of one iteration (or of the prep, gVR == nullptr): the FMax terms gathered over the CSR runs
@@ -52,8 +54,7 @@ void launch_bwd_gather(const BwdGraph& G, int64_t nruns, const Run* runs, int ns
 // of each bundle row (-1: padding).
 void launch_csr_dst(int64_t m, const int2* ks, const int* bdst, int* out, int* rowpos, int64_t brows,
                     hipStream_t st);
-void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, float* gFX, float* gRX,
-                   hipStream_t st);
+void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, hipStream_t st);
 void launch_bwd_prep_fill(const BwdGraph& G, const double* red4, const int* am0, float* gMU, hipStream_t st);
 void launch_argmin_out(int64_t m, int Q, int Qs, const int2* ks, const float* CE, const float* minCEo, int* AO,
                        hipStream_t st);

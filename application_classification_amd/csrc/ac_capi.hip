@@ -259,7 +259,7 @@ struct ac_ctx {
   DevBuf<QStat> ckQ, sgQ;
   DevBuf<Lse2> ckP, sgP;
   // Adjoint work buffers.
-  DevBuf<float> gMUin, gMU, gCV, gFX, gRX, gEF, gER, gVR, gVF, gCE, apv;
+  DevBuf<float> gMUin, gMU, gCV, gEF, gER, gVR, gVF, gCE, apv;
   DevBuf<unsigned> argbits;  // per bundle row, the pattern edges it is the FMax argmax of (backward scratch)
   DevBuf<double> cD, colsum, bpart, red4, cq, opart, osum, MUd;
   DevBuf<int> api;
@@ -1700,7 +1700,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, ctx->gMU.alloc(np));
   ac::launch_gather_rows(ctx->dn, ctx->P, ctx->d_perm.p, ctx->gMUin.p, ctx->gMU.p, st);
   HIPCHK(ctx, ctx->gCV.alloc(np));
-  for (DevBuf<float>* b : {&ctx->gFX, &ctx->gRX, &ctx->gEF, &ctx->gER, &ctx->gVR, &ctx->gVF}) HIPCHK(ctx, b->alloc(nq));
+  for (DevBuf<float>* b : {&ctx->gEF, &ctx->gER, &ctx->gVR, &ctx->gVF}) HIPCHK(ctx, b->alloc(nq));
   HIPCHK(ctx, ctx->gCE.alloc((size_t)std::max<int64_t>(1, brows) * Qs));
   HIPCHK(ctx, ctx->argbits.alloc((size_t)std::max<int64_t>(1, ctx->de) * ((Qs + 31) / 32)));
   if (ctx->bs->csr_dst.n != (size_t)std::max<int64_t>(1, ctx->de)) {  // once per bundle layout
@@ -1720,8 +1720,9 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, ctx->sgP.alloc((size_t)(K + 1) * P));
   HIPCHK(ctx, ctx->tapeFA.alloc((size_t)K * nq));
   HIPCHK(ctx, hipMemsetAsync(ctx->gCV.p, 0, np * sizeof(float), st));
-  HIPCHK(ctx, hipMemsetAsync(ctx->gFX.p, 0, nq * sizeof(float), st));
-  HIPCHK(ctx, hipMemsetAsync(ctx->gRX.p, 0, nq * sizeof(float), st));
+  // gVF / gVR of the state after the last one: zero (k_bwd_route of the first step back reads their negations).
+  HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->gVR.p, 0, nq * sizeof(float), st));
   HIPCHK(ctx, hipMemsetAsync(ctx->gCE.p, 0, ctx->gCE.n * sizeof(float), st));
   HIPCHK(ctx, hipMemsetAsync(ctx->cD.p, 0, nq * sizeof(double), st));
 
@@ -1756,7 +1757,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
       if (!have_mu) launch_mu(ctx, S1, st);
       ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
       ac::launch_bwd_out(G, ctx->MUd.p, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
-      ac::launch_bwd_route(G, S1, ctx->gMU.p, 1, ctx->gFX.p, ctx->gRX.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
+      ac::launch_bwd_route(G, S1, ctx->gMU.p, 1, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
                            ctx->red4.p, st);
       launch_mu(ctx, S0, st);  // MU of the input state, reused by the next step back
       have_mu = true;
@@ -1767,7 +1768,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
                             ctx->bs->csr_kb.p, ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gER.p, ctx->tapeFA.p + s * nq,
                             ctx->AO.p, ctx->gVR.p, ctx->gCE.p, ctx->carry_out.p, ctx->argbits.p, ctx->bs->rowpos.p,
                             brows, ctx->de, st);
-      ac::launch_bwd_mu(G, ctx->gVR.p, ctx->gVF.p, ctx->gMU.p, ctx->gFX.p, ctx->gRX.p, st);
+      ac::launch_bwd_mu(G, ctx->gVR.p, ctx->gVF.p, ctx->gMU.p, st);
     }
   }
   // Prep (src/main.cu:107-148): FMax0 / RMax0 from -CE normalised by LSE0 = log sum_e exp(-CE), filled from
@@ -1775,7 +1776,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   const ac::BwdState S0{ctx->ckF.p, ctx->ckV.p, ctx->ckQ.p, ctx->ckP.p, 1};
   std::vector<double> cq(Q1, 0.0);
   if (Q > 0) {
-    ac::launch_bwd_route(G, S0, nullptr, 0, ctx->gFX.p, ctx->gRX.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
+    ac::launch_bwd_route(G, S0, nullptr, 0, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
                          ctx->red4.p, st);
     ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p, ctx->bs->csr_kb.p,
                           ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gER.p, ctx->FA0.p, ctx->AO.p, nullptr, ctx->gCE.p,

@@ -133,33 +133,44 @@ __global__ __launch_bounds__(kCdTile) void k_cdist_rows(int64_t nrows, const int
 }
 
 // FP64 matrix cores (v_mfma_f64_16x16x4f64), GEMM form: d^2 = |x|^2 + |p|^2 - 2 x.p with the weights folded in
-// (x.p -> sum w x p, |x|^2 -> sum w x^2, |p|^2 -> sum w p^2).  One wave computes a tile of 16 rows against 16
-// patterns per instruction sequence; the operand k index is permuted so that each lane's A fragment is one contiguous
-// chunk of its row: lane l = i + 16 kk holds row i's features [kk D4, (kk + 1) D4), and step s of the product takes
-// feature kk D4 + s from both operands (the sum over k is order-free in exact arithmetic and the accumulation is FP64).
-// D[4 r + l / 16][l % 16] is register r of lane l.  Pairs with d^2 < 2^-10 (|x|^2 + |p|^2) lose digits to
-// cancellation and are recomputed directly as sum w (x - p)^2 in FP64, so the result keeps the direct form's
-// accuracy (relative error of d^2 below 1e-11 elsewhere).
+// (x.p -> sum w x p, |x|^2 -> sum w x^2, |p|^2 -> sum w p^2).  One wave takes 16 feature rows at a time and forms
+// D = C + A B with the patterns as A (16 pattern columns by 4 features per instruction, scaled by -2) and the rows as
+// B (4 features by 16 rows), accumulating from C = |x|^2 + |p|^2, so D is d^2 itself.  The operand k index is
+// permuted so that each lane's B fragment is one contiguous chunk of its row: lane l = j + 16 kk holds row j's
+// features [kk D4, (kk + 1) D4), and step s of the product takes feature kk D4 + s from both operands (the sum over k
+// is order-free in exact arithmetic and the accumulation is FP64).  Register r of lane l holds D[4 r + l / 16][l % 16]:
+// A row m = 4 r + g is staged as pattern column 4 g + r of its tile, so lane j + 16 g ends with four consecutive
+// columns 4 g .. 4 g + 3 of its own row j.  Its |x|^2, row id and output address are then its own (no lane
+// exchange), and the four distances leave as one 16-byte store.  Pairs with d^2 below 2^-10 (|x|^2 + |p|^2) (with
+// the largest |p|^2 of the lane's four columns, a superset) lose digits to cancellation and are recomputed directly
+// as sum w (x - p)^2 in FP64, so the result keeps the direct form's accuracy (relative error of d^2 below 1e-11
+// elsewhere).
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-// LDS position of pattern operand e = c * 4 D4 + d (pattern column c = 16 pt + i, feature d = D4 kk + s2): the
-// layout [pt][s2][kk][i], so that product step s2 of tile pt reads 64 consecutive doubles across the wave (lane
-// 16 kk + i), one conflict-free LDS access.  In the layout of patm itself the 64 lanes' operands lie 128 or 512 bytes
-// apart, all in one bank pair, and each access is serialised.
+// Pattern column of MFMA row m within a tile (and the reverse: the map is its own inverse).
+__device__ __forceinline__ int mfma_col(int m) { return 4 * (m & 3) + (m >> 2); }
+
+// LDS position of pattern operand e = c * 4 D4 + d (pattern column c = 16 pt + mfma_col(m), feature d = D4 kk + s2):
+// the layout [pt][s2][kk][m], so that product step s2 of tile pt reads 64 consecutive doubles across the wave (lane
+// 16 kk + m), one conflict-free LDS access.  In the layout of patm itself the 64 lanes' operands lie 128 or 512
+// bytes apart, all in one bank pair, and each access is serialised.
 __device__ __forceinline__ int lds_operand(int e, int D4) {
   const int c = e / (4 * D4), d = e - c * (4 * D4);
-  const int pt = c >> 4, i = c & 15, kk = d / D4, s2 = d - kk * D4;
-  return ((pt * D4 + s2) * 4 + kk) * 16 + i;
+  const int pt = c >> 4, m = mfma_col(c & 15), kk = d / D4, s2 = d - kk * D4;
+  return ((pt * D4 + s2) * 4 + kk) * 16 + m;
 }
 
 // Correctly rounded square root of x >= 0 in FP32 (x >= 2^-104): the hardware root corrected by sqrt_rn_fix, eight
 // VALU instructions against the library's general sequence.
 __device__ __forceinline__ float sqrt_rn(float x) { return sqrt_rn_fix(x, __builtin_amdgcn_sqrtf(x)); }
 
-// Up to 64 features the kernel fits three waves per SIMD (at most 168 registers, no spills), which keeps the matrix
-// core busy while other waves run their epilogues and loads; wider rows keep the whole register file.  kW: weighted
-// metric (a template parameter, so the unweighted loop carries no per-feature select).
-template <int D4, bool kIdx, bool kW>
+// Up to 64 features the kernel fits three waves per SIMD (at most 168 registers), which keeps the matrix core busy
+// while other waves run their epilogues and loads; wider rows keep the whole register file.  kW: weighted metric (a
+// template parameter, so the unweighted loop carries no per-feature select).  kVec: the row width is exactly 4 D4
+// (e.g. 32 or 64 features), so each lane's 16-float chunk loads as float4s with no feature past dim; otherwise each
+// feature loads on its own at a clamped index (a separate instantiation, whose per-lane offsets would otherwise
+// occupy registers in the float4 loop too).
+template <int D4, bool kIdx, bool kW, bool kVec>
 __global__ __launch_bounds__(256, D4 <= 16 ? 3 : 1) void k_cdist_mfma(int64_t nrows, const int* __restrict__ idx,
                                                     const float* __restrict__ feats, int dim,
                                                     const double* __restrict__ patm, const double* __restrict__ pp,
@@ -178,11 +189,13 @@ __global__ __launch_bounds__(256, D4 <= 16 ? 3 : 1) void k_cdist_mfma(int64_t nr
     for (int d = threadIdx.x; d < 4 * D4; d += 256) wsh[d] = d < dim ? w[d] : 0.0;
   for (int c = threadIdx.x; c < npt * 16; c += 256) psh[c] = pp[c];
   __syncthreads();
-  // Whole 16-float chunks load as float4s when the row width is exactly 4 D4 (e.g. 32 or 64 features); this lane's
-  // chunk then has no feature past dim.  Otherwise the features past dim are zeroed at conversion (lim of them
-  // exist).  Rows that do not exist are not zeroed: their loads read row 0 and their outputs are +inf regardless.
-  const bool vec = dim == 4 * D4 && (D4 % 4) == 0;
+  // Without kVec the features past dim are zeroed at conversion (lim of this lane's chunk exist).  Rows that do not
+  // exist are not zeroed: their loads read row 0 and their outputs are +inf regardless.
+  static_assert(D4 % 4 == 0, "chunks of whole float4s");
+  constexpr bool vec = kVec;
   const int lim = dim - kk * D4;
+  // Four columns leave as one 16-byte store when every group of four starts a 16-byte aligned address.
+  const bool vst = (Kstore & 3) == 0 && (out_stride & 3) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
   // Feature row of lane row i in the tile at r (-1: past the end or a padding row), in two steps so that the id load
   // stays in flight: row_raw loads idx at a valid (clamped) row, row_of interprets the loaded value later.
   auto row_raw = [&](int64_t r) -> int {
@@ -198,7 +211,7 @@ __global__ __launch_bounds__(256, D4 <= 16 ? 3 : 1) void k_cdist_mfma(int64_t nr
   // the values are converted.
   auto load_x = [&](int64_t src, float* x) {
     const float* xs = feats + (src >= 0 ? src : 0) * dim;
-    if (vec) {
+    if constexpr (vec) {
       const f4v* x4 = reinterpret_cast<const f4v*>(xs + kk * D4);
 #pragma unroll
       for (int u = 0; u < D4 / 4; ++u) {
@@ -215,7 +228,7 @@ __global__ __launch_bounds__(256, D4 <= 16 ? 3 : 1) void k_cdist_mfma(int64_t nr
     }
   };
   auto convert = [&](const float* x, double* a) {
-    if (vec) {
+    if constexpr (vec) {
 #pragma unroll
       for (int s2 = 0; s2 < D4; ++s2) a[s2] = (double)x[s2];
     } else {
@@ -253,61 +266,70 @@ __global__ __launch_bounds__(256, D4 <= 16 ? 3 : 1) void k_cdist_mfma(int64_t nr
 #pragma unroll
     for (int s2 = 0; s2 < D4; ++s2) xx = fma(kW ? wsh[kk * D4 + s2] * a[s2] : a[s2], a[s2], xx);
     xx += __shfl_xor(xx, 16, 64);
-    xx += __shfl_xor(xx, 32, 64);  // lanes i, i + 16, ...: |x|^2 of row i
-    // This lane's four output rows 4 r + kk: their |x|^2, feature rows, and where their outputs go.
-    double xxr[4];
-    int srcr[4];
-    float* orow[4];
-    bool rok[4];
+    xx += __shfl_xor(xx, 32, 64);  // every lane: |x|^2 of its row i
+    const bool live = src >= 0, rok = r0 + i < nrows;
+    float* orow = out + (rok ? r0 + i : 0) * out_stride;
+    // C = |x|^2 + |p|^2 of this lane's four columns 16 t + 4 kk + r of tile t, and the largest |p|^2 among them.
+    auto init = [&](int t, d4v& acc, double& pm) {
+      const double* pc = psh + t * 16 + 4 * kk;
+      pm = fmax(fmax(pc[0], pc[1]), fmax(pc[2], pc[3]));
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      xxr[r] = __shfl(xx, 4 * r + kk, 64);
-      srcr[r] = __shfl((int)src, 4 * r + kk, 64);
-      const int64_t rr = r0 + 4 * r + kk;
-      rok[r] = rr < nrows;
-      orow[r] = out + (rok[r] ? rr : 0) * out_stride;
-    }
-    // Pattern tiles in pairs: two independent accumulator chains interleave on the matrix core.
-    for (int pt = 0; pt < npt; pt += 2) {
-      const bool two = pt + 1 < npt;
-      const double* bp0 = bsh + (pt * D4 * 4 + kk) * 16 + i;
-      const double* bp1 = bsh + ((two ? pt + 1 : pt) * D4 * 4 + kk) * 16 + i;
-      d4v acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int s2 = 0; s2 < D4; ++s2) {
-        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], bp0[s2 * 64], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s2], bp1[s2 * 64], acc1, 0, 0, 0);
-      }
-      // Epilogue, branch-free but for the rare cancellation: d^2 in FP64, its square root in FP32 (sqrt_rn of the
-      // rounded d^2, within 0.75 ulp of the distance, since FP64 VALU work here competes with the products for the
-      // same FP64 rate).  Column c1 exists only with a second tile (two).
-      const int c0 = pt * 16 + i, c1 = c0 + 16;
-      const double p0 = psh[c0], p1 = psh[two ? c1 : c0];
-      double d2[2][4];
+      for (int r = 0; r < 4; ++r) acc[r] = xx + pc[r];
+    };
+    // Epilogue of tile t, branch-free but for the rare cancellation: the square root in FP32 (sqrt_rn of the rounded
+    // d^2, within 0.75 ulp of the distance, since FP64 VALU work here competes with the products for the same FP64
+    // rate), +inf for rows that do not exist, 0 in the padding columns [K, Kstore).
+    auto emit = [&](int t, const d4v& acc, double pm) {
+      const int cb = t * 16 + 4 * kk;
+      const double thr = 0x1p-10 * (xx + pm);
+      double d2[4];
       bool fix = false;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double ref0 = xxr[r] + p0, ref1 = xxr[r] + p1;
-        d2[0][r] = ref0 - 2.0 * acc0[r];
-        d2[1][r] = ref1 - 2.0 * acc1[r];
-        const bool live = srcr[r] >= 0;
-        fix |= live && c0 < K && d2[0][r] < 0x1p-10 * ref0;
-        fix |= live && two && c1 < K && d2[1][r] < 0x1p-10 * ref1;
+        d2[r] = acc[r];
+        fix |= live && cb + r < K && d2[r] < thr;
       }
       if (__any(fix)) {  // cancellation: recompute those pairs directly
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const bool live = srcr[r] >= 0;
-          if (live && c0 < K && d2[0][r] < 0x1p-10 * (xxr[r] + p0)) d2[0][r] = direct(srcr[r], c0);
-          if (live && two && c1 < K && d2[1][r] < 0x1p-10 * (xxr[r] + p1)) d2[1][r] = direct(srcr[r], c1);
-        }
+        for (int r = 0; r < 4; ++r)
+          if (live && cb + r < K && d2[r] < thr) d2[r] = direct(src, cb + r);
       }
+      // The root is taken unconditionally (of 0 in the padding columns and rows that do not exist), so no lane
+      // branches around it.
+      float v[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float v0 = srcr[r] < 0 ? INFINITY : c0 < K ? sqrt_rn((float)d2[0][r]) : 0.f;
-        const float v1 = srcr[r] < 0 ? INFINITY : c1 < K ? sqrt_rn((float)d2[1][r]) : 0.f;
-        if (rok[r] && c0 < Kstore) orow[r][c0] = v0;
-        if (rok[r] && two && c1 < Kstore) orow[r][c1] = v1;
+        const float s = sqrt_rn(live && cb + r < K ? (float)d2[r] : 0.f);
+        v[r] = live ? s : INFINITY;
+      }
+      if (vst) {
+        if (rok && cb < Kstore) *reinterpret_cast<f4v*>(orow + cb) = f4v{v[0], v[1], v[2], v[3]};
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (rok && cb + r < Kstore) orow[cb + r] = v[r];
+      }
+    };
+    // Pattern tiles in pairs: two independent accumulator chains interleave on the matrix core.
+    for (int pt = 0; pt < npt; pt += 2) {
+      const double* bp0 = bsh + (pt * D4 * 4 + kk) * 16 + i;
+      d4v acc0, acc1;
+      double pm0, pm1;
+      init(pt, acc0, pm0);
+      if (pt + 1 < npt) {
+        const double* bp1 = bp0 + D4 * 64;
+        init(pt + 1, acc1, pm1);
+#pragma unroll
+        for (int s2 = 0; s2 < D4; ++s2) {
+          acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(bp0[s2 * 64], a[s2], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(bp1[s2 * 64], a[s2], acc1, 0, 0, 0);
+        }
+        emit(pt, acc0, pm0);
+        emit(pt + 1, acc1, pm1);
+      } else {
+#pragma unroll
+        for (int s2 = 0; s2 < D4; ++s2) acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(bp0[s2 * 64], a[s2], acc0, 0, 0, 0);
+        emit(pt, acc0, pm0);
       }
     }
     if (more) convert(xr, a);
@@ -319,7 +341,8 @@ __global__ __launch_bounds__(256, D4 <= 16 ? 3 : 1) void k_cdist_mfma(int64_t nr
   }
 }
 
-// MFMA operands of the patterns: patm[c][4 D4] = w_d p_c[d] (zero past dim or past K), pp[c] = sum_d w_d p_c[d]^2.
+// MFMA operands of the patterns: patm[c][4 D4] = -2 w_d p_c[d] (zero past dim or past K), pp[c] = sum_d w_d p_c[d]^2.
+// The factor -2 is exact.
 __global__ void k_build_patm(int K, int Kp16, int dim, int D4, const double* __restrict__ pat,
                              const double* __restrict__ w, double* __restrict__ patm, double* __restrict__ pp) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -333,7 +356,7 @@ __global__ void k_build_patm(int K, int Kp16, int dim, int D4, const double* __r
       v = wd * pv;
       s2 += v * pv;
     }
-    patm[(int64_t)c * (4 * D4) + d] = v;
+    patm[(int64_t)c * (4 * D4) + d] = -2.0 * v;
   }
   pp[c] = s2;
 }
@@ -385,20 +408,26 @@ void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, co
     hipLaunchKernelGGL(k_build_patm, dim3((Kp16 + 63) / 64), dim3(64), 0, st, K, Kp16, dim, D4, pat, w, patm, pp);
     const int blocks = (int)std::min<int64_t>((nrows + 63) / 64, 8192);
     const size_t lds = ((size_t)(Kp16 + 1) * 4 * D4 + Kp16) * sizeof(double);  // operands, weights, norms
-#define AC_CD_MFMA(N)                                                                                            \
-  case N:                                                                                                        \
-    if (idx && w)                                                                                                \
-      hipLaunchKernelGGL((k_cdist_mfma<N, true, true>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats, dim, \
-                         patm, pp, w, pat, K, Kstore, out, out_stride);                                          \
-    else if (idx)                                                                                                \
-      hipLaunchKernelGGL((k_cdist_mfma<N, true, false>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats,     \
-                         dim, patm, pp, w, pat, K, Kstore, out, out_stride);                                     \
-    else if (w)                                                                                                  \
-      hipLaunchKernelGGL((k_cdist_mfma<N, false, true>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats,     \
-                         dim, patm, pp, w, pat, K, Kstore, out, out_stride);                                     \
-    else                                                                                                         \
-      hipLaunchKernelGGL((k_cdist_mfma<N, false, false>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats,    \
-                         dim, patm, pp, w, pat, K, Kstore, out, out_stride);                                     \
+#define AC_CD_MFMA_V(N, V)                                                                                        \
+  if (idx && w)                                                                                                   \
+    hipLaunchKernelGGL((k_cdist_mfma<N, true, true, V>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats, dim, \
+                       patm, pp, w, pat, K, Kstore, out, out_stride);                                             \
+  else if (idx)                                                                                                   \
+    hipLaunchKernelGGL((k_cdist_mfma<N, true, false, V>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats,     \
+                       dim, patm, pp, w, pat, K, Kstore, out, out_stride);                                        \
+  else if (w)                                                                                                     \
+    hipLaunchKernelGGL((k_cdist_mfma<N, false, true, V>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats,     \
+                       dim, patm, pp, w, pat, K, Kstore, out, out_stride);                                        \
+  else                                                                                                            \
+    hipLaunchKernelGGL((k_cdist_mfma<N, false, false, V>), dim3(blocks), dim3(256), lds, st, nrows, idx, feats,    \
+                       dim, patm, pp, w, pat, K, Kstore, out, out_stride);
+#define AC_CD_MFMA(N)                \
+  case N:                            \
+    if (dim == 4 * N) {              \
+      AC_CD_MFMA_V(N, true)          \
+    } else {                         \
+      AC_CD_MFMA_V(N, false)         \
+    }                                \
     return;
     switch (D4) {
       AC_CD_MFMA(4)
@@ -409,6 +438,7 @@ void launch_cdist(int64_t nrows, const int* idx, const float* feats, int dim, co
         break;
     }
 #undef AC_CD_MFMA
+#undef AC_CD_MFMA_V
   }
   if (dim <= 255) {
     // At most 64 KB of LDS: 128 rows up to 127 features, fewer rows beyond.
