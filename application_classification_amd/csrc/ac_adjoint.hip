@@ -29,6 +29,7 @@ namespace ac {
 namespace {
 
 constexpr double kInf = __builtin_huge_val();
+typedef float f4v __attribute__((ext_vector_type(4)));
 
 // MU[v][p] of a state: (Z - m) - l with Z = -CV + (first ? 0 : the incidence terms of p in ascending q, the FMax
 // term before the RMax term), exactly as the vertex pass evaluates it.
@@ -125,10 +126,15 @@ __global__ __launch_bounds__(256) void k_bwd_out(int64_t total, int P, const dou
 // first): the gradients of this state's FMax / RMax through the next step's inputs.  The edge term takes the
 // gradient where it is strictly above the fill.  Per-block partials part[b][4][Q]: sum gE_F, sum gE_R, fill gradient
 // of fillF, of fillR.
+//   The argmax masks: where the FMax edge term carries gradient (eF, gF != 0), bit q of the mask word of the CSR
+//   position of its recorded row FA[v][q] is set with an atomic OR (exact and order-free; the words are zero before).
+//   FA rows exist only where the walk recorded a maximum (vertices with in-edges, bundles not pruned); a nonzero
+//   gradient implies one, and the row and its position are bounds-checked besides.  gERsum (nullable) accumulates
+//   gE_R per (v, q) over the steps: the RMax rows AO are the same in every step, so their edge terms are applied once.
 __global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const float* __restrict__ gZ, int with_gz,
                                                    const float* __restrict__ gVF, const float* __restrict__ gVR,
-                                                   float* __restrict__ gEF, float* __restrict__ gER,
-                                                   double* __restrict__ part) {
+                                                   float* __restrict__ gEF, float* __restrict__ gER, ArgMasks am,
+                                                   float* __restrict__ gERsum, double* __restrict__ part) {
   __shared__ double red[256][4];
   const int t = threadIdx.x, Q = G.Q, RQ = 256 / Q;
   const bool active = t < RQ * Q;
@@ -148,6 +154,12 @@ __global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const
       const bool eR = rn_of(G, S, v, q) > fillR;
       gEF[i] = eF ? gF : 0.f;
       gER[i] = eR ? gR : 0.f;
+      if (am.bits && eF && gF != 0.f) {
+        const int fa = am.FA[i];
+        const int k = fa >= 0 && fa < am.brows ? am.rowpos[fa] : -1;
+        if (k >= 0 && k < am.m) atomicOr(am.bits + (int64_t)k * am.W + (q >> 5), 1u << (q & 31));
+      }
+      if (gERsum && eR && gR != 0.f) gERsum[i] += gR;
       if (eF) sF += gF; else fF += gF;
       if (eR) sR += gR; else fR += gR;
     }
@@ -202,20 +214,37 @@ __global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const do
 // Sparse edge terms, with every sum owned by one lane in a fixed order (the result is reproducible bit for bit):
 //   FMax argmax: the in-edge row FA[v][q] of v = dst(row) receives -gE_F[v][q], and the row's source accumulates
 //                gVR[src][q] += gE_F[v][q] (k_bwd_gather, per source in CSR order);
-//   RMax argmax: the minimum-cost out-edge row AO[src][q] receives -gE_R[src][q] (k_bwd_rterms, afterwards, so that a
-//                row's FMax term comes first).
+//   RMax argmax: the minimum-cost out-edge row AO[src][q] receives -gE_R[src][q], summed over the steps
+//                (k_bwd_rterms, once, after the last step: AO is the same in every step).
 // A source's gVR terms are summed in FP64 along its out-edges and added once; the pieces of a split source leave FP64
 // partials in carry slots, which k_bwd_gather_fixup adds in piece order.  gVR == nullptr: the prep, whose edge terms
 // involve no VR.  Only nonzero edge gradients count (a max whose fill won routes nothing to its recorded row).
 //
 // One group of Qs/4 lanes walks one CSR run (lane j owning pattern edges 4j..4j+3).  Whether a row is an FMax argmax
-// comes from its argmax mask (k_argbits), so an edge costs a 4-byte load; the RMax terms, which would need each
-// source's argmin rows the moment the walk reaches the source, are left to the separate pass.
+// comes from its argmax mask (set by k_bwd_route), so an edge costs a 4-byte load; the lane that owns a mask word
+// clears it once read, leaving the masks zero for the next use.  Every (row, q) receives at most one FMax term per
+// call and every (source, q) one sum, so both are added with non-returning atomic adds: exact as plain additions of
+// the same two values in the same order, and no lane waits for a read of the old value.
+//
+// The walk is pipelined three batches deep over three register sets used in turn: the CSR pairs, destinations and
+// mask words of batch t + 2 load while the gradient rows of batch t + 1's masked destinations load and batch t is
+// taken.  A gradient row is loaded only where the lane's mask bits are set (else a fixed valid address), so the
+// number of loads per batch is fixed.  Loads past the run's end are clamped to its last position (valid addresses)
+// and never taken.
+namespace {
+constexpr int kGatherBatch = 4;
+struct GatherSet {
+  int2 e[kGatherBatch];     // {bundle row, source}
+  int v[kGatherBatch];      // destination
+  unsigned w[kGatherBatch];  // the mask word holding this lane's four bits
+  f4v gf[kGatherBatch];     // gE_F[v][4j .. 4j + 3] where a bit is set
+};
+}  // namespace
+
 __global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const Run* __restrict__ runs,
                                                            const int2* __restrict__ ks, const int* __restrict__ kdst,
                                                            int Q, int Qs, const float* __restrict__ gEF,
-                                                           const unsigned* __restrict__ argbits,
-                                                           float* __restrict__ gVR,
+                                                           unsigned* __restrict__ argbits, float* __restrict__ gVR,
                                                            float* __restrict__ gCE, double* __restrict__ carry) {
   const int L4 = Qs / 4, G = 64 / L4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -229,60 +258,71 @@ __global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const 
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int q = 4 * j + c;
-      if (gVR && q < Q && acc[c] != 0.0) {
-        float* o = gVR + (int64_t)src * Qs + q;
-        *o = (float)((double)*o + acc[c]);
-      }
+      if (gVR && q < Q && acc[c] != 0.0) unsafeAtomicAdd(gVR + (int64_t)src * Qs + q, (float)acc[c]);
       acc[c] = 0.0;
     }
   };
-  // Lane j's four columns in the rows' argmax masks: word (4j) / 32, bits (4j) % 32 .. + 3.
+  // Lane j's four columns in the rows' argmax masks: word (4j) / 32, bits (4j) % 32 .. + 3.  The lane whose bits
+  // start its word clears the word.
   const int W = (Qs + 31) / 32, aw = (4 * j) >> 5, ab = (4 * j) & 31;
-  // Steps in batches of kBatch over two register sets used in turn: the pairs (ks), destinations (kdst) and argmax
-  // masks of a batch are three streams over the run's CSR positions, and batch t + 1 is in flight while batch t is
-  // processed.  Loads past the run's end are clamped to its last position (valid addresses) and never processed.
-  constexpr int kBatch = 8;
-  auto load_batch = [&](int k0, int2* e, int* v, unsigned* fa) __attribute__((always_inline)) {
+  const bool owner = ab == 0;
+  const f4v* gE4 = reinterpret_cast<const f4v*>(gEF);
+  auto load_ids = [&](int k0, GatherSet& S) __attribute__((always_inline)) {
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
+    for (int u = 0; u < kGatherBatch; ++u) {
       const int k = k0 + u < R.ke ? k0 + u : R.ke - 1;
-      e[u] = ks[k];
-      v[u] = kdst[k];
-      fa[u] = (argbits[(int64_t)k * W + aw] >> ab) & 0xFu;
+      S.e[u] = ks[k];
+      S.v[u] = kdst[k];
+      S.w[u] = argbits[(int64_t)k * W + aw];
     }
   };
-  auto process = [&](int k0, const int2* e, const int* v, const unsigned* fa) __attribute__((always_inline)) {
+  auto load_grad = [&](int k0, GatherSet& S) __attribute__((always_inline)) {
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
+    for (int u = 0; u < kGatherBatch; ++u) {
+      const bool any = ((S.w[u] >> ab) & 0xFu) != 0u;
+      S.gf[u] = gE4[any ? (int64_t)S.v[u] * L4 + j : 0];
+      if (owner && k0 + u < R.ke && S.w[u] != 0u) argbits[(int64_t)(k0 + u) * W + aw] = 0u;
+    }
+  };
+  auto take = [&](int k0, const GatherSet& S) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < kGatherBatch; ++u) {
       if (k0 + u >= R.ke) break;
-      if (e[u].y != cur) {
+      if (S.e[u].y != cur) {
         emit(cur);
-        cur = e[u].y;
+        cur = S.e[u].y;
       }
-      const int row = e[u].x;
+      const unsigned f = (S.w[u] >> ab) & 0xFu;
+      if (f == 0u) continue;
+      float* orow = gCE + (int64_t)S.e[u].x * Qs + 4 * j;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const int q = 4 * j + c;
-        if (q >= Q || !((fa[u] >> c) & 1u)) continue;
-        const float gf = gEF[(int64_t)v[u] * Qs + q];
-        if (gf == 0.f) continue;
-        float* o = gCE + (int64_t)row * Qs + q;
-        *o += -gf;
-        acc[c] += (double)gf;
+        const float gf = S.gf[u][c];
+        if (4 * j + c < Q && ((f >> c) & 1u) && gf != 0.f) {
+          unsafeAtomicAdd(orow + c, -gf);
+          acc[c] += (double)gf;
+        }
       }
     }
   };
-  int2 eA[kBatch], eB[kBatch];
-  int vA[kBatch], vB[kBatch];
-  unsigned fA[kBatch], fB[kBatch];
-  load_batch(R.kb, eA, vA, fA);
-  for (int k0 = R.kb;; k0 += 2 * kBatch) {
-    load_batch(k0 + kBatch, eB, vB, fB);  // batch t + 1 in flight while batch t is processed
-    process(k0, eA, vA, fA);
-    if (k0 + kBatch >= R.ke) break;
-    load_batch(k0 + 2 * kBatch, eA, vA, fA);
-    process(k0 + kBatch, eB, vB, fB);
-    if (k0 + 2 * kBatch >= R.ke) break;
+  GatherSet S0, S1, S2;
+  constexpr int B = kGatherBatch;
+  load_ids(R.kb, S0);
+  load_ids(R.kb + B, S1);
+  load_grad(R.kb, S0);
+  for (int k0 = R.kb;; k0 += 3 * B) {
+    load_ids(k0 + 2 * B, S2);
+    load_grad(k0 + B, S1);
+    take(k0, S0);
+    if (k0 + B >= R.ke) break;
+    load_ids(k0 + 3 * B, S0);
+    load_grad(k0 + 2 * B, S2);
+    take(k0 + B, S1);
+    if (k0 + 2 * B >= R.ke) break;
+    load_ids(k0 + 4 * B, S1);
+    load_grad(k0 + 3 * B, S0);
+    take(k0 + 2 * B, S2);
+    if (k0 + 3 * B >= R.ke) break;
   }
   if (R.slot >= 0) {
 #pragma unroll
@@ -632,9 +672,11 @@ void launch_bwd_out(const BwdGraph& G, const double* MU, const double* colsum, f
 }
 
 void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gVF,
-                      const float* gVR, float* gEF, float* gER, double* part, int nparts, double* red4, hipStream_t st) {
+                      const float* gVR, float* gEF, float* gER, const ArgMasks& am, float* gERsum, double* part,
+                      int nparts, double* red4, hipStream_t st) {
   if (G.Q == 0) return;
-  hipLaunchKernelGGL(k_bwd_route, dim3(nparts), dim3(256), 0, st, G, S, gZ, with_gz, gVF, gVR, gEF, gER, part);
+  hipLaunchKernelGGL(k_bwd_route, dim3(nparts), dim3(256), 0, st, G, S, gZ, with_gz, gVF, gVR, gEF, gER, am, gERsum,
+                     part);
   hipLaunchKernelGGL(k_sum_parts, dim3(sum_blocks(4 * G.Q)), dim3(256), 0, st, nparts, 4 * G.Q, part, red4);
 }
 
@@ -644,28 +686,8 @@ void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const
   hipLaunchKernelGGL(k_bwd_in, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, S, MU, qs1, red4, gER, gVR, gVF, cD);
 }
 
-// argbits[k][q / 32] bit q % 32, for CSR position k = rowpos[row]: the edge's row is the FMax argmax of column q of
-// its destination (FA[v][q] == row) and that edge term carries gradient (gEF[v][q] != 0), the only case in which the
-// gather uses it.  Indexed by CSR position, the masks are read by the gather as a stream.  FA rows are written only
-// where the walk recorded a maximum (vertices with in-edges, bundles not pruned); a nonzero gEF implies one, and the
-// row and its position are bounds-checked besides.  Each (v, q) sets one bit, with an atomic OR: exact and
-// independent of order.  The words must be zero beforehand.
-__global__ void k_argbits(int64_t n, int Q, int Qs, const int* __restrict__ FA, const float* __restrict__ gEF,
-                          const int* __restrict__ rowpos, int64_t brows, int64_t m, unsigned* __restrict__ argbits) {
-  const int W = (Qs + 31) / 32;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * Qs; i += (int64_t)gridDim.x * blockDim.x) {
-    const int q = (int)(i % Qs);
-    if (q >= Q || gEF[i] == 0.f) continue;
-    const int fa = FA[i];
-    if (fa < 0 || fa >= brows) continue;
-    const int k = rowpos[fa];
-    if (k >= 0 && k < m) atomicOr(argbits + (int64_t)k * W + (q >> 5), 1u << (q & 31));
-  }
-}
-
-// The RMax edge terms: gCE[AO[src][q]][q] -= gER[src][q].  Each (src, q) owns its argmin row's column q (a row has one
-// source), so the pass needs no atomics; it runs after the FMax terms, so each (row, q) gets -gF then -gR, in that
-// order.  AO is +INT32_MAX where a source has no out-edge; rows are bounds-checked.
+// RMax edge terms, once after the last step: the minimum-cost out-edge row AO[v][q] receives -gERsum[v][q] (the sum
+// over the steps of gE_R[v][q]).  Each (row, q) belongs to one (v, q).
 __global__ void k_bwd_rterms(int64_t n, int Q, int Qs, const float* __restrict__ gER, const int* __restrict__ AO,
                              int64_t brows, float* __restrict__ gCE) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * Qs; i += (int64_t)gridDim.x * blockDim.x) {
@@ -696,22 +718,22 @@ void launch_csr_dst(int64_t m, const int2* ks, const int* bdst, int* out, int* r
 }
 
 void launch_bwd_gather(const BwdGraph& G, int64_t nruns, const Run* runs, int nsplit, const Split* splits,
-                       const int2* ks, const int* kdst, const float* gEF, const float* gER, const int* FA, const int* AO,
-                       float* gVR, float* gCE, double* carry, unsigned* argbits, const int* rowpos, int64_t brows,
-                       int64_t m, hipStream_t st) {
+                       const int2* ks, const int* kdst, const float* gEF, float* gVR, float* gCE, double* carry,
+                       unsigned* argbits, hipStream_t st) {
   if (G.Q == 0 || nruns == 0) return;
-  const int W = (G.Qs + 31) / 32;
-  launch_fill_int(reinterpret_cast<int*>(argbits), m * W, 0, st);
-  const int64_t nq = G.n * G.Qs;
-  hipLaunchKernelGGL(k_argbits, dim3((unsigned)std::min<int64_t>((nq + 255) / 256, 1 << 16)), dim3(256), 0, st, G.n,
-                     G.Q, G.Qs, FA, gEF, rowpos, brows, m, argbits);
   hipLaunchKernelGGL(k_bwd_gather, dim3(walk_blocks(nruns, G.Qs)), dim3(kWalkBlock), 0, st, nruns, runs, ks, kdst, G.Q,
                      G.Qs, gEF, argbits, gVR, gCE, carry);
   if (gVR && nsplit > 0)
     hipLaunchKernelGGL(k_bwd_gather_fixup, dim3((unsigned)((nsplit + 3) / 4)), dim3(256), 0, st, nsplit, splits, carry,
                        G.Q, G.Qs, gVR);
+}
+
+void launch_bwd_rterms(const BwdGraph& G, const float* gERsum, const int* AO, int64_t brows, float* gCE,
+                       hipStream_t st) {
+  if (G.Q == 0) return;
+  const int64_t nq = G.n * G.Qs;
   hipLaunchKernelGGL(k_bwd_rterms, dim3((unsigned)std::min<int64_t>((nq + 255) / 256, 1 << 16)), dim3(256), 0, st,
-                     G.n, G.Q, G.Qs, gER, AO, brows, gCE);
+                     G.n, G.Q, G.Qs, gERsum, AO, brows, gCE);
 }
 
 void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, hipStream_t st) {

@@ -260,7 +260,8 @@ struct ac_ctx {
   DevBuf<Lse2> ckP, sgP;
   // Adjoint work buffers.
   DevBuf<float> gMUin, gMU, gCV, gEF, gER, gVR, gVF, gCE, apv;
-  DevBuf<unsigned> argbits;  // per bundle row, the pattern edges it is the FMax argmax of (backward scratch)
+  DevBuf<unsigned> argbits;  // per CSR position, the pattern edges its row is the FMax argmax of (backward scratch)
+  DevBuf<float> gERsum;      // per (vertex, pattern edge), gE_R summed over the steps back
   DevBuf<double> cD, colsum, bpart, red4, cq, opart, osum, MUd;
   DevBuf<int> api;
 
@@ -1702,7 +1703,9 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, ctx->gCV.alloc(np));
   for (DevBuf<float>* b : {&ctx->gEF, &ctx->gER, &ctx->gVR, &ctx->gVF}) HIPCHK(ctx, b->alloc(nq));
   HIPCHK(ctx, ctx->gCE.alloc((size_t)std::max<int64_t>(1, brows) * Qs));
-  HIPCHK(ctx, ctx->argbits.alloc((size_t)std::max<int64_t>(1, ctx->de) * ((Qs + 31) / 32)));
+  const int W = (Qs + 31) / 32;
+  HIPCHK(ctx, ctx->argbits.alloc((size_t)std::max<int64_t>(1, ctx->de) * W));
+  HIPCHK(ctx, ctx->gERsum.alloc(nq));
   if (ctx->bs->csr_dst.n != (size_t)std::max<int64_t>(1, ctx->de)) {  // once per bundle layout
     HIPCHK(ctx, ctx->bs->csr_dst.alloc((size_t)std::max<int64_t>(1, ctx->de)));
     HIPCHK(ctx, ctx->bs->rowpos.alloc((size_t)std::max<int64_t>(1, brows)));
@@ -1725,6 +1728,9 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, hipMemsetAsync(ctx->gVR.p, 0, nq * sizeof(float), st));
   HIPCHK(ctx, hipMemsetAsync(ctx->gCE.p, 0, ctx->gCE.n * sizeof(float), st));
   HIPCHK(ctx, hipMemsetAsync(ctx->cD.p, 0, nq * sizeof(double), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->gERsum.p, 0, nq * sizeof(float), st));
+  // The argmax masks start at zero; every gather clears the words it reads.
+  HIPCHK(ctx, hipMemsetAsync(ctx->argbits.p, 0, ctx->argbits.n * sizeof(unsigned), st));
 
   ac::BwdGraph G{n, P, Q, Qs, ctx->CV.p, ctx->minCEo.p, ctx->lsO.p, ctx->d_ps.p, ctx->d_pd.p, ctx->d_inc_off.p,
                  ctx->d_inc.p};
@@ -1757,17 +1763,17 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
       if (!have_mu) launch_mu(ctx, S1, st);
       ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
       ac::launch_bwd_out(G, ctx->MUd.p, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
-      ac::launch_bwd_route(G, S1, ctx->gMU.p, 1, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
-                           ctx->red4.p, st);
+      const ac::ArgMasks am{ctx->tapeFA.p + s * nq, ctx->bs->rowpos.p, brows, ctx->de, W, ctx->argbits.p};
+      ac::launch_bwd_route(G, S1, ctx->gMU.p, 1, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, am, ctx->gERsum.p,
+                           ctx->bpart.p, nparts, ctx->red4.p, st);
       launch_mu(ctx, S0, st);  // MU of the input state, reused by the next step back
       have_mu = true;
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVR.p, 0, nq * sizeof(float), st));
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
       ac::launch_bwd_in(G, S0, ctx->MUd.p, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, st);
       ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p,
-                            ctx->bs->csr_kb.p, ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gER.p, ctx->tapeFA.p + s * nq,
-                            ctx->AO.p, ctx->gVR.p, ctx->gCE.p, ctx->carry_out.p, ctx->argbits.p, ctx->bs->rowpos.p,
-                            brows, ctx->de, st);
+                            ctx->bs->csr_kb.p, ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gVR.p, ctx->gCE.p,
+                            ctx->carry_out.p, ctx->argbits.p, st);
       ac::launch_bwd_mu(G, ctx->gVR.p, ctx->gVF.p, ctx->gMU.p, st);
     }
   }
@@ -1776,11 +1782,12 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   const ac::BwdState S0{ctx->ckF.p, ctx->ckV.p, ctx->ckQ.p, ctx->ckP.p, 1};
   std::vector<double> cq(Q1, 0.0);
   if (Q > 0) {
-    ac::launch_bwd_route(G, S0, nullptr, 0, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, ctx->bpart.p, nparts,
-                         ctx->red4.p, st);
+    const ac::ArgMasks am{ctx->FA0.p, ctx->bs->rowpos.p, brows, ctx->de, W, ctx->argbits.p};
+    ac::launch_bwd_route(G, S0, nullptr, 0, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, am, ctx->gERsum.p,
+                         ctx->bpart.p, nparts, ctx->red4.p, st);
     ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p, ctx->bs->csr_kb.p,
-                          ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gER.p, ctx->FA0.p, ctx->AO.p, nullptr, ctx->gCE.p,
-                          ctx->carry_out.p, ctx->argbits.p, ctx->bs->rowpos.p, brows, ctx->de, st);
+                          ctx->bs->csr_dst.p, ctx->gEF.p, nullptr, ctx->gCE.p, ctx->carry_out.p, ctx->argbits.p, st);
+    ac::launch_bwd_rterms(G, ctx->gERsum.p, ctx->AO.p, brows, ctx->gCE.p, st);
     ac::launch_bwd_prep_fill(G, ctx->red4.p, ctx->am0.p, ctx->gMU.p, st);
     std::vector<double> red4((size_t)4 * Q);
     std::vector<QStat> qs0(Q);
