@@ -125,21 +125,23 @@ __global__ __launch_bounds__(256) void k_bwd_out(int64_t total, int P, const dou
 // (with_gz), gR = -gVR + gZ[ps], where gVF / gVR are the next state's (the step processed before this one; zero at the
 // first): the gradients of this state's FMax / RMax through the next step's inputs.  The edge term takes the
 // gradient where it is strictly above the fill.  Per-block partials part[b][4][Q]: sum gE_F, sum gE_R, fill gradient
-// of fillF, of fillR.
-//   The argmax masks: where the FMax edge term carries gradient (eF, gF != 0), bit q of the mask word of the CSR
-//   position of its recorded row FA[v][q] is set with an atomic OR (exact and order-free; the words are zero before).
-//   FA rows exist only where the walk recorded a maximum (vertices with in-edges, bundles not pruned); a nonzero
-//   gradient implies one, and the row and its position are bounds-checked besides.  gERsum (nullable) accumulates
-//   gE_R per (v, q) over the steps: the RMax rows AO are the same in every step, so their edge terms are applied once.
+// of fillF, of fillR.  gmax (nullable) receives the largest |gE_F| (as float bits, an atomic maximum: exact and
+// order-free), which sets the fixed-point scale of the FMax edge terms' source sums.  gERsum (nullable) accumulates
+// gE_R per (v, q) over the steps: the RMax rows AO are the same in every step, so their edge terms are applied once.
 __global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const float* __restrict__ gZ, int with_gz,
                                                    const float* __restrict__ gVF, const float* __restrict__ gVR,
-                                                   float* __restrict__ gEF, float* __restrict__ gER, ArgMasks am,
-                                                   float* __restrict__ gERsum, double* __restrict__ part) {
+                                                   float* __restrict__ gEF, float* __restrict__ gER,
+                                                   unsigned* __restrict__ gmax, float* __restrict__ gERsum,
+                                                   double* __restrict__ part) {
   __shared__ double red[256][4];
+  __shared__ unsigned bmax;
   const int t = threadIdx.x, Q = G.Q, RQ = 256 / Q;
+  if (t == 0) bmax = 0u;
+  __syncthreads();
   const bool active = t < RQ * Q;
   const int q = active ? t % Q : 0;
   double sF = 0.0, sR = 0.0, fF = 0.0, fR = 0.0;
+  unsigned mx = 0u;
   if (active) {
     const int pdq = G.pd[q], psq = G.ps[q];
     const double fillF = S.qs[q].fillF, fillR = S.qs[q].fillR;
@@ -154,21 +156,19 @@ __global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const
       const bool eR = rn_of(G, S, v, q) > fillR;
       gEF[i] = eF ? gF : 0.f;
       gER[i] = eR ? gR : 0.f;
-      if (am.bits && eF && gF != 0.f) {
-        const int fa = am.FA[i];
-        const int k = fa >= 0 && fa < am.brows ? am.rowpos[fa] : -1;
-        if (k >= 0 && k < am.m) atomicOr(am.bits + (int64_t)k * am.W + (q >> 5), 1u << (q & 31));
-      }
+      if (eF) mx = max(mx, __float_as_uint(fabsf(gF)));
       if (gERsum && eR && gR != 0.f) gERsum[i] += gR;
       if (eF) sF += gF; else fF += gF;
       if (eR) sR += gR; else fR += gR;
     }
   }
+  if (gmax && mx) atomicMax(&bmax, mx);
   red[t][0] = sF;
   red[t][1] = sR;
   red[t][2] = fF;
   red[t][3] = fR;
   __syncthreads();
+  if (gmax && t == 0 && bmax) atomicMax(gmax, bmax);
   if (t < Q)
     for (int k = 0; k < 4; ++k) {
       double a = 0.0;
@@ -177,16 +177,55 @@ __global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const
     }
 }
 
+// The fixed-point scale of the FMax edge terms' source sums: 2^(39 - e) for the largest |gE_F| in [2^e, 2^(e+1)), so
+// every term is below 2^40 in magnitude and a sum of up to 2^23 of them stays inside int64.  A power of two, so
+// scaling is exact and only the rounding to an integer (at 2^-40 of the largest term) loses anything.
+__device__ __forceinline__ int fix_exponent(unsigned gmax_bits) {
+  const int e = (int)((gmax_bits >> 23) & 0xFFu) - 127;  // subnormal maxima: e = -127, a larger scale than needed
+  return 39 - max(-126, min(e, 127));
+}
+
+// The FMax edge term of (v, q), where it carries gradient gE = gEF[v][q] != 0: its recorded row FA[v][q] receives
+// -gE in gCE (one term per (row, q) and step, so the float atomic adds the same two values in the same order as a
+// plain update), and the row's source accumulates gE in the fixed-point sum gVRfix[src][q] (integer additions: exact,
+// so the sum does not depend on their order).  FA rows exist only where the walk recorded a maximum (vertices with
+// in-edges, bundles not pruned); a nonzero gradient implies one, and the row is bounds-checked besides.
+__device__ __forceinline__ void fmax_edge_term(int64_t o, int q, int Qs, float ge, const int* __restrict__ FA,
+                                               const int* __restrict__ bsrc, int64_t brows, float* __restrict__ gCE,
+                                               unsigned long long* __restrict__ gVRfix, double fscale) {
+  const int row = FA[o];
+  if (row < 0 || row >= brows) return;
+  unsafeAtomicAdd(gCE + (int64_t)row * Qs + q, -ge);
+  if (gVRfix) {
+    const int64_t src = bsrc[row] & ~kNewVertex;
+    atomicAdd(gVRfix + src * Qs + q, (unsigned long long)__double2ll_rn((double)ge * fscale));
+  }
+}
+
+// The prep's FMax edge terms (no source sums: the prep state has no VR).
+__global__ void k_bwd_fscatter(int64_t n, int Q, int Qs, const float* __restrict__ gEF, const int* __restrict__ FA,
+                               const int* __restrict__ bsrc, int64_t brows, float* __restrict__ gCE) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * Qs; i += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(i % Qs);
+    if (q >= Q) continue;
+    const float ge = gEF[i];
+    if (ge != 0.f) fmax_edge_term(i, q, Qs, ge, FA, bsrc, brows, gCE, nullptr, 0.0);
+  }
+}
+
 // Input side of one iteration (state S = S_i, scalars qs1 of S_{i+1}; red4 = {S_F, S_R, gfillF, gfillR}[Q]):
 //   gVR = -S_F exp(VR - LSE_FE - minCEo + lsO) + [v = argmax VR] gfillF
 //   gVF = gE_R - S_R exp(VF - LSE_RE - minCEo + lsO) + [v = argmax VF] gfillR
 //   cD += S_F exp(VR - LSE_FE) + S_R exp(VF - LSE_RE)            (vertices with out-edges; FP64)
+// and the FMax edge terms of the step (fmax_edge_term; FE: the step's gE_F, argmax rows and fixed-point scale), whose
+// source sums go to gVRfix and are added to gVR by the vertex kernel that follows.
 __global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const double* __restrict__ MU,
                                                 const QStat* __restrict__ qs1, const double* __restrict__ red4,
                                                 const float* __restrict__ gER, float* __restrict__ gVR,
-                                                float* __restrict__ gVF, double* __restrict__ cD) {
+                                                float* __restrict__ gVF, double* __restrict__ cD, FmaxTerms FE) {
   const int Q = G.Q;
   const int64_t total = G.n * Q;
+  const double fscale = ldexp(1.0, fix_exponent(*FE.gmax));
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = i / Q;
     const int q = (int)(i - v * Q);
@@ -208,182 +247,64 @@ __global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const do
     if (v == s1.vR) gvf += red4[3 * Q + q];
     gVR[o] = (float)gvr;
     gVF[o] = (float)gvf;
+    const float ge = FE.gEF[o];
+    if (ge != 0.f) fmax_edge_term(o, q, G.Qs, ge, FE.FA, FE.bsrc, FE.brows, FE.gCE, FE.gVRfix, fscale);
   }
 }
 
-// Sparse edge terms, with every sum owned by one lane in a fixed order (the result is reproducible bit for bit):
-//   FMax argmax: the in-edge row FA[v][q] of v = dst(row) receives -gE_F[v][q], and the row's source accumulates
-//                gVR[src][q] += gE_F[v][q] (k_bwd_gather, per source in CSR order);
-//   RMax argmax: the minimum-cost out-edge row AO[src][q] receives -gE_R[src][q], summed over the steps
-//                (k_bwd_rterms, once, after the last step: AO is the same in every step).
-// A source's gVR terms are summed in FP64 along its out-edges and added once; the pieces of a split source leave FP64
-// partials in carry slots, which k_bwd_gather_fixup adds in piece order.  gVR == nullptr: the prep, whose edge terms
-// involve no VR.  Only nonzero edge gradients count (a max whose fill won routes nothing to its recorded row).
-//
-// One group of Qs/4 lanes walks one CSR run (lane j owning pattern edges 4j..4j+3).  Whether a row is an FMax argmax
-// comes from its argmax mask (set by k_bwd_route), so an edge costs a 4-byte load; the lane that owns a mask word
-// clears it once read, leaving the masks zero for the next use.  Every (row, q) receives at most one FMax term per
-// call and every (source, q) one sum, so both are added with non-returning atomic adds: exact as plain additions of
-// the same two values in the same order, and no lane waits for a read of the old value.
-//
-// The walk is pipelined three batches deep over three register sets used in turn: the CSR pairs, destinations and
-// mask words of batch t + 2 load while the gradient rows of batch t + 1's masked destinations load and batch t is
-// taken.  A gradient row is loaded only where the lane's mask bits are set (else a fixed valid address), so the
-// number of loads per batch is fixed.  Loads past the run's end are clamped to its last position (valid addresses)
-// and never taken.
-namespace {
-constexpr int kGatherBatch = 4;
-struct GatherSet {
-  int2 e[kGatherBatch];     // {bundle row, source}
-  int v[kGatherBatch];      // destination
-  unsigned w[kGatherBatch];  // the mask word holding this lane's four bits
-  f4v gf[kGatherBatch];     // gE_F[v][4j .. 4j + 3] where a bit is set
-};
-}  // namespace
-
-__global__ __launch_bounds__(kWalkBlock) void k_bwd_gather(int64_t nruns, const Run* __restrict__ runs,
-                                                           const int2* __restrict__ ks, const int* __restrict__ kdst,
-                                                           int Q, int Qs, const float* __restrict__ gEF,
-                                                           unsigned* __restrict__ argbits, float* __restrict__ gVR,
-                                                           float* __restrict__ gCE, double* __restrict__ carry) {
-  const int L4 = Qs / 4, G = 64 / L4;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int g = lane / L4, j = lane - g * L4;
-  const int64_t r = ((int64_t)blockIdx.x * (kWalkBlock / 64) + wave) * G + g;
-  if (g >= G || r >= nruns) return;
-  const Run R = runs[r];
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  int cur = ks[R.kb].y;
-  auto emit = [&](int src) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int q = 4 * j + c;
-      if (gVR && q < Q && acc[c] != 0.0) unsafeAtomicAdd(gVR + (int64_t)src * Qs + q, (float)acc[c]);
-      acc[c] = 0.0;
-    }
-  };
-  // Lane j's four columns in the rows' argmax masks: word (4j) / 32, bits (4j) % 32 .. + 3.  The lane whose bits
-  // start its word clears the word.
-  const int W = (Qs + 31) / 32, aw = (4 * j) >> 5, ab = (4 * j) & 31;
-  const bool owner = ab == 0;
-  const f4v* gE4 = reinterpret_cast<const f4v*>(gEF);
-  auto load_ids = [&](int k0, GatherSet& S) __attribute__((always_inline)) {
-#pragma unroll
-    for (int u = 0; u < kGatherBatch; ++u) {
-      const int k = k0 + u < R.ke ? k0 + u : R.ke - 1;
-      S.e[u] = ks[k];
-      S.v[u] = kdst[k];
-      S.w[u] = argbits[(int64_t)k * W + aw];
-    }
-  };
-  auto load_grad = [&](int k0, GatherSet& S) __attribute__((always_inline)) {
-#pragma unroll
-    for (int u = 0; u < kGatherBatch; ++u) {
-      const bool any = ((S.w[u] >> ab) & 0xFu) != 0u;
-      S.gf[u] = gE4[any ? (int64_t)S.v[u] * L4 + j : 0];
-      if (owner && k0 + u < R.ke && S.w[u] != 0u) argbits[(int64_t)(k0 + u) * W + aw] = 0u;
-    }
-  };
-  auto take = [&](int k0, const GatherSet& S) __attribute__((always_inline)) {
-#pragma unroll
-    for (int u = 0; u < kGatherBatch; ++u) {
-      if (k0 + u >= R.ke) break;
-      if (S.e[u].y != cur) {
-        emit(cur);
-        cur = S.e[u].y;
-      }
-      const unsigned f = (S.w[u] >> ab) & 0xFu;
-      if (f == 0u) continue;
-      float* orow = gCE + (int64_t)S.e[u].x * Qs + 4 * j;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const float gf = S.gf[u][c];
-        if (4 * j + c < Q && ((f >> c) & 1u) && gf != 0.f) {
-          unsafeAtomicAdd(orow + c, -gf);
-          acc[c] += (double)gf;
-        }
-      }
-    }
-  };
-  GatherSet S0, S1, S2;
-  constexpr int B = kGatherBatch;
-  load_ids(R.kb, S0);
-  load_ids(R.kb + B, S1);
-  load_grad(R.kb, S0);
-  for (int k0 = R.kb;; k0 += 3 * B) {
-    load_ids(k0 + 2 * B, S2);
-    load_grad(k0 + B, S1);
-    take(k0, S0);
-    if (k0 + B >= R.ke) break;
-    load_ids(k0 + 3 * B, S0);
-    load_grad(k0 + 2 * B, S2);
-    take(k0 + B, S1);
-    if (k0 + 2 * B >= R.ke) break;
-    load_ids(k0 + 4 * B, S1);
-    load_grad(k0 + 3 * B, S0);
-    take(k0 + 2 * B, S2);
-    if (k0 + 3 * B >= R.ke) break;
-  }
-  if (R.slot >= 0) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) carry[(int64_t)R.slot * Qs + 4 * j + c] = acc[c];
-  } else {
-    emit(cur);
-  }
-}
-
-// gVR of the split sources: one wave per split, groups of Qs/4 lanes taking its pieces round-robin, the groups'
-// FP64 sums merged in group order through LDS, then added once.
-__global__ __launch_bounds__(256) void k_bwd_gather_fixup(int nsplit, const Split* __restrict__ splits,
-                                                          const double* __restrict__ carry, int Q, int Qs,
-                                                          float* __restrict__ gVR) {
-  __shared__ double red[4][64][4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int L4 = Qs / 4, G = 64 / L4;
-  const int g = lane / L4, j = lane - g * L4;
-  const int sidx = blockIdx.x * 4 + wave;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  Split S{0, 0, 0, 0};
-  if (sidx < nsplit) S = splits[sidx];
-  if (sidx < nsplit && g < G)
-    for (int c = g; c < S.count; c += G) {
-      const double* o = carry + (int64_t)(S.first + c) * Qs + 4 * j;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) s[u] += o[u];
-    }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) red[wave][lane][u] = s[u];
+// The vertex side of a step back, one tile of T vertices per block iteration:
+//   1. gVR[v][q] += gVRfix[v][q] 2^-k, the fixed-point sums of the step's FMax edge terms (where nonzero; the
+//      accumulator is cleared for the next step), staged with gVF in LDS;
+//   2. gMU[v][p] = sum over the incidences of p (ascending q) of gVF (q ends at p) or gVR (q starts at p);
+//   3. per-block partial column sums of gMU in FP64, vertices in order: part[b][p].
+// The input state's FMax / RMax gradients are -gVF / -gVR, which k_bwd_route of the next step back reads directly.
+__global__ __launch_bounds__(256) void k_bwd_vtx(BwdGraph G, float* __restrict__ gVR, const float* __restrict__ gVF,
+                                                 unsigned long long* __restrict__ gVRfix,
+                                                 const unsigned* __restrict__ gmax, float* __restrict__ gMU,
+                                                 double* __restrict__ part, int T) {
+  extern __shared__ float vsh[];  // [T][Qs] gVR, [T][Qs] gVF, [T][P] gMU, then the incidence lists
+  const int Qs = G.Qs, P = G.P, t = threadIdx.x;
+  float* tR = vsh;
+  float* tF = tR + T * Qs;
+  float* tM = tF + T * Qs;
+  int* ioff = reinterpret_cast<int*>(tM + T * P);
+  int* iinc = ioff + (P + 1);
+  const int ninc = G.inc_off[P];
+  for (int k = t; k <= P; k += 256) ioff[k] = G.inc_off[k];
+  for (int k = t; k < ninc; k += 256) iinc[k] = G.inc[k];
+  const double inv = ldexp(1.0, -fix_exponent(*gmax));
+  double col = 0.0;  // thread t < P: this block's partial sum of column t
   __syncthreads();
-  if (sidx < nsplit && g == 0) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      double a = 0.0;
-      for (int gg = 0; gg < G; ++gg) a += red[wave][gg * L4 + j][u];
-      const int q = 4 * j + u;
-      if (q < Q && a != 0.0) {
-        float* o = gVR + (int64_t)S.v * Qs + q;
-        *o = (float)((double)*o + a);
+  for (int64_t v0 = (int64_t)blockIdx.x * T; v0 < G.n; v0 += (int64_t)gridDim.x * T) {
+    const int nv = (int)min((int64_t)T, G.n - v0);
+    for (int e = t; e < nv * Qs; e += 256) {
+      const int64_t o = v0 * Qs + e;
+      float r = gVR[o];
+      const long long f = (long long)gVRfix[o];
+      if (f != 0) {
+        r = (float)((double)r + (double)f * inv);
+        gVR[o] = r;
+        gVRfix[o] = 0ull;
       }
+      tR[e] = r;
+      tF[e] = gVF[o];
     }
-  }
-}
-
-// gMU[v][p] = sum over the incidences of p (ascending q) of gVF (q ends at p) or gVR (q starts at p).  The input
-// state's FMax / RMax gradients are -gVF / -gVR, which k_bwd_route of the next step back reads directly.
-__global__ __launch_bounds__(256) void k_bwd_mu(BwdGraph G, const float* __restrict__ gVR, const float* __restrict__ gVF,
-                                                float* __restrict__ gMU) {
-  const int64_t total = G.n * G.P;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = i / G.P;
-    const int p = (int)(i - v * G.P);
-    float a = 0.f;
-    for (int k = G.inc_off[p]; k < G.inc_off[p + 1]; ++k) {
-      const int code = G.inc[k];
-      const int64_t o = v * G.Qs + (code >> 1);
-      a += (code & 1) ? gVR[o] : gVF[o];
+    __syncthreads();
+    for (int e = t; e < nv * P; e += 256) {
+      const int vl = e / P, p = e - vl * P;
+      float a = 0.f;
+      for (int k = ioff[p]; k < ioff[p + 1]; ++k) {
+        const int code = iinc[k];
+        a += (code & 1) ? tR[vl * Qs + (code >> 1)] : tF[vl * Qs + (code >> 1)];
+      }
+      tM[e] = a;
+      gMU[v0 * P + e] = a;
     }
-    gMU[i] = a;
+    __syncthreads();
+    if (t < P)
+      for (int vl = 0; vl < nv; ++vl) col += (double)tM[vl * P + t];
   }
+  if (t < P) part[(int64_t)blockIdx.x * P + t] = col;
 }
 
 // Fill subgradients of the prep state go to the argmax of MU0 = LSM(-CV) of the source / destination pattern vertex:
@@ -672,18 +593,25 @@ void launch_bwd_out(const BwdGraph& G, const double* MU, const double* colsum, f
 }
 
 void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gVF,
-                      const float* gVR, float* gEF, float* gER, const ArgMasks& am, float* gERsum, double* part,
+                      const float* gVR, float* gEF, float* gER, unsigned* gmax, float* gERsum, double* part,
                       int nparts, double* red4, hipStream_t st) {
   if (G.Q == 0) return;
-  hipLaunchKernelGGL(k_bwd_route, dim3(nparts), dim3(256), 0, st, G, S, gZ, with_gz, gVF, gVR, gEF, gER, am, gERsum,
+  hipLaunchKernelGGL(k_bwd_route, dim3(nparts), dim3(256), 0, st, G, S, gZ, with_gz, gVF, gVR, gEF, gER, gmax, gERsum,
                      part);
   hipLaunchKernelGGL(k_sum_parts, dim3(sum_blocks(4 * G.Q)), dim3(256), 0, st, nparts, 4 * G.Q, part, red4);
 }
 
 void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const QStat* qs1, const double* red4,
-                   const float* gER, float* gVR, float* gVF, double* cD, hipStream_t st) {
+                   const float* gER, float* gVR, float* gVF, double* cD, const FmaxTerms& FE, hipStream_t st) {
   if (G.Q == 0) return;
-  hipLaunchKernelGGL(k_bwd_in, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, S, MU, qs1, red4, gER, gVR, gVF, cD);
+  hipLaunchKernelGGL(k_bwd_in, dim3(grid_for(G.n * G.Q)), dim3(256), 0, st, G, S, MU, qs1, red4, gER, gVR, gVF, cD, FE);
+}
+
+void launch_bwd_fscatter(const BwdGraph& G, const float* gEF, const int* FA, const int* bsrc, int64_t brows,
+                         float* gCE, hipStream_t st) {
+  if (G.Q == 0) return;
+  hipLaunchKernelGGL(k_bwd_fscatter, dim3(grid_for(G.n * G.Qs)), dim3(256), 0, st, G.n, G.Q, G.Qs, gEF, FA, bsrc, brows,
+                     gCE);
 }
 
 // RMax edge terms, once after the last step: the minimum-cost out-edge row AO[v][q] receives -gERsum[v][q] (the sum
@@ -700,33 +628,8 @@ __global__ void k_bwd_rterms(int64_t n, int Q, int Qs, const float* __restrict__
   }
 }
 
-__global__ void k_csr_dst(int64_t m, const int2* __restrict__ ks, const int* __restrict__ bdst, int* __restrict__ out,
-                          int* __restrict__ rowpos) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-    const int row = ks[i].x;
-    out[i] = bdst[row];
-    rowpos[row] = (int)i;
-  }
-}
 
-void launch_csr_dst(int64_t m, const int2* ks, const int* bdst, int* out, int* rowpos, int64_t brows,
-                    hipStream_t st) {
-  launch_fill_int(rowpos, brows, -1, st);  // padding rows have no CSR position
-  if (m == 0) return;
-  const int blocks = (int)std::min<int64_t>((m + 255) / 256, 1 << 16);
-  hipLaunchKernelGGL(k_csr_dst, dim3(blocks), dim3(256), 0, st, m, ks, bdst, out, rowpos);
-}
 
-void launch_bwd_gather(const BwdGraph& G, int64_t nruns, const Run* runs, int nsplit, const Split* splits,
-                       const int2* ks, const int* kdst, const float* gEF, float* gVR, float* gCE, double* carry,
-                       unsigned* argbits, hipStream_t st) {
-  if (G.Q == 0 || nruns == 0) return;
-  hipLaunchKernelGGL(k_bwd_gather, dim3(walk_blocks(nruns, G.Qs)), dim3(kWalkBlock), 0, st, nruns, runs, ks, kdst, G.Q,
-                     G.Qs, gEF, argbits, gVR, gCE, carry);
-  if (gVR && nsplit > 0)
-    hipLaunchKernelGGL(k_bwd_gather_fixup, dim3((unsigned)((nsplit + 3) / 4)), dim3(256), 0, st, nsplit, splits, carry,
-                       G.Q, G.Qs, gVR);
-}
 
 void launch_bwd_rterms(const BwdGraph& G, const float* gERsum, const int* AO, int64_t brows, float* gCE,
                        hipStream_t st) {
@@ -736,8 +639,18 @@ void launch_bwd_rterms(const BwdGraph& G, const float* gERsum, const int* AO, in
                      G.n, G.Q, G.Qs, gERsum, AO, brows, gCE);
 }
 
-void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, hipStream_t st) {
-  hipLaunchKernelGGL(k_bwd_mu, dim3(grid_for(G.n * G.P)), dim3(256), 0, st, G, gVR, gVF, gMU);
+int bwd_vtx_tile(const BwdGraph& G) { return std::max(1, std::min(64, 8192 / std::max(1, 2 * G.Qs + G.P))); }
+
+int bwd_vtx_blocks(const BwdGraph& G) {
+  const int T = bwd_vtx_tile(G);
+  return (int)std::max<int64_t>(1, std::min<int64_t>((G.n + T - 1) / T, 2048));
+}
+
+void launch_bwd_vtx(const BwdGraph& G, float* gVR, const float* gVF, unsigned long long* gVRfix, const unsigned* gmax,
+                    float* gMU, double* part, hipStream_t st) {
+  const int T = bwd_vtx_tile(G), nb = bwd_vtx_blocks(G);
+  const size_t lds = (size_t)T * (2 * G.Qs + G.P) * sizeof(float) + (size_t)(G.P + 1 + 2 * G.Q) * sizeof(int);
+  hipLaunchKernelGGL(k_bwd_vtx, dim3(nb), dim3(256), lds, st, G, gVR, gVF, gVRfix, gmax, gMU, part, T);
 }
 
 void launch_bwd_prep_fill(const BwdGraph& G, const double* red4, const int* am0, float* gMU, hipStream_t st) {

@@ -35,38 +35,39 @@ void launch_fill_int(int* p, int64_t count, int value, hipStream_t st);
 void launch_sum_parts(int nparts, int C, const double* part, double* out, hipStream_t st);
 void launch_colsum(int64_t n, int C, const float* X, double* part, int nparts, double* out, hipStream_t st);
 void launch_bwd_out(const BwdGraph& G, const double* MU, const double* colsum, float* gMU, float* gCV, hipStream_t st);
-// The argmax masks of the m CSR positions: bit q of word [k][q / 32] marks that the row at position k is the FMax
-// argmax of column q of its destination and that edge term carries gradient.  k_bwd_route sets them from FA and the
-// gather reads and clears them, so they are zero between uses (and must be zero at the first).
-struct ArgMasks {
-  const int* FA;      // [n][Qs] the state's FMax argmax rows
-  const int* rowpos;  // [brows] the CSR position of each bundle row (-1: padding)
-  int64_t brows, m;
-  int W;              // words per position: ceil(Qs / 32)
-  unsigned* bits;     // [m][W] (nullptr: no masks)
+// The FMax edge terms of one step back: the row FA[v][q] recorded for each (v, q) whose edge term carries gradient
+// gE = gEF[v][q] != 0 receives -gE in gCE, and its source's sum gVRfix[src][q] gains gE in fixed point (scale from the
+// step's largest |gE_F|, gmax).
+struct FmaxTerms {
+  const float* gEF;             // [n][Qs] the step's gE_F
+  const int* FA;                // [n][Qs] the state's FMax argmax rows
+  const int* bsrc;              // [brows] the source of each bundle row (bit 31: the run's vertex flag)
+  int64_t brows;
+  float* gCE;                   // [brows][Qs]
+  unsigned long long* gVRfix;   // [n][Qs] fixed-point source sums (two's complement), zero between steps
+  const unsigned* gmax;         // the step's largest |gE_F| as float bits (k_bwd_route)
 };
 
 // gVF / gVR: those of the next state (the previous step back; zeros at the first), whose negations are this state's
-// FMax / RMax gradients.  Sets the argmax masks am; gERsum (nullable) accumulates gE_R over the steps.
+// FMax / RMax gradients.  gmax (nullable, zero before): receives the largest |gE_F|; gERsum (nullable) accumulates
+// gE_R over the steps.
 void launch_bwd_route(const BwdGraph& G, const BwdState& S, const float* gZ, int with_gz, const float* gVF,
-                      const float* gVR, float* gEF, float* gER, const ArgMasks& am, float* gERsum, double* part,
+                      const float* gVR, float* gEF, float* gER, unsigned* gmax, float* gERsum, double* part,
                       int nparts, double* red4, hipStream_t st);
+// Also the step's FMax edge terms FE.
 void launch_bwd_in(const BwdGraph& G, const BwdState& S, const double* MU, const QStat* qs1, const double* red4,
-                   const float* gER, float* gVR, float* gVF, double* cD, hipStream_t st);
-// FMax edge terms of one iteration (or of the prep, gVR == nullptr), gathered over the CSR runs `runs` (pairs ks =
-// {bundle row, source}, kdst = the destination at each CSR position) from the argmax masks that k_bwd_route set, which
-// it clears; carry: one [Qs] FP64 row per carry slot of the runs.  Deterministic.
-void launch_bwd_gather(const BwdGraph& G, int64_t nruns, const Run* runs, int nsplit, const Split* splits,
-                       const int2* ks, const int* kdst, const float* gEF, float* gVR, float* gCE, double* carry,
-                       unsigned* argbits, hipStream_t st);
+                   const float* gER, float* gVR, float* gVF, double* cD, const FmaxTerms& FE, hipStream_t st);
+// The prep's FMax edge terms: gCE only (its state has no VR).
+void launch_bwd_fscatter(const BwdGraph& G, const float* gEF, const int* FA, const int* bsrc, int64_t brows,
+                         float* gCE, hipStream_t st);
 // RMax edge terms, once after the last step: gCE[AO[v][q]][q] -= gERsum[v][q].
 void launch_bwd_rterms(const BwdGraph& G, const float* gERsum, const int* AO, int64_t brows, float* gCE,
                        hipStream_t st);
-// out[k] = bdst[ks[k].x]: the destination of the edge at each of the m CSR positions; rowpos[row] = the CSR position
-// of each bundle row (-1: padding).
-void launch_csr_dst(int64_t m, const int2* ks, const int* bdst, int* out, int* rowpos, int64_t brows,
-                    hipStream_t st);
-void launch_bwd_mu(const BwdGraph& G, const float* gVR, const float* gVF, float* gMU, hipStream_t st);
+// The vertex side of a step back (k_bwd_vtx): adds the fixed-point sums gVRfix (scale from gmax) to gVR and clears
+// them, writes gMU, and leaves bwd_vtx_blocks(G) partial column sums of gMU in part ([blocks][P]).
+int bwd_vtx_blocks(const BwdGraph& G);
+void launch_bwd_vtx(const BwdGraph& G, float* gVR, const float* gVF, unsigned long long* gVRfix, const unsigned* gmax,
+                    float* gMU, double* part, hipStream_t st);
 void launch_bwd_prep_fill(const BwdGraph& G, const double* red4, const int* am0, float* gMU, hipStream_t st);
 void launch_argmin_out(int64_t m, int Q, int Qs, const int2* ks, const float* CE, const float* minCEo, int* AO,
                        hipStream_t st);

@@ -215,8 +215,6 @@ struct ac_ctx {
     DevBuf<Bundle> bundles;
     DevBuf<int> gslot, bsrc, bdst, beid;
     DevBuf<int2> csr_kb;
-    DevBuf<int> csr_dst;  // the destination at each CSR position (built on the first backward pass)
-    DevBuf<int> rowpos;   // the CSR position of each bundle row, -1 for padding (idem)
     int64_t nbundles = 0, brows = 0;
   };
   std::vector<std::unique_ptr<BundleSet>> bsets;
@@ -260,8 +258,10 @@ struct ac_ctx {
   DevBuf<Lse2> ckP, sgP;
   // Adjoint work buffers.
   DevBuf<float> gMUin, gMU, gCV, gEF, gER, gVR, gVF, gCE, apv;
-  DevBuf<unsigned> argbits;  // per CSR position, the pattern edges its row is the FMax argmax of (backward scratch)
-  DevBuf<float> gERsum;      // per (vertex, pattern edge), gE_R summed over the steps back
+  DevBuf<unsigned long long> gVRfix;  // per (vertex, pattern edge), fixed-point sums of a step's FMax edge terms
+  DevBuf<unsigned> gmax;              // per step back, the largest |gE_F| (float bits)
+  DevBuf<double> vpart;               // partial column sums of gMU from the vertex kernel
+  DevBuf<float> gERsum;               // per (vertex, pattern edge), gE_R summed over the steps back
   DevBuf<double> cD, colsum, bpart, red4, cq, opart, osum, MUd;
   DevBuf<int> api;
 
@@ -1703,14 +1703,9 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, ctx->gCV.alloc(np));
   for (DevBuf<float>* b : {&ctx->gEF, &ctx->gER, &ctx->gVR, &ctx->gVF}) HIPCHK(ctx, b->alloc(nq));
   HIPCHK(ctx, ctx->gCE.alloc((size_t)std::max<int64_t>(1, brows) * Qs));
-  const int W = (Qs + 31) / 32;
-  HIPCHK(ctx, ctx->argbits.alloc((size_t)std::max<int64_t>(1, ctx->de) * W));
+  HIPCHK(ctx, ctx->gVRfix.alloc(nq));
+  HIPCHK(ctx, ctx->gmax.alloc((size_t)n_iter + 1));
   HIPCHK(ctx, ctx->gERsum.alloc(nq));
-  if (ctx->bs->csr_dst.n != (size_t)std::max<int64_t>(1, ctx->de)) {  // once per bundle layout
-    HIPCHK(ctx, ctx->bs->csr_dst.alloc((size_t)std::max<int64_t>(1, ctx->de)));
-    HIPCHK(ctx, ctx->bs->rowpos.alloc((size_t)std::max<int64_t>(1, brows)));
-    ac::launch_csr_dst(ctx->de, ctx->bs->csr_kb.p, ctx->bs->bdst.p, ctx->bs->csr_dst.p, ctx->bs->rowpos.p, brows, st);
-  }
   HIPCHK(ctx, ctx->cD.alloc(nq));
   HIPCHK(ctx, ctx->MUd.alloc(np));
   HIPCHK(ctx, ctx->colsum.alloc((size_t)P));
@@ -1729,11 +1724,13 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, hipMemsetAsync(ctx->gCE.p, 0, ctx->gCE.n * sizeof(float), st));
   HIPCHK(ctx, hipMemsetAsync(ctx->cD.p, 0, nq * sizeof(double), st));
   HIPCHK(ctx, hipMemsetAsync(ctx->gERsum.p, 0, nq * sizeof(float), st));
-  // The argmax masks start at zero; every gather clears the words it reads.
-  HIPCHK(ctx, hipMemsetAsync(ctx->argbits.p, 0, ctx->argbits.n * sizeof(unsigned), st));
+  // The fixed-point sums start at zero; the vertex kernel of each step clears the ones it adds.
+  HIPCHK(ctx, hipMemsetAsync(ctx->gVRfix.p, 0, nq * sizeof(unsigned long long), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->gmax.p, 0, ((size_t)n_iter + 1) * sizeof(unsigned), st));
 
   ac::BwdGraph G{n, P, Q, Qs, ctx->CV.p, ctx->minCEo.p, ctx->lsO.p, ctx->d_ps.p, ctx->d_pd.p, ctx->d_inc_off.p,
                  ctx->d_inc.p};
+  HIPCHK(ctx, ctx->vpart.alloc((size_t)ac::bwd_vtx_blocks(G) * std::max(1, P)));
   // Segment states: state 0 is the checkpoint itself, states 1..K live in the segment buffers, whose constant rows
   // (-inf) are set once here and never written afterwards.
   ac::launch_fill(ctx->sgF.p, (int64_t)(K + 1) * nq, -INFINITY, st);
@@ -1761,20 +1758,23 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
       const ac::BwdState S0{B0.F, B0.V, B0.qs, B0.pst, i == 0};
       const ac::BwdState S1{B1.F, B1.V, B1.qs, B1.pst, 0};
       if (!have_mu) launch_mu(ctx, S1, st);
-      ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
+      // Column sums of gMU: from the previous step's vertex kernel, or (first step back) of the caller's gradient.
+      if (have_mu)
+        ac::launch_sum_parts(ac::bwd_vtx_blocks(G), P, ctx->vpart.p, ctx->colsum.p, st);
+      else
+        ac::launch_colsum(n, P, ctx->gMU.p, ctx->bpart.p, nparts, ctx->colsum.p, st);
       ac::launch_bwd_out(G, ctx->MUd.p, ctx->colsum.p, ctx->gMU.p, ctx->gCV.p, st);
-      const ac::ArgMasks am{ctx->tapeFA.p + s * nq, ctx->bs->rowpos.p, brows, ctx->de, W, ctx->argbits.p};
-      ac::launch_bwd_route(G, S1, ctx->gMU.p, 1, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, am, ctx->gERsum.p,
+      unsigned* gmax = ctx->gmax.p + i;
+      ac::launch_bwd_route(G, S1, ctx->gMU.p, 1, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, gmax, ctx->gERsum.p,
                            ctx->bpart.p, nparts, ctx->red4.p, st);
       launch_mu(ctx, S0, st);  // MU of the input state, reused by the next step back
       have_mu = true;
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVR.p, 0, nq * sizeof(float), st));
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
-      ac::launch_bwd_in(G, S0, ctx->MUd.p, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, st);
-      ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p,
-                            ctx->bs->csr_kb.p, ctx->bs->csr_dst.p, ctx->gEF.p, ctx->gVR.p, ctx->gCE.p,
-                            ctx->carry_out.p, ctx->argbits.p, st);
-      ac::launch_bwd_mu(G, ctx->gVR.p, ctx->gVF.p, ctx->gMU.p, st);
+      const ac::FmaxTerms FE{ctx->gEF.p, ctx->tapeFA.p + s * nq, ctx->bs->bsrc.p, brows, ctx->gCE.p, ctx->gVRfix.p,
+                             gmax};
+      ac::launch_bwd_in(G, S0, ctx->MUd.p, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, FE, st);
+      ac::launch_bwd_vtx(G, ctx->gVR.p, ctx->gVF.p, ctx->gVRfix.p, gmax, ctx->gMU.p, ctx->vpart.p, st);
     }
   }
   // Prep (src/main.cu:107-148): FMax0 / RMax0 from -CE normalised by LSE0 = log sum_e exp(-CE), filled from
@@ -1782,11 +1782,9 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   const ac::BwdState S0{ctx->ckF.p, ctx->ckV.p, ctx->ckQ.p, ctx->ckP.p, 1};
   std::vector<double> cq(Q1, 0.0);
   if (Q > 0) {
-    const ac::ArgMasks am{ctx->FA0.p, ctx->bs->rowpos.p, brows, ctx->de, W, ctx->argbits.p};
-    ac::launch_bwd_route(G, S0, nullptr, 0, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, am, ctx->gERsum.p,
+    ac::launch_bwd_route(G, S0, nullptr, 0, ctx->gVF.p, ctx->gVR.p, ctx->gEF.p, ctx->gER.p, nullptr, ctx->gERsum.p,
                          ctx->bpart.p, nparts, ctx->red4.p, st);
-    ac::launch_bwd_gather(G, ctx->nruns_out, ctx->runs_out.p, ctx->nsplit_out, ctx->splits_out.p, ctx->bs->csr_kb.p,
-                          ctx->bs->csr_dst.p, ctx->gEF.p, nullptr, ctx->gCE.p, ctx->carry_out.p, ctx->argbits.p, st);
+    ac::launch_bwd_fscatter(G, ctx->gEF.p, ctx->FA0.p, ctx->bs->bsrc.p, brows, ctx->gCE.p, st);
     ac::launch_bwd_rterms(G, ctx->gERsum.p, ctx->AO.p, brows, ctx->gCE.p, st);
     ac::launch_bwd_prep_fill(G, ctx->red4.p, ctx->am0.p, ctx->gMU.p, st);
     std::vector<double> red4((size_t)4 * Q);
