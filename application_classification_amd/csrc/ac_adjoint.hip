@@ -84,30 +84,29 @@ __global__ __launch_bounds__(256) void k_colsum(int64_t n, int C, const float* _
 // summing parts l, l + L, ... in order, the lanes then summed in order (deterministic).
 __global__ __launch_bounds__(256) void k_sum_parts(int nparts, int C, const double* __restrict__ part,
                                                    double* __restrict__ out) {
+  // One block per column: lane l sums partials l, l + 256, ... in order (eight loads in flight before the adds), then
+  // a fixed tree over the lanes.  The order is the same on every call.
   __shared__ double red[256];
-  const int CB = C < 32 ? C : 32, L = 256 / CB, t = threadIdx.x;
-  const int c = blockIdx.x * CB + t % CB, l = t / CB;
+  const int t = threadIdx.x, c = blockIdx.x;
   double a = 0.0;
-  if (l < L && c < C)
-    for (int b0 = l; b0 < nparts; b0 += 8 * L) {
-      // Eight loads in flight before the adds, which keep their order (out-of-range slots are not added).
-      double v[8];
+  for (int b0 = t; b0 < nparts; b0 += 8 * 256) {
+    double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int b = b0 + u * L;
-        v[u] = part[(int64_t)(b < nparts ? b : nparts - 1) * C + c];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (b0 + u * L < nparts) a += v[u];
+    for (int u = 0; u < 8; ++u) {
+      const int b = b0 + u * 256;
+      v[u] = part[(int64_t)(b < nparts ? b : nparts - 1) * C + c];
     }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (b0 + u * 256 < nparts) a += v[u];
+  }
   red[t] = a;
   __syncthreads();
-  if (t < CB && c < C) {
-    double s = 0.0;
-    for (int k = 0; k < L; ++k) s += red[k * CB + t];
-    out[c] = s;
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
   }
+  if (t == 0) out[c] = red[0];
 }
 
 // Log-softmax Jacobian of MU = LSM_v(Z): gMU <- gZ = gMU - exp(MU) colsum(gMU); gCV -= gZ.
@@ -572,10 +571,7 @@ void launch_fill_int(int* p, int64_t count, int value, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_int, dim3(grid_for(count)), dim3(256), 0, st, p, count, value);
 }
 
-static int sum_blocks(int C) {
-  const int CB = C < 32 ? C : 32;
-  return (C + CB - 1) / CB;
-}
+static int sum_blocks(int C) { return std::max(1, C); }
 
 void launch_sum_parts(int nparts, int C, const double* part, double* out, hipStream_t st) {
   if (C == 0) return;

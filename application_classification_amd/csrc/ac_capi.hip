@@ -1694,7 +1694,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs, K = ctx->tape_K, n_iter = ctx->tape_iters;
   const int Q1 = std::max(1, Q);
   const size_t nq = (size_t)n * Qs, np = (size_t)n * P;
-  const int nparts = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 512));
+  const int nparts = (int)std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 2048));
   const int64_t brows = ctx->bs->brows;
   // grad_mu is in the caller's vertex order; the adjoint works in the internal one.
   HIPCHK(ctx, ctx->gMUin.upload(grad_mu, np, st));
