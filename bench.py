@@ -735,8 +735,9 @@ def main():
     m.close()
     del m, mu, w, d, p  # the R-MAT 22 graphs leave the GPU and the host before the other measurements
 
-    # The metric's other two parts: batched against sequential plain runs, and forward+backward against forward
-    # (one warmup pass and one timed pass each; --config batched / diff time them with --steps / --warmup).
+    # The metric's other two parts: batched against sequential plain runs (one warmup pass and one timed pass each),
+    # and forward+backward against forward (one warmup pass and the best of three, since a single pass of either side
+    # can be slowed by what ran just before it); --config batched / diff time them with --steps / --warmup.
     batched = fwd_bwd = None
     if not args.no_extras:
         log("batched: 256 uneven patterns on R-MAT 20 ...")
@@ -749,7 +750,7 @@ def main():
                    "sampled_patterns": b["checks"]["sampled_patterns"]}
         log(f"batched: {json.dumps(batched)}")
         log("forward+backward: 64 patterns on R-MAT 18 ...")
-        f = measure_diff(1, 1)
+        f = measure_diff(3, 1)
         fwd_bwd = {"ratio": f["ratio"], "forward_backward_ms": f["forward_backward_ms"], "forward_ms": f["forward_ms"],
                    "patterns": f["config"]["patterns"], "checkpoint_every": 4, "grads_finite": f["checks"]["all_finite"]}
         log(f"forward+backward: {json.dumps(fwd_bwd)}")
