@@ -124,8 +124,11 @@ struct DevBuf {
   }
 };
 
+// Runs over n vertices whose edge lists are the ranges [off[t], off[t + 1]) in sequence: vertex vid[t] (t itself
+// when vid is null).  consecutive: a run may only join vertices whose ids follow each other (the CSC walk steps its
+// destination by one at each new vertex); the CSR walk reads each edge's source and takes any order.
 void build_runs(const std::vector<int64_t>& off, int64_t n, std::vector<Run>& runs, std::vector<Split>& splits,
-                int& nslots) {
+                int& nslots, const int* vid = nullptr, bool consecutive = true) {
   const int kRunEdges = run_edges();
   runs.clear();
   splits.clear();
@@ -136,25 +139,25 @@ void build_runs(const std::vector<int64_t>& off, int64_t n, std::vector<Run>& ru
     if (open) runs.push_back(Run{(int)b, (int)e, -1, 0});
     open = false;
   };
-  for (int64_t v = 0; v < n; ++v) {
-    const int64_t deg = off[v + 1] - off[v];
+  for (int64_t t = 0; t < n; ++t) {
+    const int64_t v = vid ? vid[t] : t;
+    const int64_t deg = off[t + 1] - off[t];
     if (deg == 0) continue;
     if (deg > kRunEdges) {
-      close(off[v]);
+      close(off[t]);
       const int pieces = (int)((deg + kRunEdges - 1) / kRunEdges);
       splits.push_back(Split{(int)v, nslots, pieces, 0});
       for (int i = 0; i < pieces; ++i) {
-        const int64_t kb = off[v] + (int64_t)i * kRunEdges;
-        runs.push_back(Run{(int)kb, (int)std::min(off[v + 1], kb + kRunEdges), nslots++, 0});
+        const int64_t kb = off[t] + (int64_t)i * kRunEdges;
+        runs.push_back(Run{(int)kb, (int)std::min(off[t + 1], kb + kRunEdges), nslots++, 0});
       }
       continue;
     }
-    // A run covers consecutive vertices only: the walk steps its destination by one at each new vertex.
-    if (open && (len + deg > kRunEdges || v != last_v + 1)) close(off[v]);
+    if (open && (len + deg > kRunEdges || (consecutive && v != last_v + 1))) close(off[t]);
     last_v = v;
     if (!open) {
       open = true;
-      b = off[v];
+      b = off[t];
       len = 0;
     }
     len += deg;
@@ -409,12 +412,31 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
       csc_pos[e] = (int)k;
     }
   }
-  for (int64_t i = 0; i < nE; ++i) csr_ks[i] = make_int2(csc_pos[by_src[i]], src[by_src[i]]);
+  // The CSR pairs list the sources by decreasing out-degree (ties by id), each source's edges in CSR order.  The
+  // per-source walk (k_walk_src) gives a run to each group of lanes and finishes a source wave-wide whenever any
+  // group's source changes; with the runs of a wave made of sources of one degree, the groups change source on the
+  // same steps.
+  std::vector<int> sord;
+  std::vector<int64_t> sord_off(1, 0);
+  {
+    for (int64_t v = 0; v < nV; ++v)
+      if (out_off[v + 1] > out_off[v]) sord.push_back((int)v);
+    std::stable_sort(sord.begin(), sord.end(), [&](int a, int b) {
+      return out_off[a + 1] - out_off[a] > out_off[b + 1] - out_off[b];
+    });
+    sord_off.reserve(sord.size() + 1);
+    int64_t i = 0;
+    for (const int v : sord) {
+      for (int64_t k = out_off[v]; k < out_off[v + 1]; ++k, ++i)
+        csr_ks[i] = make_int2(csc_pos[by_src[k]], src[by_src[k]]);
+      sord_off.push_back(i);
+    }
+  }
   std::vector<Run> rin, rout;
   std::vector<Split> sin, sout;
   int slin = 0, slout = 0;
   build_runs(in_off, nV, rin, sin, slin);
-  build_runs(out_off, nV, rout, sout, slout);
+  build_runs(sord_off, (int64_t)sord.size(), rout, sout, slout, sord.data(), false);
   std::vector<Split> fch, fms;
   int nparts = 0;
   for (const Split& S : sin) {
