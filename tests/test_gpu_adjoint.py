@@ -175,8 +175,8 @@ def test_packed_costs_give_identical_gradients():
 
 def test_backward_is_reproducible():
     """R-MAT scale 12: hub sources are the recorded argmax in-edge of many destinations at once, and split hubs spread
-    over several CSR pieces.  The edge terms are gathered per source in a fixed order, so two backward passes (and
-    two separate contexts) give bit-identical gradients."""
+    over several CSR pieces.  A source's edge terms are summed in fixed point (integer additions, whose result does
+    not depend on their order), so two backward passes (and two separate contexts) give bit-identical gradients."""
     w = W.rmat(12, 16, 10, 14, seed=8)
     d, p = w.data, w.pattern
     outdeg = np.bincount(d.src, minlength=d.num_nodes)
@@ -195,6 +195,32 @@ def test_backward_is_reproducible():
                 res.append(np.concatenate(m.backward(gm)))
     for r in res[1:]:
         np.testing.assert_array_equal(r, res[0])
+
+
+def test_backward_scales_exactly_with_the_output_gradient():
+    """The adjoint is linear in the output gradient, and every scale inside it follows the gradients: the fixed-point
+    scale of the source sums is a power of two set by the largest edge-term gradient.  Multiplying grad_mu by 2^k
+    therefore multiplies both weight gradients by 2^k, exactly wherever no intermediate value is subnormal (printed:
+    whether the result is bit-identical); the check allows 1e-6 of the gradients' scale for such values.  A wrong
+    fixed-point scale errs by powers of two at the hub sources."""
+    w = W.rmat(12, 16, 10, 14, seed=8)
+    d, p = w.data, w.pattern
+    wv, we = weights(9, 16, 16)
+    gm = loss_grad_mu(d.num_nodes, p.vmap).astype(np.float32)
+    with Matcher(0) as m:
+        m.set_option(AC_OPT_TAPE, 4)
+        m.set_data(d)
+        m.set_weights(wv, we)
+        m.set_pattern(p)
+        m.run()
+        base = np.concatenate(m.backward(gm))
+        assert np.abs(base).max() > 0
+        for k in (20, -20):
+            got = np.concatenate(m.backward(np.ldexp(gm, k)))
+            want = np.ldexp(base, k)
+            err = np.abs(got.astype(np.float64) - want).max() / np.abs(want).max()
+            say(f"2^{k} x grad_mu: bit-identical = {np.array_equal(got, want)}, max |diff| / scale = {err:.2e}")
+            assert err <= 1e-6
 
 
 def test_weights_are_learned():
