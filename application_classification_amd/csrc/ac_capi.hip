@@ -171,6 +171,10 @@ struct ac_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // A second stream (default priority, like the first) for prep work that does not depend on the edge costs or
+  // that only reads them beside another kernel; xev fork and join it (no timing).
+  hipStream_t aux = nullptr;
+  hipEvent_t xev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::string err;
 
   // Data graph (src/main.cu:8-19 / 49-62) plus its CSC and CSR views.
@@ -324,9 +328,17 @@ extern "C" int ac_create(ac_ctx** out, int device) {
   }
   ac_ctx* c = new ac_ctx();
   c->device = device;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
-      hipEventCreate(&c->ev[2]) != hipSuccess) {
+  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) == hipSuccess;
+  for (auto& e : c->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  for (auto& e : c->xev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    for (auto& e : c->ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->xev)
+      if (e) (void)hipEventDestroy(e);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     ac::set_global_error("ac_create: could not create the stream / events");
     return AC_ERR_HIP;
@@ -339,9 +351,13 @@ extern "C" int ac_destroy(ac_ctx* ctx) {
   if (!ctx) return AC_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->xev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->pev) (void)hipEventDestroy(e);
+  if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return AC_OK;
@@ -841,24 +857,48 @@ static int enqueue_prep(ac_ctx* ctx) {
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
   MARK(-1);
   // ---- prep (src/main.cu:107-148)
-  HIPCHK(ctx, ctx->cd_scratch.alloc(std::max<size_t>(1, std::max(ac::cdist_scratch_doubles(P, (int)ctx->dV),
-                                                                  ac::cdist_scratch_doubles(Qs, (int)ctx->dE)))));
-  double* cds = ctx->use_mfma_cdist ? ctx->cd_scratch.p : nullptr;
+  // Vertex and edge distances run at the same time, so each has its own pattern operands.
+  const size_t vsd = ac::cdist_scratch_doubles(P, (int)ctx->dV), esd = ac::cdist_scratch_doubles(Qs, (int)ctx->dE);
+  HIPCHK(ctx, ctx->cd_scratch.alloc(std::max<size_t>(1, vsd + esd)));
+  double* cds_v = ctx->use_mfma_cdist ? ctx->cd_scratch.p : nullptr;
+  double* cds_e = ctx->use_mfma_cdist ? ctx->cd_scratch.p + vsd : nullptr;
+  if (Q > 0) {
+    HIPCHK(ctx, ctx->bmin.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles) * Qs));
+    if (ctx->pack_costs) HIPCHK(ctx, ctx->bpack.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles)));
+    if (ctx->pack_costs) HIPCHK(ctx, ctx->CEp.alloc((size_t)std::max<int64_t>(1, ctx->bs->brows * (Qs / 4) * 3)));
+  }
+  // The vertex side (distances, column statistics, isolated columns) and the constant rows of the per-vertex arrays
+  // go to the second stream, beside the edge distances on the first; the first waits for them (xev[1]) before the
+  // per-source statistics, which reuse the partial buffer, and the prep scalars, which read lse_cv.
+  hipStream_t ax = ctx->aux;
+  HIPCHK(ctx, hipEventRecord(ctx->xev[0], st));
+  HIPCHK(ctx, hipStreamWaitEvent(ax, ctx->xev[0], 0));
   ac::launch_cdist(n, ctx->d_perm.p, ctx->vfeat.p, (int)ctx->dV, ctx->pf_v.p, P, round8(P), P, ctx->CV.p, P,
-                   ctx->has_wv ? ctx->wv.p : nullptr, cds, st);
-  ac::launch_col_lse_neg(n, P, ctx->CV.p, ctx->lpart.p, ctx->nb_col, st);
-  ac::launch_reduce_lse(ctx->nb_col, P, ctx->lpart.p, ctx->lse_cv.p, st);
+                   ctx->has_wv ? ctx->wv.p : nullptr, cds_v, ax);
+  ac::launch_col_lse_neg(n, P, ctx->CV.p, ctx->lpart.p, ctx->nb_col, ax);
+  ac::launch_reduce_lse(ctx->nb_col, P, ctx->lpart.p, ctx->lse_cv.p, ax);
   if (ctx->n_iso > 0)
     ac::launch_iso_cols(ctx->n_iso, P, ctx->CV.p + ctx->n_act * P, (int)ctx->n_act, ctx->lpart.p, ctx->apv.p,
-                        ctx->api.p, std::min(ctx->nb_col, 256), ctx->iso.p, st);
-  HIPCHK(ctx, hipMemcpyAsync(ctx->pstat.p, ctx->lse_cv.p, sizeof(Lse2) * P, hipMemcpyDeviceToDevice, st));
+                        ctx->api.p, std::min(ctx->nb_col, 256), ctx->iso.p, ax);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->pstat.p, ctx->lse_cv.p, sizeof(Lse2) * P, hipMemcpyDeviceToDevice, ax));
+  if (Q > 0) {
+    ac::launch_fill(ctx->minCEo.p, n * Qs, INFINITY, ax);
+    ac::launch_fill(ctx->lsO.p, n * Qs, 0.f, ax);
+    ac::launch_fill(ctx->VFm.p, n * Qs, -INFINITY, ax);
+    ac::launch_fill(ctx->FMaxN.p, n * Qs, -INFINITY, ax);
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->xev[1], ax));
   if (Q > 0) {
     ac::launch_cdist(ctx->bs->brows, ctx->bs->beid.p, ctx->efeat.p, (int)ctx->dE, ctx->pf_e.p, Q, round8(Q), Qs, ctx->CE.p,
-                     Qs, ctx->has_we ? ctx->we.p : nullptr, cds, st);
-    ac::launch_fill(ctx->minCEo.p, n * Qs, INFINITY, st);
-    ac::launch_fill(ctx->lsO.p, n * Qs, 0.f, st);
-    ac::launch_fill(ctx->VFm.p, n * Qs, -INFINITY, st);
-    ac::launch_fill(ctx->FMaxN.p, n * Qs, -INFINITY, st);
+                     Qs, ctx->has_we ? ctx->we.p : nullptr, cds_e, st);
+    // The bundle minima (a stream over CE) on the second stream, beside the per-source statistics (a gather over
+    // CE, bound by its own instruction chains); the first stream waits for them (xev[3]) before the prep walk.
+    HIPCHK(ctx, hipEventRecord(ctx->xev[2], st));
+    HIPCHK(ctx, hipStreamWaitEvent(ax, ctx->xev[2], 0));
+    ac::launch_bundle_min(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Q, Qs, ctx->bmin.p,
+                          ctx->pack_costs ? ctx->bpack.p : nullptr, ax);
+    HIPCHK(ctx, hipEventRecord(ctx->xev[3], ax));
+    HIPCHK(ctx, hipStreamWaitEvent(st, ctx->xev[1], 0));
     if (ctx->nruns_out > 0) {
       ac::launch_walk_src(ctx->nruns_out, ctx->runs_out.p, ctx->bs->csr_kb.p, ctx->CE.p, Q, Qs, ctx->minCEo.p,
                           ctx->lsO.p, ctx->VFm.p, ctx->carry_out.p, ctx->lpart.p, ctx->nb_src, st);
@@ -871,12 +911,8 @@ static int enqueue_prep(ac_ctx* ctx) {
       HIPCHK(ctx, hipStreamSynchronize(st));
     }
     ac::launch_prep_scalars(P, Q, ctx->d_ps.p, ctx->d_pd.p, ctx->lse_cv.p, ctx->lse_ce.p, qs_prep, st);
-    HIPCHK(ctx, ctx->bmin.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles) * Qs));
-    if (ctx->pack_costs) HIPCHK(ctx, ctx->bpack.alloc((size_t)std::max<int64_t>(1, ctx->bs->nbundles)));
-    ac::launch_bundle_min(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->CE.p, Q, Qs, ctx->bmin.p,
-                          ctx->pack_costs ? ctx->bpack.p : nullptr, st);
+    HIPCHK(ctx, hipStreamWaitEvent(st, ctx->xev[3], 0));
     ctx->costs_packed = false;
-    if (ctx->pack_costs) HIPCHK(ctx, ctx->CEp.alloc((size_t)std::max<int64_t>(1, ctx->bs->brows * (Qs / 4) * 3)));
     // The prep's max over in-edges of -CE reads every cost row once, so it also writes the 24-bit codes of the
     // bundles that pack (one pass over CE instead of a separate packing pass).
     const bool tape = ctx->tape_K > 0;
@@ -897,6 +933,7 @@ static int enqueue_prep(ac_ctx* ctx) {
       ac::launch_argmin_out(ctx->de, Q, Qs, ctx->bs->csr_kb.p, ctx->CE.p, ctx->minCEo.p, ctx->AO.p, st);
     }
   }
+  HIPCHK(ctx, hipStreamWaitEvent(st, ctx->xev[1], 0));  // (again, for Q = 0) everything of the second stream
   if (ctx->tape_K > 0) {
     ac::launch_col_argmin(n, P, ctx->CV.p, ctx->apv.p, ctx->api.p, std::min(ctx->nb_col, 128), ctx->am0.p, st);
     int rc = checkpoint(ctx, 0, qs_prep);
