@@ -176,12 +176,14 @@ __global__ __launch_bounds__(256) void k_bwd_route(BwdGraph G, BwdState S, const
     }
 }
 
-// The fixed-point scale of the FMax edge terms' source sums: 2^(39 - e) for the largest |gE_F| in [2^e, 2^(e+1)), so
-// every term is below 2^40 in magnitude and a sum of up to 2^23 of them stays inside int64.  A power of two, so
-// scaling is exact and only the rounding to an integer (at 2^-40 of the largest term) loses anything.
-__device__ __forceinline__ int fix_exponent(unsigned gmax_bits) {
+// The fixed-point scale of the FMax edge terms' source sums: 2^(39 - shift - e) for the largest |gE_F| in
+// [2^e, 2^(e+1)), so every term is below 2^(40 - shift) in magnitude.  A source's sum has at most its out-degree of
+// terms, and shift (BwdGraph::fix_shift) leaves room for the graph's largest out-degree: up to 2^(23 + shift) terms
+// stay inside int64.  A power of two, so scaling is exact and only the rounding to an integer (at 2^-(40 - shift)
+// of the largest term) loses anything.
+__device__ __forceinline__ int fix_exponent(unsigned gmax_bits, int shift) {
   const int e = (int)((gmax_bits >> 23) & 0xFFu) - 127;  // subnormal maxima: e = -127, a larger scale than needed
-  return 39 - max(-126, min(e, 127));
+  return 39 - shift - max(-126, min(e, 127));
 }
 
 // The FMax edge term of (v, q), where it carries gradient gE = gEF[v][q] != 0: its recorded row FA[v][q] receives
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(256) void k_bwd_in(BwdGraph G, BwdState S, const do
                                                 float* __restrict__ gVF, double* __restrict__ cD, FmaxTerms FE) {
   const int Q = G.Q;
   const int64_t total = G.n * Q;
-  const double fscale = ldexp(1.0, fix_exponent(*FE.gmax));
+  const double fscale = ldexp(1.0, fix_exponent(*FE.gmax, G.fix_shift));
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = i / Q;
     const int q = (int)(i - v * Q);
@@ -271,7 +273,7 @@ __global__ __launch_bounds__(256) void k_bwd_vtx(BwdGraph G, float* __restrict__
   const int ninc = G.inc_off[P];
   for (int k = t; k <= P; k += 256) ioff[k] = G.inc_off[k];
   for (int k = t; k < ninc; k += 256) iinc[k] = G.inc[k];
-  const double inv = ldexp(1.0, -fix_exponent(*gmax));
+  const double inv = ldexp(1.0, -fix_exponent(*gmax, G.fix_shift));
   double col = 0.0;  // thread t < P: this block's partial sum of column t
   __syncthreads();
   for (int64_t v0 = (int64_t)blockIdx.x * T; v0 < G.n; v0 += (int64_t)gridDim.x * T) {

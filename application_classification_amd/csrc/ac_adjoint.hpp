@@ -20,6 +20,7 @@ struct BwdGraph {
   const int* pd;
   const int* inc_off;  // [P + 1]
   const int* inc;      // (q << 1) | (1 if q starts at p), ascending q
+  int fix_shift;       // headroom of the fixed-point source sums: the largest out-degree is at most 2^(23 + fix_shift)
 };
 
 // One state of the iteration: FMaxN / VFm rows, its scalars and the log-sum-exp of its MU columns.

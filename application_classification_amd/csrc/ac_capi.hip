@@ -180,6 +180,7 @@ struct ac_ctx {
   // Data graph (src/main.cu:8-19 / 49-62) plus its CSC and CSR views.
   bool has_data = false;
   int64_t dn = 0, dV = 0, de = 0, dE = 0;
+  int64_t max_outdeg = 0;  // the largest out-degree (sets the headroom of the adjoint's fixed-point sums)
   DevBuf<float> vfeat, efeat;
   DevBuf<int2> csc_sd, csr_ks;
   DevBuf<int> csc_eid;
@@ -434,9 +435,12 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   // same steps.
   std::vector<int> sord;
   std::vector<int64_t> sord_off(1, 0);
+  int64_t max_outdeg = 0;
   {
-    for (int64_t v = 0; v < nV; ++v)
+    for (int64_t v = 0; v < nV; ++v) {
       if (out_off[v + 1] > out_off[v]) sord.push_back((int)v);
+      max_outdeg = std::max(max_outdeg, out_off[v + 1] - out_off[v]);
+    }
     std::stable_sort(sord.begin(), sord.end(), [&](int a, int b) {
       return out_off[a + 1] - out_off[a] > out_off[b + 1] - out_off[b];
     });
@@ -486,6 +490,7 @@ static int set_data(ac_ctx* ctx, int64_t nV, int64_t dV, const float* vf, int64_
   ctx->nfix_multi = (int)fms.size();
   ctx->nfix_parts = nparts;
   ctx->dn = nV;
+  ctx->max_outdeg = max_outdeg;
   ctx->n_iso = n_iso;
   ctx->n_both = n_both;
   ctx->n_inonly = n_inonly;
@@ -1787,8 +1792,10 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, hipMemsetAsync(ctx->gVRfix.p, 0, nq * sizeof(unsigned long long), st));
   HIPCHK(ctx, hipMemsetAsync(ctx->gmax.p, 0, ((size_t)n_iter + 1) * sizeof(unsigned), st));
 
+  int fix_shift = 0;
+  while (((int64_t)1 << (23 + fix_shift)) < ctx->max_outdeg) ++fix_shift;
   ac::BwdGraph G{n, P, Q, Qs, ctx->CV.p, ctx->minCEo.p, ctx->lsO.p, ctx->d_ps.p, ctx->d_pd.p, ctx->d_inc_off.p,
-                 ctx->d_inc.p};
+                 ctx->d_inc.p, fix_shift};
   HIPCHK(ctx, ctx->vpart.alloc((size_t)ac::bwd_vtx_blocks(G) * std::max(1, P)));
   // Segment states: state 0 is the checkpoint itself, states 1..K live in the segment buffers, whose constant rows
   // (-inf) are set once here and never written afterwards.
