@@ -740,6 +740,16 @@ def main():
     # can be slowed by what ran just before it); --config batched / diff time them with --steps / --warmup.
     batched = fwd_bwd = None
     if not args.no_extras:
+        # Forward+backward first: its forward pass is host-heavy (64 small runs), and right after the batched
+        # measurement the release of that workload's host arrays slowed it by about a third.
+        import gc
+        gc.collect()
+        log("forward+backward: 64 patterns on R-MAT 18 ...")
+        f = measure_diff(3, 1)
+        fwd_bwd = {"ratio": f["ratio"], "forward_backward_ms": f["forward_backward_ms"], "forward_ms": f["forward_ms"],
+                   "patterns": f["config"]["patterns"], "checkpoint_every": 4, "grads_finite": f["checks"]["all_finite"]}
+        log(f"forward+backward: {json.dumps(fwd_bwd)}")
+        gc.collect()
         log("batched: 256 uneven patterns on R-MAT 20 ...")
         b = measure_batched(1, 1)
         batched = {"vs_sequential_plain": b["vs_sequential_plain"],
@@ -749,11 +759,6 @@ def main():
                    "max_err_over_tol_vs_sequential": b["checks"]["max_err_over_tol_vs_sequential"],
                    "sampled_patterns": b["checks"]["sampled_patterns"]}
         log(f"batched: {json.dumps(batched)}")
-        log("forward+backward: 64 patterns on R-MAT 18 ...")
-        f = measure_diff(3, 1)
-        fwd_bwd = {"ratio": f["ratio"], "forward_backward_ms": f["forward_backward_ms"], "forward_ms": f["forward_ms"],
-                   "patterns": f["config"]["patterns"], "checkpoint_every": 4, "grads_finite": f["checks"]["all_finite"]}
-        log(f"forward+backward: {json.dumps(fwd_bwd)}")
 
     cpu = None
     if not args.no_cpu_baseline:
