@@ -155,7 +155,8 @@ int ac_get_timing(const ac_ctx* ctx, double* prep_ms, double* loop_ms);
 /* No counterpart in the reference: {edge bundles, bundles whose costs the last run's walk read as 24-bit codes}
  * (AC_OPT_PACK_COSTS); synchronises the context's stream. */
 int ac_get_cost_packing(ac_ctx* ctx, int64_t* out2);
-/* Device stream the context runs on (a hipStream_t). */
+/* Device stream the context runs on (a hipStream_t).  The prep forks part of its work onto a second stream of the
+ * context and joins it back with events, so work ordered after a run on this stream sees all of the run. */
 void* ac_stream(ac_ctx* ctx);
 
 /* Stages of one run, timed separately by ac_run_profiled (they replace the reference's NVTX ranges "prep", "loop",
