@@ -577,7 +577,8 @@ def main():
     ap.add_argument("--out", metavar="DIR", help="--summarize-batched: write the summary here instead of profiles/")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (scales rocprofv3 FETCH_SIZE / "
-                         "WRITE_SIZE to bytes) and exit")
+                         "WRITE_SIZE to bytes), then time ten more (the copy's bandwidth, printed as one JSON line) "
+                         "and exit")
     args = ap.parse_args()
     if args.summarize_profile:
         from application_classification_amd import workloads as W
@@ -602,6 +603,19 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(f"copied 3 x {n * 4} bytes", flush=True)
+        # A plain device copy's bandwidth (read + write), HIP events around ten launches on torch's current stream
+        # (where they run).
+        def timed(fn, reps=10):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            fn()
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps / 1e3
+        t_copy = timed(lambda: b.copy_(a))
+        print(json.dumps({"copy_GBps": 2 * n * 4 / t_copy / 1e9, "bytes": n * 4}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the matcher is a single-GPU path here (multi-GPU sharding is a design "
