@@ -197,9 +197,10 @@ def summarize_batched_profile(src, out_dir):
                 for r in table(sub, "*_kernel_stats.csv")}
 
     def copy_scale(sub, counter):
-        c = counter_total(sub, counter)
-        k = next((k for k in c if "copyBuffer" in k), None)
-        return 3 * (1 << 30) / c[k] if k else None  # three 1 GiB copies
+        # Per copy launch (--copy-calibration runs several 1 GiB copies; the raw table has a row per launch).
+        vals = [float(r["Counter_Value"]) * 1024.0 for r in table(sub, "*_counter_collection.csv")
+                if r.get("Counter_Name") == counter and "copyBuffer" in r["Kernel_Name"]]
+        return (1 << 30) / (sum(vals) / len(vals)) if vals else None
 
     fs, ws = copy_scale("cfetch", "FETCH_SIZE"), copy_scale("cwrite", "WRITE_SIZE")
     out = {"workload": "rmat20-d32-256-uneven-patterns", "fetch_scale": fs, "write_scale": ws, "phases": {}}
