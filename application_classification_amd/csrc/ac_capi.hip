@@ -887,10 +887,15 @@ static int enqueue_prep(ac_ctx* ctx) {
                         ctx->api.p, std::min(ctx->nb_col, 256), ctx->iso.p, ax);
   HIPCHK(ctx, hipMemcpyAsync(ctx->pstat.p, ctx->lse_cv.p, sizeof(Lse2) * P, hipMemcpyDeviceToDevice, ax));
   if (Q > 0) {
-    ac::launch_fill(ctx->minCEo.p, n * Qs, INFINITY, ax);
-    ac::launch_fill(ctx->lsO.p, n * Qs, 0.f, ax);
-    ac::launch_fill(ctx->VFm.p, n * Qs, -INFINITY, ax);
-    ac::launch_fill(ctx->FMaxN.p, n * Qs, -INFINITY, ax);
+    // Only the constant rows: the per-source statistics write every row of a vertex with out-edges, and the prep
+    // walk every row of a vertex with in-edges (internal order: both, in-edges only, out-edges only, isolated).
+    const int64_t in_lo = ctx->n_both, in_hi = ctx->n_both + ctx->n_inonly, iso = ctx->n_act;
+    for (const auto& r : {std::make_pair(in_lo, in_hi), std::make_pair(iso, n)}) {
+      ac::launch_fill(ctx->minCEo.p + r.first * Qs, (r.second - r.first) * Qs, INFINITY, ax);
+      ac::launch_fill(ctx->lsO.p + r.first * Qs, (r.second - r.first) * Qs, 0.f, ax);
+      ac::launch_fill(ctx->VFm.p + r.first * Qs, (r.second - r.first) * Qs, -INFINITY, ax);
+    }
+    ac::launch_fill(ctx->FMaxN.p + in_hi * Qs, (n - in_hi) * Qs, -INFINITY, ax);
   }
   HIPCHK(ctx, hipEventRecord(ctx->xev[1], ax));
   if (Q > 0) {
