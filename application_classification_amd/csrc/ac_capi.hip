@@ -256,10 +256,14 @@ struct ac_ctx {
   int last_iters = 0;
 
   // Adjoint tape: with tape_K > 0 a run records the argmax rows / vertices its max stages used and checkpoints the
-  // iteration state every tape_K iterations; ac_backward recomputes each segment from its checkpoint.
+  // iteration state every tape_K iterations; ac_backward recomputes each segment from its checkpoint.  With
+  // tape_K = 1 every state is a checkpoint and the run's walks also record their argmax rows (tapeFA, one [dn][Qs]
+  // slot per iteration), so ac_backward recomputes nothing: the state after the last iteration is the live one, with
+  // the scalars at tape_final_qs.
   int tape_K = 0;
   bool tape_valid = false;
   int tape_iters = 0;
+  const QStat* tape_final_qs = nullptr;
   DevBuf<int> FA0, AO, am0, carg_in, tapeFA;
   DevBuf<float> ckF, ckV, sgF, sgV;
   DevBuf<QStat> ckQ, sgQ;
@@ -842,6 +846,7 @@ static int ensure_tape(ac_ctx* ctx, int n_iter) {
   HIPCHK(ctx, ctx->ckP.alloc(nck * P));
   HIPCHK(ctx, ctx->apv.alloc((size_t)ctx->nb_col * P));
   HIPCHK(ctx, ctx->api.alloc((size_t)ctx->nb_col * P));
+  if (K == 1) HIPCHK(ctx, ctx->tapeFA.alloc((size_t)std::max(1, n_iter) * nq));
   return AC_OK;
 }
 
@@ -989,8 +994,10 @@ static ac::VertexArgs vertex_args(ac_ctx* ctx) {
 }
 
 // One iteration (src/main.cu:160-245) reading the scalars of state `qs_state`, writing those of the new state into
-// the parity slot of `it`.  Kernels only (no copies or fills), so it can be captured into a graph.
-static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QStat*& qs_state) {
+// the parity slot of `it`.  Kernels only (no copies or fills), so it can be captured into a graph.  FA (nullable):
+// the walk also records its argmax rows there (the split vertices' pieces are then combined with theirs); the maxima
+// are the same as a plain walk's, bit for bit.
+static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QStat*& qs_state, int* FA = nullptr) {
   hipStream_t st = ctx->stream;
   const int P = ctx->P, Q = ctx->Q, Qs = ctx->Qs;
   QStat* qs_new = ctx->qstat.p + (1 + (it & 1)) * std::max(1, Q);
@@ -1008,10 +1015,14 @@ static int enqueue_iteration(ac_ctx* ctx, ac::VertexArgs& va, int it, const QSta
     Range r("step2+step4b+step4c");  // FE messages, normalised, scatter-max by destination
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, qs_new, ctx->FMaxN.p, ctx->carry_in.p,
-                        walk_batch_of(ctx), st, nullptr, nullptr, ctx->prune ? ctx->bmin.p : nullptr,
+                        walk_batch_of(ctx), st, FA, FA ? ctx->carg_in.p : nullptr, ctx->prune ? ctx->bmin.p : nullptr,
                         ctx->costs_packed ? ctx->CEp.p : nullptr, ctx->costs_packed ? ctx->bpack.p : nullptr);
     MARK(AC_STAGE_EDGE_WALK);
-    launch_fixup32(ctx, st);
+    if (FA)
+      ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, qs_new, ctx->FMaxN.p, st,
+                                ctx->carg_in.p, FA);
+    else
+      launch_fixup32(ctx, st);
     MARK(AC_STAGE_WALK_FIXUP);
     qs_state = qs_new;
   }
@@ -1127,7 +1138,8 @@ static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& sn
       qs_state = ctx->Q == 0 ? ctx->qstat.p : ctx->qstat.p + (1 + ((it + 1) & 1)) * qstride;
       it += 2;
     } else {
-      rc = enqueue_iteration(ctx, va, it, qs_state);
+      int* fa = ctx->tape_K == 1 ? ctx->tapeFA.p + (size_t)it * ctx->dn * ctx->Qs : nullptr;
+      rc = enqueue_iteration(ctx, va, it, qs_state, fa);
       if (rc != AC_OK) break;
       ++it;
       if (ctx->tape_K > 0 && it % ctx->tape_K == 0 && it < n_iter) {
@@ -1145,6 +1157,7 @@ static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& sn
     HIPCHK(ctx, es);
   }
   if (rc != AC_OK) return rc;
+  ctx->tape_final_qs = qs_state;
   if (snaps.empty()) {
     rc = enqueue_output(ctx, va, n_iter, qs_state);
     if (rc != AC_OK) return rc;
@@ -1785,7 +1798,9 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, ctx->sgV.alloc((size_t)(K + 1) * nq));
   HIPCHK(ctx, ctx->sgQ.alloc((size_t)(K + 1) * Q1));
   HIPCHK(ctx, ctx->sgP.alloc((size_t)(K + 1) * P));
-  HIPCHK(ctx, ctx->tapeFA.alloc((size_t)K * nq));
+  // With a checkpoint at every iteration the run recorded every walk's argmax rows, and nothing is recomputed.
+  const bool tape_full = K == 1;
+  if (!tape_full) HIPCHK(ctx, ctx->tapeFA.alloc((size_t)K * nq));
   HIPCHK(ctx, hipMemsetAsync(ctx->gCV.p, 0, np * sizeof(float), st));
   // gVF / gVR of the state after the last one: zero (k_bwd_route of the first step back reads their negations).
   HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
@@ -1810,18 +1825,26 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   bool have_mu = false;  // MUd holds MU of the output state of the next iteration to reverse
   for (int c = nck - 1; c >= 0; --c) {
     const int i0 = c * K, i1 = std::min(i0 + K, n_iter);
-    auto state = [&](int s) {
-      return s == 0 ? StateBufs{ctx->ckF.p + c * nq, ctx->ckV.p + c * nq, ctx->ckQ.p + (size_t)c * Q1,
-                                ctx->ckP.p + (size_t)c * P}
-                    : StateBufs{ctx->sgF.p + s * nq, ctx->sgV.p + s * nq, ctx->sgQ.p + (size_t)s * Q1,
-                                ctx->sgP.p + (size_t)s * P};
+    auto checkpoint_state = [&](int cc) {
+      return StateBufs{ctx->ckF.p + cc * nq, ctx->ckV.p + cc * nq, ctx->ckQ.p + (size_t)cc * Q1,
+                       ctx->ckP.p + (size_t)cc * P};
     };
+    auto state = [&](int s) {
+      if (s == 0) return checkpoint_state(c);
+      if (tape_full)  // the next checkpoint, or the live state after the last iteration (read only)
+        return c + 1 < nck ? checkpoint_state(c + 1)
+                           : StateBufs{ctx->FMaxN.p, ctx->VFm.p, const_cast<QStat*>(ctx->tape_final_qs), ctx->pstat.p};
+      return StateBufs{ctx->sgF.p + s * nq, ctx->sgV.p + s * nq, ctx->sgQ.p + (size_t)s * Q1, ctx->sgP.p + (size_t)s * P};
+    };
+    // Argmax rows of iteration i's walk: recorded by the run (full tape) or by the recompute below.
+    auto tape_of = [&](int i) { return ctx->tapeFA.p + (size_t)(tape_full ? i : i - i0) * nq; };
     // Recompute the segment from its checkpoint, keeping every state and the walk's argmax rows.
-    for (int i = i0; i < i1; ++i) {
-      const int s = i - i0;
-      const int rc = recompute_iteration(ctx, i, state(s), state(s + 1), ctx->tapeFA.p + (size_t)s * nq);
-      if (rc != AC_OK) return rc;
-    }
+    if (!tape_full)
+      for (int i = i0; i < i1; ++i) {
+        const int s = i - i0;
+        const int rc = recompute_iteration(ctx, i, state(s), state(s + 1), tape_of(i));
+        if (rc != AC_OK) return rc;
+      }
     // Reverse through the segment.
     for (int i = i1 - 1; i >= i0; --i) {
       const size_t s = (size_t)(i - i0);
@@ -1842,7 +1865,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
       have_mu = true;
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVR.p, 0, nq * sizeof(float), st));
       if (Q == 0) HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
-      const ac::FmaxTerms FE{ctx->gEF.p, ctx->tapeFA.p + s * nq, ctx->bs->bsrc.p, brows, ctx->gCE.p, ctx->gVRfix.p,
+      const ac::FmaxTerms FE{ctx->gEF.p, tape_of(i), ctx->bs->bsrc.p, brows, ctx->gCE.p, ctx->gVRfix.p,
                              gmax};
       ac::launch_bwd_in(G, S0, ctx->MUd.p, S1.qs, ctx->red4.p, ctx->gER.p, ctx->gVR.p, ctx->gVF.p, ctx->cD.p, FE, st);
       ac::launch_bwd_vtx(G, ctx->gVR.p, ctx->gVF.p, ctx->gVRfix.p, gmax, ctx->gMU.p, ctx->vpart.p, st);

@@ -1309,7 +1309,9 @@ __global__ __launch_bounds__(256) void k_walk_max_fixup(int nsplit, const Split*
           aa = rarg[wave][gg * L4 + j][u];
         }
       const int q = 4 * j + u;
-      po[u] = q < Q ? (float)((acc - qs[q].mFE) - qs[q].lFE) : 0.f;
+      // Normalised as the walk normalises a vertex it finishes (one FP64 subtraction of mFE + lFE, rounded once),
+      // so a recording walk leaves the same FMaxN as a plain one, whose pieces are normalised that way.
+      po[u] = (float)(acc - (q < Q ? qs[q].mFE + qs[q].lFE : 0.0));
       oarg[u] = aa;
     }
     reinterpret_cast<float4*>(FMaxN)[(int64_t)S.v * L4 + j] = out;

@@ -306,7 +306,7 @@ def summarize_diff_profile(src, out_dir):
              "    python bench.py --summarize-diff <dir>", "",
              "| pass over 64 patterns | ms |", "|---|---|",
              f"| forward (ac_run, weighted cdist) | {d['forward_ms_per_step']:.1f} |",
-             f"| forward with tape (checkpoint every {c.get('checkpoint_every')}) + ac_backward of the planted "
+             f"| forward with tape ({'a checkpoint at every iteration, no recompute' if c.get('checkpoint_every') == 1 else 'checkpoint every ' + str(c.get('checkpoint_every'))}) + ac_backward of the planted "
              f"cross-entropy | {d['ms_per_step']:.1f} |", "",
              f"forward+backward / forward = **{d['value']:.2f}** (best of {d['steps']} steps after {d['warmup']} "
              f"warm-up). Checks: {d.get('checks')}.", ""]
@@ -336,9 +336,10 @@ def measured_traffic(workload, stage):
     return None
 
 
-def measure_diff(steps, warmup):
+def measure_diff(steps, warmup, K=1):
     """Differentiation: R-MAT scale 18 (32-dim) with 64 planted 8-16-vertex patterns; per pattern the forward
-    (prep + iterations + output) alone, then forward with the tape (checkpoint every 4 iterations) + ac_backward of
+    (prep + iterations + output) alone, then forward with the tape (a checkpoint every K iterations; with K = 1 the run
+    also records every walk's argmax rows, so ac_backward recomputes nothing) + ac_backward of
     the planted cross-entropy.  A step is one pass over the 64 patterns; the figure is the best step of each side."""
     from application_classification_amd import workloads as W
     from application_classification_amd.matcher import AC_OPT_TAPE, Matcher
@@ -370,7 +371,7 @@ def measure_diff(steps, warmup):
     times = {}
     for tape in (False, True):
         m = Matcher(0)
-        m.set_option(AC_OPT_TAPE, 4 if tape else 0)
+        m.set_option(AC_OPT_TAPE, K if tape else 0)
         m.set_data(g)
         m.set_weights(wv, we)
         for _ in range(warmup):
@@ -383,7 +384,7 @@ def measure_diff(steps, warmup):
         "forward_backward_ms": times[True],
         "forward_ms": times[False],
         "config": {"workload": "rmat18-d32-64-patterns-8to16", "data_vertices": g.num_nodes,
-                   "data_edges": g.num_edges, "patterns": len(pats), "checkpoint_every": 4,
+                   "data_edges": g.num_edges, "patterns": len(pats), "checkpoint_every": K,
                    "updates_per_pass": upd, "parallelism": "single-gpu"},
         "checks": {"grad_wv_norm": float(np.linalg.norm(grads[0].sum(0))),
                    "grad_we_norm": float(np.linalg.norm(grads[1].sum(0))),
@@ -392,9 +393,10 @@ def measure_diff(steps, warmup):
 
 
 def bench_diff(args):
-    r = measure_diff(args.steps, args.warmup)
+    r = measure_diff(args.steps, args.warmup, args.checkpoint_every)
+    every = "every iteration" if args.checkpoint_every == 1 else f"every {args.checkpoint_every}"
     out = {
-        "metric": "forward+backward time / forward time (64 patterns, R-MAT 18, 32-dim, checkpoint every 4)",
+        "metric": f"forward+backward time / forward time (64 patterns, R-MAT 18, 32-dim, checkpoint {every})",
         "value": r["ratio"],
         "unit": "ratio",
         "n_gpus": 1,
@@ -576,6 +578,9 @@ def main():
                     help="write the adjoint profile (json and md) into profiles/ from DIR/diff.json and DIR/trace, "
                          "and exit")
     ap.add_argument("--out", metavar="DIR", help="--summarize-batched: write the summary here instead of profiles/")
+    ap.add_argument("--checkpoint-every", type=int, default=1,
+                    help="the adjoint measurement's checkpoint interval (AC_OPT_TAPE); 1 keeps every state and the "
+                         "walks' argmax rows, so the backward recomputes nothing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (scales rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE to bytes), then time ten more (the copy's bandwidth, printed as one JSON line) "
@@ -760,9 +765,10 @@ def main():
         import gc
         gc.collect()
         log("forward+backward: 64 patterns on R-MAT 18 ...")
-        f = measure_diff(3, 1)
+        f = measure_diff(3, 1, args.checkpoint_every)
         fwd_bwd = {"ratio": f["ratio"], "forward_backward_ms": f["forward_backward_ms"], "forward_ms": f["forward_ms"],
-                   "patterns": f["config"]["patterns"], "checkpoint_every": 4, "grads_finite": f["checks"]["all_finite"]}
+                   "patterns": f["config"]["patterns"], "checkpoint_every": args.checkpoint_every,
+                   "grads_finite": f["checks"]["all_finite"]}
         log(f"forward+backward: {json.dumps(fwd_bwd)}")
         gc.collect()
         log("batched: 256 uneven patterns on R-MAT 20 ...")

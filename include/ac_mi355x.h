@@ -181,7 +181,8 @@ int ac_run_profiled(ac_ctx* ctx, int n_iter, double* stage_ms);
 #define AC_OPT_WALK_BATCH 1  /* edge-walk steps per software-pipelined batch: 1, 2 or 4 (0 = default, 4)         */
 #define AC_OPT_GRAPH 2       /* 1: replay iteration pairs from a captured HIP graph (fewer launches); 0: launch each */
 #define AC_OPT_SLAB_EDGES 3  /* batched runs: most pattern edges packed into one slab, 4..256 (default 128)         */
-#define AC_OPT_TAPE 4        /* > 0: runs record what ac_backward needs, checkpointing every this many iterations  */
+#define AC_OPT_TAPE 4        /* > 0: runs record what ac_backward needs, checkpointing every this many iterations;
+                              * 1: every state, and every walk's argmax rows too, so ac_backward recomputes nothing */
 #define AC_OPT_PRUNE 5       /* 1 (default): the edge walk skips bundles of edges whose costs are all at or above a
                               * per-pattern-edge bound past which no edge term can exceed the fill value, so the
                               * result is bit-identical; 0: every edge is walked                                  */
@@ -202,7 +203,8 @@ int ac_get_option(const ac_ctx* ctx, int option, int* value);
  * Adjoint (no counterpart in the reference): per-dimension cdist weights, d = sqrt(sum_k w_k (a_k - b_k)^2), and
  * the gradient of a loss on the output MU with respect to them, by reverse mode through the prep and every
  * iteration (max subgradients routed through the argmax rows / vertices recorded in the forward pass; the
- * log-softmax Jacobians in FP32), recomputing each segment of AC_OPT_TAPE iterations from its checkpoint.
+ * log-softmax Jacobians in FP32), recomputing each segment of AC_OPT_TAPE iterations from its checkpoint (with
+ * AC_OPT_TAPE = 1 the run has recorded every state and nothing is recomputed).
  * ------------------------------------------------------------------------------------------------------------- */
 /* Weights of the vertex (dV) and edge (dE) feature distances, host arrays, >= 0; NULL = unweighted (the
  * reference's cdist, src/ac.hxx:33-56).  Set after the data graph. */

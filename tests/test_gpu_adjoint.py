@@ -197,6 +197,31 @@ def test_backward_is_reproducible():
         np.testing.assert_array_equal(r, res[0])
 
 
+def test_full_tape_equals_recomputed_segments():
+    """With a checkpoint at every iteration (AC_OPT_TAPE = 1) the run's own walks record their argmax rows and
+    ac_backward recomputes nothing; with longer intervals it recomputes each segment from its checkpoint.  The
+    recording walk leaves the same maxima as the plain one (split hubs included: R-MAT 12, in-lists of more than
+    128 edges), so the two tapes are the same and so are the forward output and both weight gradients, bit for bit."""
+    w = W.rmat(12, 16, 10, 14, seed=8)
+    d, p = w.data, w.pattern
+    assert np.bincount(d.dst, minlength=d.num_nodes).max() > 128  # split in-edge lists
+    wv, we = weights(9, 16, 16)
+    gm = loss_grad_mu(d.num_nodes, p.vmap)
+    res = {}
+    for K in (1, 4):
+        with Matcher(0) as m:
+            m.set_option(AC_OPT_TAPE, K)
+            m.set_data(d)
+            m.set_weights(wv, we)
+            m.set_pattern(p)
+            mu = m.run()
+            res[K] = (mu, np.concatenate(m.backward(gm)))
+    say(f"full tape vs K = 4: MU bit-identical = {np.array_equal(res[1][0], res[4][0])}, "
+        f"gradients bit-identical = {np.array_equal(res[1][1], res[4][1])}")
+    np.testing.assert_array_equal(res[1][0], res[4][0])
+    np.testing.assert_array_equal(res[1][1], res[4][1])
+
+
 def test_backward_scales_exactly_with_the_output_gradient():
     """The adjoint is linear in the output gradient, and every scale inside it follows the gradients: the fixed-point
     scale of the source sums is a power of two set by the largest edge-term gradient.  Multiplying grad_mu by 2^k
