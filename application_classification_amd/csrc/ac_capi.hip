@@ -257,13 +257,11 @@ struct ac_ctx {
 
   // Adjoint tape: with tape_K > 0 a run records the argmax rows / vertices its max stages used and checkpoints the
   // iteration state every tape_K iterations; ac_backward recomputes each segment from its checkpoint.  With
-  // tape_K = 1 every state is a checkpoint and the run's walks also record their argmax rows (tapeFA, one [dn][Qs]
-  // slot per iteration), so ac_backward recomputes nothing: the state after the last iteration is the live one, with
-  // the scalars at tape_final_qs.
+  // tape_K = 1 every state is a checkpoint (the run writes each into its slot, the last one included) and the run's
+  // walks also record their argmax rows (tapeFA, one [dn][Qs] slot per iteration), so ac_backward recomputes nothing.
   int tape_K = 0;
   bool tape_valid = false;
   int tape_iters = 0;
-  const QStat* tape_final_qs = nullptr;
   DevBuf<int> FA0, AO, am0, carg_in, tapeFA;
   DevBuf<float> ckF, ckV, sgF, sgV;
   DevBuf<QStat> ckQ, sgQ;
@@ -834,7 +832,8 @@ static int checkpoint(ac_ctx* ctx, int c, const QStat* qs) {
 // argmax records of the prep.
 static int ensure_tape(ac_ctx* ctx, int n_iter) {
   const int K = ctx->tape_K;
-  const size_t nck = (size_t)std::max(1, (n_iter + K - 1) / K);
+  // With K = 1 one more slot: the state after the last iteration (the run writes every state into its slot).
+  const size_t nck = (size_t)std::max(1, (n_iter + K - 1) / K) + (K == 1 ? 1 : 0);
   const size_t nq = (size_t)ctx->dn * ctx->Qs, Q = (size_t)std::max(1, ctx->Q), P = (size_t)ctx->P;
   HIPCHK(ctx, ctx->FA0.alloc(nq));
   HIPCHK(ctx, ctx->AO.alloc(nq));
@@ -1138,8 +1137,7 @@ static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& sn
       qs_state = ctx->Q == 0 ? ctx->qstat.p : ctx->qstat.p + (1 + ((it + 1) & 1)) * qstride;
       it += 2;
     } else {
-      int* fa = ctx->tape_K == 1 ? ctx->tapeFA.p + (size_t)it * ctx->dn * ctx->Qs : nullptr;
-      rc = enqueue_iteration(ctx, va, it, qs_state, fa);
+      rc = enqueue_iteration(ctx, va, it, qs_state);
       if (rc != AC_OK) break;
       ++it;
       if (ctx->tape_K > 0 && it % ctx->tape_K == 0 && it < n_iter) {
@@ -1157,7 +1155,6 @@ static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& sn
     HIPCHK(ctx, es);
   }
   if (rc != AC_OK) return rc;
-  ctx->tape_final_qs = qs_state;
   if (snaps.empty()) {
     rc = enqueue_output(ctx, va, n_iter, qs_state);
     if (rc != AC_OK) return rc;
@@ -1167,6 +1164,8 @@ static int enqueue_loop(ac_ctx* ctx, int n_iter, const std::vector<Snapshot>& sn
   MARK(AC_STAGE_OUTPUT);
   return AC_OK;
 }
+
+static int enqueue_taped_loop(ac_ctx* ctx, int n_iter);
 
 static int enqueue_run(ac_ctx* ctx, int n_iter) {
   int rc = prepare_run(ctx);
@@ -1179,7 +1178,9 @@ static int enqueue_run(ac_ctx* ctx, int n_iter) {
   }
   ctx->pev_used = 0;
   rc = enqueue_prep(ctx);
-  if (rc == AC_OK) rc = enqueue_loop(ctx, n_iter, {}, ctx->use_graph && !ctx->profiling && ctx->tape_K == 0);
+  if (rc == AC_OK)
+    rc = ctx->tape_K == 1 ? enqueue_taped_loop(ctx, n_iter)
+                          : enqueue_loop(ctx, n_iter, {}, ctx->use_graph && !ctx->profiling && ctx->tape_K == 0);
   if (rc != AC_OK) return rc;
   ctx->have_result = true;
   ctx->last_iters = n_iter;
@@ -1736,6 +1737,39 @@ static int recompute_iteration(ac_ctx* ctx, int it, const StateBufs& In, const S
   return AC_OK;
 }
 
+// The iterations of a run with a checkpoint at every iteration (AC_OPT_TAPE = 1): iteration it reads the state in
+// slot it and writes the next state into slot it + 1, its walk recording its argmax rows into tape slot it (the
+// maxima are the same as a plain walk's, bit for bit), so no state is copied and ac_backward recomputes nothing.
+// Slot 0 holds the prep state (enqueue_prep); the constant rows of the other slots (FMaxN without in-edges, VFm
+// without out-edges) are -inf, written once here.  The output pass reads the last slot.
+static int enqueue_taped_loop(ac_ctx* ctx, int n_iter) {
+  hipStream_t st = ctx->stream;
+  const size_t nq = (size_t)ctx->dn * ctx->Qs, Q1 = (size_t)std::max(1, ctx->Q), P = (size_t)ctx->P;
+  auto slot = [&](int c) {
+    return StateBufs{ctx->ckF.p + c * nq, ctx->ckV.p + c * nq, ctx->ckQ.p + c * Q1, ctx->ckP.p + c * P};
+  };
+  if (n_iter > 0) {
+    ac::launch_fill(ctx->ckF.p + nq, (int64_t)n_iter * nq, -INFINITY, st);
+    ac::launch_fill(ctx->ckV.p + nq, (int64_t)n_iter * nq, -INFINITY, st);
+  }
+  for (int it = 0; it < n_iter; ++it) {
+    const int rc = recompute_iteration(ctx, it, slot(it), slot(it + 1), ctx->tapeFA.p + (size_t)it * nq);
+    if (rc != AC_OK) return rc;
+  }
+  const StateBufs S = slot(n_iter);
+  ac::VertexArgs va = vertex_args(ctx);
+  va.FMaxN = S.F;
+  va.VFm = S.V;
+  va.VFm_out = S.V;
+  va.pst = S.pst;
+  const int rc = enqueue_output(ctx, va, n_iter, S.qs);
+  if (rc != AC_OK) return rc;
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
+  MARK(AC_STAGE_OUTPUT);
+  return AC_OK;
+}
+
 // MU (FP64, internal vertex order) of a recorded state into MUd, by the vertex pass's own evaluation.
 static void launch_mu(ac_ctx* ctx, const ac::BwdState& S, hipStream_t st) {
   ac::VertexArgs va = vertex_args(ctx);
@@ -1831,9 +1865,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
     };
     auto state = [&](int s) {
       if (s == 0) return checkpoint_state(c);
-      if (tape_full)  // the next checkpoint, or the live state after the last iteration (read only)
-        return c + 1 < nck ? checkpoint_state(c + 1)
-                           : StateBufs{ctx->FMaxN.p, ctx->VFm.p, const_cast<QStat*>(ctx->tape_final_qs), ctx->pstat.p};
+      if (tape_full) return checkpoint_state(c + 1);  // the run wrote every state into its slot
       return StateBufs{ctx->sgF.p + s * nq, ctx->sgV.p + s * nq, ctx->sgQ.p + (size_t)s * Q1, ctx->sgP.p + (size_t)s * P};
     };
     // Argmax rows of iteration i's walk: recorded by the run (full tape) or by the recompute below.
