@@ -171,6 +171,65 @@ def summarize_profile(src, workload, out_dir):
     print("\n".join(lines))
 
 
+def summarize_counters(src, workload, out_dir):
+    """Writes profiles/<workload>_counters.md from rocprofv3 counter passes under src: req/ and creq/ (L2 read requests
+    to memory by size, of the bench and of --copy-calibration), hit/ (L2 hits and misses), sqa/ and sqb/ (wave
+    counters).  Figures are averages per dispatch."""
+    import collections
+    import csv
+    import glob
+
+    def load(sub):
+        agg = collections.defaultdict(lambda: collections.defaultdict(list))
+        for f in glob.glob(os.path.join(src, sub, "**", "*_counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                agg[_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in agg.items()}
+
+    req, creq, hit, sqa, sqb = (load(d) for d in ("req", "creq", "hit", "sqa", "sqb"))
+    cmd = "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-profile --no-pruned --no-extras"
+    passes = [("req", "TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum TCC_EA0_RDREQ_64B_sum TCC_EA0_RDREQ_128B_sum", cmd),
+              ("creq", "TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum TCC_EA0_RDREQ_64B_sum TCC_EA0_RDREQ_128B_sum",
+               "python -u bench.py --copy-calibration"),
+              ("hit", "TCC_HIT_sum TCC_MISS_sum", cmd),
+              ("sqa", "SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_INSTS_VALU "
+                      "SQ_INSTS_SALU SQ_ACTIVE_INST_VALU", cmd),
+              ("sqb", "SQ_INSTS_VMEM SQ_INST_LEVEL_VMEM SQ_ACTIVE_INST_ANY SQ_INSTS_BRANCH", cmd)]
+    lines = [f"# Counters: {workload}", "", "Commands (run from the repository root on one MI355X, one pass each):", ""]
+    lines += [f"    rocprofv3 --pmc {c} --output-format csv -d <dir>/{d} -o c -- {x}" for d, c, x in passes]
+    lines += [f"    python bench.py --summarize-counters <dir>", "",
+              "Per dispatch, averaged over the dispatches of each kernel. `lines` are L2 read requests to memory "
+              "(HBM or the MALL) by size; `read GB` = 128 B x lines128 + 64 B x lines64 + 32 B x lines32. "
+              "`L2 hit` = TCC_HIT / (TCC_HIT + TCC_MISS). `dep. wait` = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES, the "
+              "share of wave time spent waiting for a dependency (an outstanding load, mostly). `VALU/wave`, "
+              "`VMEM/wave` = instructions per wave.", "",
+              "| kernel | lines128 | lines64 | lines32 | read GB | L2 hit | waves | dep. wait | VALU/wave | VMEM/wave |",
+              "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+
+    def row(k, r, h, a, b):
+        l128, l64, l32 = (r.get(f"TCC_EA0_RDREQ_{s}_sum", float("nan")) for s in ("128B", "64B", "32B"))
+        gb = (128 * l128 + 64 * l64 + 32 * l32) / 1e9
+        ht = h.get("TCC_HIT_sum"), h.get("TCC_MISS_sum")
+        hr = f"{ht[0] / (ht[0] + ht[1]):.2f}" if None not in ht and ht[0] + ht[1] > 0 else "-"
+        w = a.get("SQ_WAVES")
+        wait = f"{a['SQ_WAIT_INST_ANY'] / a['SQ_WAVE_CYCLES']:.2f}" if a.get("SQ_WAVE_CYCLES") else "-"
+        valu = f"{a['SQ_INSTS_VALU'] / w:.0f}" if w else "-"
+        vmem = f"{b['SQ_INSTS_VMEM'] / w:.0f}" if w and "SQ_INSTS_VMEM" in b else "-"
+        return (f"| {k} | {l128:.4g} | {l64:.4g} | {l32:.4g} | {gb:.3f} | {hr} | {w:.4g} | {wait} | {valu} | {vmem} |"
+                if w else f"| {k} | {l128:.4g} | {l64:.4g} | {l32:.4g} | {gb:.3f} | {hr} | - | - | - | - |")
+
+    order = sorted(req, key=lambda k: -req[k].get("TCC_EA0_RDREQ_sum", 0.0))
+    for k in order[:12]:
+        lines.append(row(k, req[k], hit.get(k, {}), sqa.get(k, {}), sqb.get(k, {})))
+    for k in creq:
+        if "copyBuffer" in k:
+            lines.append(row(k + " (1 GiB copy)", creq[k], {}, {}, {}))
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, f"{workload}_counters.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def summarize_batched_profile(src, out_dir):
     """Per-pass kernel time and HBM traffic of the batched workload, sequential plain runs against batched runs, from
     rocprofv3 outputs under src: for each phase P in (seq, bat) and step count S in (1, 2), P<S>/trace (--kernel-trace
@@ -571,6 +630,8 @@ def main():
                          "headline launches only)")
     ap.add_argument("--summarize-profile", metavar="DIR",
                     help="summarise rocprofv3 outputs under DIR into profiles/ and exit")
+    ap.add_argument("--summarize-counters", metavar="DIR",
+                    help="summarise the counter passes (req, creq, hit, sqa, sqb) under DIR into profiles/ and exit")
     ap.add_argument("--summarize-batched", metavar="DIR",
                     help="summarise the batched workload's per-pass rocprofv3 outputs under DIR into profiles/ and "
                          "exit")
@@ -591,6 +652,10 @@ def main():
 
         name = {"s22": "rmat22-d64-p16x24", "s18": "rmat18-d32-p12x16", "small": "small-er1000"}[args.config]
         summarize_profile(args.summarize_profile, name, os.path.join(ROOT, "profiles"))
+        return
+    if args.summarize_counters:
+        name = {"s22": "rmat22-d64-p16x24", "s18": "rmat18-d32-p12x16", "small": "small-er1000"}[args.config]
+        summarize_counters(args.summarize_counters, name, args.out or os.path.join(ROOT, "profiles"))
         return
     if args.summarize_diff:
         summarize_diff_profile(args.summarize_diff, args.out or os.path.join(ROOT, "profiles"))
