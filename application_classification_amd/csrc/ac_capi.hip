@@ -1715,17 +1715,22 @@ static int recompute_iteration(ac_ctx* ctx, int it, const StateBufs& In, const S
     va.VFm_out = Out.V;
     va.pst = In.pst;
     ac::launch_vertex(ac::kModeA, va, ctx->nb_vertex, st);
+    MARK(AC_STAGE_VERTEX_A);
     ac::launch_reduce_q(ctx->nb_vertex + 1, Q, ctx->qpart.p, Out.qs, st);
+    MARK(AC_STAGE_REDUCE_A);
     ac::launch_walk_max(ctx->bs->nbundles, ctx->bs->bundles.p, ctx->bs->gslot.p, ctx->bs->bsrc.p, ctx->bs->bdst.p,
                         ctx->CE.p, ctx->VR.p, ctx->VRs, Q, Qs, Out.qs, Out.F, ctx->carry_in.p, walk_batch_of(ctx),
                         st, FA, ctx->carg_in.p, ctx->prune ? ctx->bmin.p : nullptr,
                         ctx->costs_packed ? ctx->CEp.p : nullptr, ctx->costs_packed ? ctx->bpack.p : nullptr);
+    MARK(AC_STAGE_EDGE_WALK);
     ac::launch_walk_max_fixup(ctx->nsplit_in, ctx->splits_in.p, ctx->carry_in.p, Q, Qs, Out.qs, Out.F, st,
                               ctx->carg_in.p, FA);
+    MARK(AC_STAGE_WALK_FIXUP);
   } else {
     HIPCHK(ctx, hipMemcpyAsync(Out.F, In.F, nq * sizeof(float), hipMemcpyDeviceToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(Out.V, In.V, nq * sizeof(float), hipMemcpyDeviceToDevice, st));
     HIPCHK(ctx, hipMemcpyAsync(Out.qs, In.qs, sizeof(QStat), hipMemcpyDeviceToDevice, st));
+    MARK(AC_STAGE_VERTEX_A);
   }
   va.first = 0;
   va.qs = Out.qs;
@@ -1733,7 +1738,9 @@ static int recompute_iteration(ac_ctx* ctx, int it, const StateBufs& In, const S
   va.VFm = Out.V;
   va.VFm_out = Out.V;
   ac::launch_vertex(ac::kModeC, va, ctx->nb_vertex_c, st);
+  MARK(AC_STAGE_VERTEX_C);
   ac::launch_reduce_lse(ctx->nb_vertex_c + 1, P, ctx->ppart.p, Out.pst, st);
+  MARK(AC_STAGE_REDUCE_C);
   return AC_OK;
 }
 
@@ -1743,6 +1750,7 @@ static int recompute_iteration(ac_ctx* ctx, int it, const StateBufs& In, const S
 // Slot 0 holds the prep state (enqueue_prep); the constant rows of the other slots (FMaxN without in-edges, VFm
 // without out-edges) are -inf, written once here.  The output pass reads the last slot.
 static int enqueue_taped_loop(ac_ctx* ctx, int n_iter) {
+  Range loop("loop");
   hipStream_t st = ctx->stream;
   const size_t nq = (size_t)ctx->dn * ctx->Qs, Q1 = (size_t)std::max(1, ctx->Q), P = (size_t)ctx->P;
   auto slot = [&](int c) {

@@ -145,6 +145,27 @@ def test_taped_forward_equals_plain_forward(small):
     np.testing.assert_array_equal(mu, Matcher_plain(d, p))
 
 
+def test_profiled_taped_run_reports_every_stage(small):
+    """ac_run_profiled of a run with a checkpoint at every iteration (whose iterations write straight into the
+    checkpoint slots) times each stage, as a plain profiled run does, and returns the same MU."""
+    d, p = small.data, small.pattern
+    st = {}
+    mus = []
+    for tape in (0, 1):
+        with Matcher(0) as m:
+            m.set_option(AC_OPT_TAPE, tape)
+            m.set_data(d)
+            m.set_pattern(p)
+            st[tape] = m.run_profiled()
+            mus.append(m.mu())
+            say(f"tape={tape}: {st[tape]}")
+    np.testing.assert_array_equal(mus[0], mus[1])
+    for s in ("vertex_a", "reduce_a", "edge_walk", "vertex_c", "reduce_c", "output"):
+        assert st[1][s] > 0.0, s
+    # The output stage is one pass, not the whole loop.
+    assert st[1]["output"] < st[1]["edge_walk"] + st[1]["vertex_a"] + st[1]["vertex_c"]
+
+
 def test_packed_costs_give_identical_gradients():
     """The taped forward and the recompute walks of ac_backward read the 24-bit cost codes (AC_OPT_PACK_COSTS) where
     bundles pack; the forward and both gradients equal those with FP32 costs exactly.  64-dim features, so most
