@@ -107,14 +107,16 @@ def Matcher_plain(d, p):
         return m.run()
 
 
-@pytest.mark.parametrize("n_iter", [0, 3, -1])
-def test_gradient_matches_finite_differences_small(small, n_iter):
+@pytest.mark.parametrize("n_iter,K", [(0, 4), (3, 4), (-1, 4), (0, 1), (3, 1)])
+def test_gradient_matches_finite_differences_small(small, n_iter, K):
+    """K = 1 keeps every state (the iterations write straight into their checkpoint slots; zero iterations read the
+    prep's slot), K = 4 recomputes segments from checkpoints."""
     d, p = small.data, small.pattern
     wv, we = weights(2, 13, 13)
     iters = p.num_nodes if n_iter < 0 else n_iter
-    _, gv, ge = gpu_grad(d, p, wv, we, iters)
+    _, gv, ge = gpu_grad(d, p, wv, we, iters, K=K)
     fv, fe = fd_grad(g64(d), g64(p), p.vmap, wv.copy(), we.copy(), iters)
-    check_grad(np.concatenate([gv, ge]), np.concatenate([fv, fe]), f"small, {iters} iterations")
+    check_grad(np.concatenate([gv, ge]), np.concatenate([fv, fe]), f"small, {iters} iterations, K = {K}")
 
 
 def test_gradient_matches_finite_differences_rmat_subgraph():
