@@ -1744,6 +1744,16 @@ static int recompute_iteration(ac_ctx* ctx, int it, const StateBufs& In, const S
   return AC_OK;
 }
 
+// The constant rows of nslots consecutive iteration states (FMaxN and VFm slots of dn x Qs floats): -inf in the
+// FMaxN rows of vertices without in-edges and in the VFm rows of vertices without out-edges, which no iteration
+// writes (internal order: both, in-edges only, out-edges only, isolated), as the prep sets them in the live state.
+static void fill_constant_rows(ac_ctx* ctx, float* F, float* V, int nslots, hipStream_t st) {
+  const int64_t Qs = ctx->Qs, nq = ctx->dn * Qs;
+  const int64_t in_lo = ctx->n_both, in_hi = ctx->n_both + ctx->n_inonly, iso = ctx->n_act, n = ctx->dn;
+  ac::launch_fill_slots(F, V, nslots, nq, in_hi * Qs, (n - in_hi) * Qs, in_lo * Qs, (in_hi - in_lo) * Qs, iso * Qs,
+                        (n - iso) * Qs, st);
+}
+
 // The iterations of a run with a checkpoint at every iteration (AC_OPT_TAPE = 1): iteration it reads the state in
 // slot it and writes the next state into slot it + 1, its walk recording its argmax rows into tape slot it (the
 // maxima are the same as a plain walk's, bit for bit), so no state is copied and ac_backward recomputes nothing.
@@ -1756,10 +1766,7 @@ static int enqueue_taped_loop(ac_ctx* ctx, int n_iter) {
   auto slot = [&](int c) {
     return StateBufs{ctx->ckF.p + c * nq, ctx->ckV.p + c * nq, ctx->ckQ.p + c * Q1, ctx->ckP.p + c * P};
   };
-  if (n_iter > 0) {
-    ac::launch_fill(ctx->ckF.p + nq, (int64_t)n_iter * nq, -INFINITY, st);
-    ac::launch_fill(ctx->ckV.p + nq, (int64_t)n_iter * nq, -INFINITY, st);
-  }
+  fill_constant_rows(ctx, ctx->ckF.p + nq, ctx->ckV.p + nq, n_iter, st);
   for (int it = 0; it < n_iter; ++it) {
     const int rc = recompute_iteration(ctx, it, slot(it), slot(it + 1), ctx->tapeFA.p + (size_t)it * nq);
     if (rc != AC_OK) return rc;
@@ -1836,13 +1843,16 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, ctx->bpart.alloc((size_t)nparts * std::max(4 * Q1, P)));
   HIPCHK(ctx, ctx->red4.alloc((size_t)4 * Q1));
   HIPCHK(ctx, ctx->cq.alloc((size_t)Q1));
-  HIPCHK(ctx, ctx->sgF.alloc((size_t)(K + 1) * nq));
-  HIPCHK(ctx, ctx->sgV.alloc((size_t)(K + 1) * nq));
-  HIPCHK(ctx, ctx->sgQ.alloc((size_t)(K + 1) * Q1));
-  HIPCHK(ctx, ctx->sgP.alloc((size_t)(K + 1) * P));
-  // With a checkpoint at every iteration the run recorded every walk's argmax rows, and nothing is recomputed.
+  // With a checkpoint at every iteration the run recorded every walk's argmax rows, and nothing is recomputed: no
+  // segment buffers.
   const bool tape_full = K == 1;
-  if (!tape_full) HIPCHK(ctx, ctx->tapeFA.alloc((size_t)K * nq));
+  if (!tape_full) {
+    HIPCHK(ctx, ctx->sgF.alloc((size_t)(K + 1) * nq));
+    HIPCHK(ctx, ctx->sgV.alloc((size_t)(K + 1) * nq));
+    HIPCHK(ctx, ctx->sgQ.alloc((size_t)(K + 1) * Q1));
+    HIPCHK(ctx, ctx->sgP.alloc((size_t)(K + 1) * P));
+    HIPCHK(ctx, ctx->tapeFA.alloc((size_t)K * nq));
+  }
   HIPCHK(ctx, hipMemsetAsync(ctx->gCV.p, 0, np * sizeof(float), st));
   // gVF / gVR of the state after the last one: zero (k_bwd_route of the first step back reads their negations).
   HIPCHK(ctx, hipMemsetAsync(ctx->gVF.p, 0, nq * sizeof(float), st));
@@ -1861,8 +1871,7 @@ extern "C" int ac_backward(ac_ctx* ctx, const float* grad_mu, float* grad_wv, fl
   HIPCHK(ctx, ctx->vpart.alloc((size_t)ac::bwd_vtx_blocks(G) * std::max(1, P)));
   // Segment states: state 0 is the checkpoint itself, states 1..K live in the segment buffers, whose constant rows
   // (-inf) are set once here and never written afterwards.
-  ac::launch_fill(ctx->sgF.p, (int64_t)(K + 1) * nq, -INFINITY, st);
-  ac::launch_fill(ctx->sgV.p, (int64_t)(K + 1) * nq, -INFINITY, st);
+  if (!tape_full) fill_constant_rows(ctx, ctx->sgF.p, ctx->sgV.p, K + 1, st);
   const int nck = n_iter == 0 ? 0 : (n_iter + K - 1) / K;
   bool have_mu = false;  // MUd holds MU of the output state of the next iteration to reverse
   for (int c = nck - 1; c >= 0; --c) {

@@ -124,6 +124,8 @@ void launch_bundle_min(int64_t nbundles, const Bundle* bundles, const float* CE,
 void launch_prep_scalars(int P, int Q, const int* ps, const int* pd, const Lse2* cv, const Lse2* ce, QStat* qs,
                          hipStream_t st);
 void launch_fill(float* p, int64_t count, float value, hipStream_t st);
+void launch_fill_slots(float* A, float* B, int64_t nslots, int64_t stride, int64_t a0, int64_t al, int64_t b0,
+                       int64_t bl, int64_t c0, int64_t cl, hipStream_t st);
 void launch_walk_src(int64_t nruns, const Run* runs, const int2* ks, const float* CE, int Q, int Qs, float* minCEo,
                      float* lsO, float* VFm, double* carry, Lse* gpart, int nblocks, hipStream_t st);
 void launch_walk_src_fixup(int nsplit, const Split* splits, const double* carry, int Q, int Qs, float* minCEo,

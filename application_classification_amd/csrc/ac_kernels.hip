@@ -690,6 +690,30 @@ void launch_fill(float* p, int64_t count, float value, hipStream_t st) {
   hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, p, count, value);
 }
 
+// -inf into three ranges of each of nslots slots (stride `stride` floats): [a0, a0 + al) of A, and [b0, b0 + bl) and
+// [c0, c0 + cl) of B.  One launch for every slot.
+__global__ void k_fill_slots(float* __restrict__ A, float* __restrict__ B, int64_t nslots, int64_t stride, int64_t a0,
+                             int64_t al, int64_t b0, int64_t bl, int64_t c0, int64_t cl) {
+  const int64_t per = al + bl + cl, total = nslots * per;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = i / per, r = i - s * per;
+    if (r < al)
+      A[s * stride + a0 + r] = -INFINITY;
+    else if (r < al + bl)
+      B[s * stride + b0 + (r - al)] = -INFINITY;
+    else
+      B[s * stride + c0 + (r - al - bl)] = -INFINITY;
+  }
+}
+
+void launch_fill_slots(float* A, float* B, int64_t nslots, int64_t stride, int64_t a0, int64_t al, int64_t b0,
+                       int64_t bl, int64_t c0, int64_t cl, hipStream_t st) {
+  const int64_t total = nslots * (al + bl + cl);
+  if (total == 0) return;
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 16);
+  hipLaunchKernelGGL(k_fill_slots, dim3(blocks), dim3(256), 0, st, A, B, nslots, stride, a0, al, b0, bl, c0, cl);
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Edge walks.  A group of Qs/4 lanes walks one run of consecutive edges; lane j owns pattern edges 4j..4j+3 and
 // reads them as one float4 of the edge's cost row (and of the gathered VR row).  Runs cover whole vertices except
