@@ -119,6 +119,21 @@ def test_gradient_matches_finite_differences_small(small, n_iter, K):
     check_grad(np.concatenate([gv, ge]), np.concatenate([fv, fe]), f"small, {iters} iterations, K = {K}")
 
 
+@pytest.mark.parametrize("K", [1, 3])
+def test_gradient_for_a_pattern_without_edges(small, K):
+    """A pattern with vertices but no edges: every iteration only renormalises the vertex costs (the taped run copies
+    its state from slot to slot).  The edge weights do not enter, so their gradient is exactly zero; the vertex
+    weights' gradient matches FP64 central differences of the oracle."""
+    d, p = small.data, small.pattern
+    patt = W.Pattern(p.vfeat, np.zeros((0, 13), np.float32), np.zeros(0, np.int32), np.zeros(0, np.int32),
+                     vmap=p.vmap)
+    wv, we = weights(4, 13, 13)
+    _, gv, ge = gpu_grad(d, patt, wv, we, 3, K=K)
+    assert np.all(ge == 0.0)
+    fv, _ = fd_grad(g64(d), g64(patt), patt.vmap, wv.copy(), we.copy(), 3)
+    check_grad(gv, fv, f"pattern without edges, K = {K}")
+
+
 def test_gradient_matches_finite_differences_rmat_subgraph():
     """R-MAT scale 11 (2,048 vertices: hubs, split in-edge lists, vertices without edges), 32-dim features,
     a planted 12-vertex / 16-edge pattern; all 64 weights."""
