@@ -134,6 +134,20 @@ def test_gradient_for_a_pattern_without_edges(small, K):
     check_grad(gv, fv, f"pattern without edges, K = {K}")
 
 
+@pytest.mark.parametrize("K", [1, 3])
+def test_gradient_for_data_without_edges(small, K):
+    """A data graph without edges (every vertex isolated, no edge rows): the maxima are the fills, the edge weights
+    do not enter (gradient exactly zero), and the vertex weights' gradient matches FP64 central differences of the
+    oracle."""
+    d, p = small.data, small.pattern
+    data = W.Graph(d.vfeat, np.zeros((0, 13), np.float32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    wv, we = weights(5, 13, 13)
+    _, gv, ge = gpu_grad(data, p, wv, we, 2, K=K)
+    assert np.all(ge == 0.0)
+    fv, _ = fd_grad(g64(data), g64(p), p.vmap, wv.copy(), we.copy(), 2)
+    check_grad(gv, fv, f"data without edges, K = {K}")
+
+
 def test_gradient_matches_finite_differences_rmat_subgraph():
     """R-MAT scale 11 (2,048 vertices: hubs, split in-edge lists, vertices without edges), 32-dim features,
     a planted 12-vertex / 16-edge pattern; all 64 weights."""
