@@ -72,6 +72,20 @@ def test_batch_matches_plain_runs(small, uneven, slab_edges, graph):
             assert_close(m.batch_mu(i), plain(d, p), f"pattern {i} ({p.num_nodes}/{p.num_edges})")
 
 
+def test_batch_with_a_pattern_without_edges(small, uneven):
+    """An edge-less pattern among others: it only adds vertices to its slab (its columns keep the vertex costs'
+    renormalisation), and every member matches its plain run."""
+    d = small.data
+    e0 = W.Graph(uneven[2].vfeat, np.zeros((0, 13), np.float32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    pats = [uneven[0], e0, uneven[1]]
+    with Matcher(0) as m:
+        m.set_data(d)
+        m.batch_set_patterns(pats)
+        m.batch_run()
+        for i, p in enumerate(pats):
+            assert_close(m.batch_mu(i), plain(d, p), f"pattern {i} ({p.num_nodes}/{p.num_edges})")
+
+
 def test_batch_against_oracle(small, uneven):
     d = small.data
     with Matcher(0) as m:
